@@ -1,0 +1,207 @@
+"""Benchmark: time to true residual 1e-10 on one MI355X, 24^3x48, and the
+hopping-kernel HBM bandwidth.
+
+Workload (single right-hand side, even-odd preconditioned): 24^3x48 lattice,
+gauge U = exp(i 0.3 H) reunitarised to SU(3) (seeded), kappa = 0.1632,
+mu = 0.004, antiperiodic time boundary, Z2 source.  A "step" is one complete
+solve D x = b from a device-resident source to FP64-verified true relative
+residual <= 1e-10.  The timed steps use the fastest solver (mixed precision);
+the plain FP64 CG (full lattice, D^dag D, the reference algorithm) and the
+other precisions are timed once each for comparison, and every solution's
+distance to an FP64 solution (solved to 1e-13) is reported.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-1e-10 resid (mixed/learned vs FP64 CG, 24^3x48); Dslash HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+L = (24, 24, 24, 48)
+EPS, KAPPA, MU = 0.3, 0.1632, 0.004
+TOL = 1e-10
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def cpu_baseline(U, src, fp64_iters):
+    """Reference CPU path on this host: per-iteration time of the reference CG
+    (oracle/_ref/ref_harness, the reference's own lib/ code, one MPI rank) over
+    a bounded sample of iterations, scaled by the iteration count the FP64 CG
+    needs for this system.  Falls back to the C restatement (oracle) if the
+    reference build cannot run here."""
+    import oracle as orc
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    sample_iters = 10
+    if os.path.exists(harness):
+        try:
+            import struct
+            with tempfile.TemporaryDirectory() as d:
+                fin = os.path.join(d, "in.bin")
+                with open(fin, "wb") as f:
+                    f.write(struct.pack("<4i", *L))
+                    f.write(struct.pack("<dd", KAPPA, MU))
+                    f.write(struct.pack("<ii", 1, -1))
+                    f.write(struct.pack("<d", TOL))
+                    f.write(struct.pack("<i", sample_iters))
+                    f.write(np.ascontiguousarray(U).tobytes())
+                    f.write(np.ascontiguousarray(src).tobytes())
+                    f.write(np.zeros_like(src).tobytes())
+                r = subprocess.run([harness, "time_cg", fin, str(sample_iters)], capture_output=True, text=True,
+                                   timeout=300, env=dict(os.environ, OMP_NUM_THREADS="1"))
+                per_iter = float(r.stderr.strip().splitlines()[-1])
+            return {"value": per_iter * fp64_iters, "unit": "s", "cores": 1, "kind": "reference",
+                    "sample": f"{sample_iters} iterations of the reference CG (lib/solvers.cpp) on D^dag D at 24^3x48,"
+                              f" one MPI rank, {per_iter:.3f} s/iteration, scaled to the {fp64_iters} iterations"
+                              f" the FP64 CG needs"}
+        except Exception as e:  # noqa: BLE001
+            print(f"reference harness unavailable ({e}); timing the oracle instead", file=sys.stderr)
+    Ub = orc.apply_bc(U, L, -1)
+    t = time.time()
+    n = 2
+    for _ in range(n):
+        orc.tm(src, Ub, L, KAPPA, MU, 1, orc.UKQCD, orc.TM_DAG_TM)
+    per_iter = (time.time() - t) / n * 1.1
+    return {"value": per_iter * fp64_iters, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"{n} applications of D^dag D (C restatement, 1 thread) at 24^3x48 scaled to {fp64_iters}"
+                      f" CG iterations (+10% for the BLAS)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import qkxtm_mpi_amd as q
+    q.init(0)
+
+    U = q.gen_gauge(L, EPS, 1)
+    src = q.gen_z2(L, 2)
+    u = q.Gauge(L, q.DOUBLE, q.RECON_18, boundaryT=-1, host=U)
+    p = q.params(L, KAPPA, MU, 1, -1)
+    b = q.Spinor(L, host=src)
+    x = q.Spinor(L)
+    V = int(np.prod(L))
+
+    # ---- FP64 reference solution (tight) -------------------------------------
+    q.solve(x, b, u, p, eo=True, inner_prec=q.DOUBLE, tol=1e-13, maxIter=20000)
+    xref = x.download()
+    nref = np.linalg.norm(xref)
+
+    variants = {
+        "fp64_cg_full": dict(eo=False, inner_prec=q.DOUBLE),
+        "fp64_cg_eo": dict(eo=True, inner_prec=q.DOUBLE),
+        "fp32_eo_rel": dict(eo=True, inner_prec=q.SINGLE),
+        "fp16_eo_rel": dict(eo=True, inner_prec=q.HALF),
+        "bf16_eo_rel": dict(eo=True, inner_prec=q.BF16),
+    }
+    results = {}
+    for name, kw in variants.items():
+        for _ in range(1):  # warm (first call builds the inner gauge / temporaries)
+            x.zero()
+            q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
+        q.sync()
+        t = time.perf_counter()
+        x.zero()
+        st = q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
+        q.sync()
+        dt = time.perf_counter() - t
+        xs = x.download()
+        results[name] = {"time_s": dt, "iterations": st.iterations, "reliable_updates": st.reliable_updates,
+                         "true_residual": st.true_residual,
+                         "err_vs_fp64": float(np.linalg.norm(xs - xref) / nref)}
+
+    best = min((k for k in results if k.startswith(("fp32", "fp16", "bf16"))), key=lambda k: results[k]["time_s"])
+    kw = variants[best]
+
+    # ---- timed region: K solves with the best solver --------------------------
+    for _ in range(args.warmup):
+        x.zero()
+        q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
+    q.sync()
+    q.prof_enable(True)
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        x.zero()
+        st = q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
+        q.sync()
+        times.append(time.perf_counter() - ts)
+    q.sync()
+    total = time.perf_counter() - t0
+    launches, hop_ms, hop_bytes = q.prof_read()
+    q.prof_enable(False)
+    if st.true_residual > TOL:
+        raise SystemExit(f"timed solve missed the residual target: {st.true_residual}")
+    med = float(np.median(times))
+
+    # ---- FP64 tm() kernel bandwidth (the drop-in Dslash) ----------------------
+    y = q.Spinor(L)
+    for _ in range(10):
+        y.tm(b, u, p)
+    q.prof_enable(True)
+    nrep = 100
+    for _ in range(nrep):
+        y.tm(b, u, p)
+    n64, ms64, by64 = q.prof_read()
+    q.prof_enable(False)
+    tm_us = ms64 / n64 * 1e3
+    tm_gbs = by64 / n64 / (ms64 / n64 * 1e-3) / 1e9
+
+    cpu = None if args.no_cpu else cpu_baseline(U, src, results["fp64_cg_full"]["iterations"])
+
+    achieved = hop_bytes / (hop_ms * 1e-3) / 1e9 if hop_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": med,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": med * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp16_eo_rel": "fp16-stored/fp32-accumulate inner, fp64 outer",
+                  "bf16_eo_rel": "bf16-stored/fp32-accumulate inner, fp64 outer",
+                  "fp32_eo_rel": "fp32 inner, fp64 outer"}[best],
+        "data": "synthetic (seeded SU(3) gauge eps=0.3, Z2 source)",
+        "config": {"workload": "24^3x48 single-RHS twisted-mass solve to true residual 1e-10, even-odd "
+                               "preconditioned, kappa=0.1632 mu=0.004 antiperiodic t",
+                   "lattice": list(L), "solver": best, "sites": V},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "hop_kernel (even-odd half-lattice hops of the timed solves)",
+                     "launches": launches, "avg_launch_us": hop_ms / max(launches, 1) * 1e3,
+                     "bytes_per_launch": hop_bytes / max(launches, 1)},
+        "cpu_baseline": cpu,
+        "time_to_solution": results,
+        "speedup_vs_fp64_cg": results["fp64_cg_full"]["time_s"] / med,
+        "dslash_fp64_tm": {"avg_launch_us": tm_us, "GB/s": tm_gbs, "frac": tm_gbs / HBM_PEAK_GBS,
+                           "bytes_per_site": 960},
+        "timed_total_s": total,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+/*
+ * qkx_hip.h -- C ABI of libqkx_hip.so: the twisted-mass Wilson operator,
+ * its CG solvers and BLAS on one MI355X (gfx950).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No HIP
+ * or C++ types appear here, so the header can be included from the C++98
+ * reference-compatible classes (include/qkxtm/) and bound from ctypes.
+ *
+ * Every entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Conventions (identical to the reference):
+ *   site order   iv = ((t*Lz + z)*Ly + y)*Lx + x          include/qkxTM.h:130
+ *   host spinor  Complex M[V][4][3]   (re,im interleaved)  include/lattice_util.h:90
+ *   host gauge   Complex M[V][4][3][3], mu 0=x 1=y 2=z 3=t include/lattice_util.h:196
+ *   parity       EO=0 OE=1 OO=2 EE=3                      include/lattice_util.h:23
+ * Device storage is the library's own (parity-blocked struct-of-arrays); the
+ * upload/download calls convert.
+ *
+ * Threading: all calls run on one library stream per process and are
+ * serialised by the caller (the reference is single-threaded per rank).
+ */
+#ifndef QKX_HIP_H
+#define QKX_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* The reference aborts through errorQKXTM (include/qkxTM.h:137-141) and only
+ * prints a warning when CG hits maxIter (lib/solvers.cpp:68-70).  This
+ * library returns a status instead; qkx_last_error() gives the message. */
+#define QKX_SUCCESS        0
+#define QKX_NOT_CONVERGED  1   /* solver hit maxIter (reference: warning only) */
+#define QKX_ERR_ARG       -1   /* bad argument (shape, parity, handle)         */
+#define QKX_ERR_HIP       -2   /* HIP runtime error                             */
+#define QKX_ERR_NOMEM     -3   /* device allocation failed                      */
+#define QKX_ERR_NONFINITE -4   /* NaN/Inf met in a solver                       */
+#define QKX_ERR_NODEVICE  -5   /* no usable GPU                                 */
+#define QKX_ERR_BREAKDOWN -6   /* solver breakdown (p^dag A p <= 0)             */
+
+/* ---- enums ------------------------------------------------------------- */
+#define QKX_PARITY_ALL  -1
+#define QKX_PARITY_EO    0     /* hopping: even output sites (odd zeroed)      */
+#define QKX_PARITY_OE    1     /* hopping: odd output sites (even zeroed)      */
+#define QKX_PARITY_OO    2     /* twist-PC / Schur: odd sites                  */
+#define QKX_PARITY_EE    3     /* twist-PC / Schur: even sites                 */
+
+#define QKX_BASIS_UKQCD  0     /* include/hopping_term.h                        */
+#define QKX_BASIS_CHIRAL 1     /* include/hopping_term_chiral.h                 */
+
+#define QKX_PREC_DOUBLE  0
+#define QKX_PREC_SINGLE  1
+#define QKX_PREC_HALF    2     /* FP16 storage + one FP32 scale per site        */
+#define QKX_PREC_BF16    3     /* BF16 storage (gauge and spinor)               */
+
+#define QKX_RECON_18    18     /* full 3x3 complex link                         */
+#define QKX_RECON_12    12     /* two rows stored, third = conj(row0 x row1)    */
+
+#define QKX_SUBSET_FULL  0     /* all V sites                                   */
+#define QKX_SUBSET_EVEN  1     /* V/2 even sites                                */
+#define QKX_SUBSET_ODD   2     /* V/2 odd sites                                 */
+
+/* site-local twist family (qkx_twist) */
+#define QKX_TWIST          0   /* A      = 1 + i a g5     applyTwist        */
+#define QKX_TWIST_DAG      1   /* A^dag  = 1 - i a g5     applyTwistDag     */
+#define QKX_TWIST_INV      2   /* A^-1                     applyTwistInv     */
+#define QKX_TWIST_DAG_INV  3   /* (A^dag)^-1               applyTwistDagInv  */
+#define QKX_GAMMA5         4   /* g5                       applyGamma5       */
+
+/* hop + site term (qkx_twist_add_dslash) */
+#define QKX_XPAY_TM        0   /* -k D + psi + i a g5 psi   applyTwistAddDslash       */
+#define QKX_XPAY_TM_DAG    1   /* -k D + psi - i a g5 psi   applyTwistDagAddDslashDag */
+#define QKX_XPAY_WILSON    2   /* psi - k D                 applyDeltaAddDslash       */
+#define QKX_XPAY_PC        3   /* -k^2 D + (psi + i a g5 psi) on one parity  applyTwistAddDslash_PC */
+
+/* full-lattice operators (qkx_tm) */
+#define QKX_OP_TM          0   /* Vector::tm / tm_chiral */
+#define QKX_OP_TM_DAG      1   /* Vector::tmDag          */
+#define QKX_OP_TM_DAG_TM   2   /* Vector::tmDagtm        */
+#define QKX_OP_WILSON      3   /* Vector::Wilson         */
+
+/* ---- parameters (hot-path subset of LatticeInfo, include/qkxTM.h:156-204) */
+typedef struct {
+    int    L[4];           /* LatticeInfo::L[4]          qkxTM.h:166          */
+    double kappa;          /* LatticeInfo::kappa         qkxTM.h:170          */
+    double mu;             /* LatticeInfo::mu            qkxTM.h:172          */
+    int    twistSign;      /* LatticeInfo::twistSign     qkxTM.h:173 (+1/-1)  */
+    int    boundaryT;      /* LatticeInfo::boundaryT     qkxTM.h:184; see qkx_gauge_create */
+    double tol;            /* LatticeInfo::tol           qkxTM.h:174          */
+    int    maxIter;        /* LatticeInfo::maxIter       qkxTM.h:175          */
+    double reliableDelta;  /* LatticeInfo::reliableDelta qkxTM.h:176          */
+} qkx_params;
+
+typedef struct qkx_gauge_s  qkx_gauge;
+typedef struct qkx_spinor_s qkx_spinor;
+
+/* ---- library --------------------------------------------------------------- */
+/* Selects the device and creates the library stream.  Idempotent.  No
+ * reference counterpart (the reference has no device). */
+int  qkx_init(int device);
+int  qkx_finalize(void);
+const char *qkx_last_error(void);
+int  qkx_device_count(int *n);
+int  qkx_sync(void);
+/* HIP events on the library stream, for timing (ms between start and stop). */
+int  qkx_timer_start(void);
+int  qkx_timer_stop(double *ms);
+/* Per-kernel instrumentation: when on, every hopping-kernel launch is
+ * bracketed by events; qkx_prof_read returns launches and summed ms of the
+ * hopping kernels since the last reset. */
+int  qkx_prof_enable(int on);
+int  qkx_prof_read(int *launches, double *ms, double *bytes);
+
+/* ---- fields (Gauge / Vector, include/lattice_util.h:80-212) ---------------- */
+/* Gauge::Gauge lib/lattice_util.cpp:649-678.  boundaryT = -1 multiplies the
+ * t-links of the last time slice by -1 inside the kernels, which is what
+ * Gauge::applyBoundaryCondition (lib/lattice_util.cpp:708-724) does to the
+ * stored links; use +1 when the uploaded links already carry the sign.
+ * RECON_12 needs SU(3) links. */
+int qkx_gauge_create(qkx_gauge **g, const int L[4], int prec, int recon, int boundaryT);
+int qkx_gauge_destroy(qkx_gauge *g);
+/* Host AoS (reference layout) -> device. */
+int qkx_gauge_upload(qkx_gauge *g, const double *U_aos);
+/* Device -> device precision/compression change (for mixed-precision solvers). */
+int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src);
+
+/* Vector::Vector lib/lattice_util.cpp:382-413 */
+int qkx_spinor_create(qkx_spinor **s, const int L[4], int prec, int subset);
+int qkx_spinor_destroy(qkx_spinor *s);
+/* Host AoS Complex[V][4][3] <-> device.  A parity subset takes its own sites
+ * from the full host array; download writes zeros on the other parity. */
+int qkx_spinor_upload(qkx_spinor *s, const double *psi_aos);
+int qkx_spinor_download(const qkx_spinor *s, double *psi_aos);
+/* Vector::zero :443-447, Vector::copy :449-453 (copy also converts precision
+ * and moves a parity between full and subset fields). */
+int qkx_spinor_zero(qkx_spinor *s);
+int qkx_spinor_copy(qkx_spinor *dst, const qkx_spinor *src);
+
+/* ---- Dirac operators (Vector methods, include/lattice_util.h:102-137) ------ */
+/* applyDslash :1158, applyDslash_chiral :1215, applyDslashDag :1643,
+ * applyDslash_xx :1272, applyDslash_xx_chiral :1346 (lib/lattice_util.cpp).
+ * parity: QKX_PARITY_ALL, _EO (even outputs) or _OE (odd outputs); on a full
+ * output field the other parity is zeroed, as in the reference.  out != in. */
+int qkx_dslash(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
+               int basis, int dagger, int parity);
+
+/* applyTwist :1563, applyTwistDag :1726, applyTwistInv :1588,
+ * applyTwistInv_chiral :1615, applyTwistDagInv :1750, applyGamma5 :1778,
+ * applyGamma5_chiral :1800.  out may alias in. */
+int qkx_twist(qkx_spinor *out, const qkx_spinor *in, const qkx_params *p,
+              int basis, int kind);
+
+/* applyTwistAddDslash :1420 (+_chiral :1444), applyTwistDagAddDslashDag :1701,
+ * applyDeltaAddDslash :1552, applyTwistAddDslash_PC :1468 (+_chiral :1510;
+ * parity OO/EE).  out may alias dslash. */
+int qkx_twist_add_dslash(qkx_spinor *out, const qkx_spinor *dslash,
+                         const qkx_spinor *psi, const qkx_params *p,
+                         int basis, int kind, int parity);
+
+/* tm :1925, tm_chiral :1930, tmDag :1935, tmDagtm :1940, Wilson :1949.
+ * The hop and the site term run as one fused kernel.  out != in. */
+int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
+           const qkx_params *p, int basis, int op);
+
+/* tm_xx :2001, tm_xx_chiral :2037 (normal = 0) and tm_xxDagTm_xx :2024,
+ * tm_xxDagTm_xx_chiral :2060 (normal = 1): the even-odd Schur operator
+ * A - k^2 H A^-1 H on parity EE or OO.  Full fields in and out (other parity
+ * of out zeroed), or parity-subset fields of the chosen parity. */
+int qkx_tm_xx(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
+              const qkx_params *p, int basis, int parity, int normal);
+
+/* ---- BLAS (include/blas_qkxTM.h:3-5, lib/blas_qkxTM.cpp) -------------------- */
+int qkx_redot(double *res, const qkx_spinor *x, const qkx_spinor *y); /* reCdotXdagY :4  */
+int qkx_axpy(qkx_spinor *x, double a, const qkx_spinor *y);          /* X_eq_X_p_aY :31 */
+int qkx_xpay(qkx_spinor *x, double a, const qkx_spinor *y);          /* X_eq_aX_p_Y :45 */
+int qkx_norm2(double *res, const qkx_spinor *x);                     /* Vector::norm2 lattice_util.cpp:464 */
+
+/* ---- solvers ---------------------------------------------------------------- */
+typedef struct {
+    int    iterations;        /* inner iterations (all precisions)             */
+    int    reliable_updates;  /* FP64 residual recomputations                  */
+    int    converged;         /* 1 if the stopping test passed                 */
+    double iterated_r2;       /* final iterated |r|^2 (normal equations)       */
+    double true_residual;     /* |b - D x| / |b| recomputed in FP64 (-1: n/a)  */
+    double seconds;           /* device wall time of the solve                 */
+} qkx_solver_stats;
+
+/* CG::CG + CG::operator() (include/solvers.h:12,21; lib/solvers.cpp:8-81):
+ * CG on D^dag D x = b over the full lattice, UKQCD basis, FP64, x accumulated
+ * onto its input value with r0 = b (the reference assumes x = 0), stop when
+ * |r|^2 <= tol^2 |b|^2 or after maxIter iterations.  true_residual is left at
+ * -1 (the reference computes none). */
+int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+           const qkx_params *p, qkx_solver_stats *st);
+
+/* Solve D x = b (UKQCD basis) to true relative residual |b - D x|/|b| <= tol.
+ * No reference counterpart: the reference has no even-odd, mixed-precision
+ * or true-residual solver.  Options: */
+typedef struct {
+    int    eo;               /* 1: even-odd Schur system on the even sites      */
+    int    inner_prec;       /* QKX_PREC_* of the inner CG (DOUBLE = pure FP64) */
+    int    inner_recon;      /* gauge compression of the inner operator         */
+    double delta;            /* reliable-update trigger (residual drop)         */
+    int    check_every;      /* FP64 true-residual check period (pure FP64)     */
+    int    maxIter;          /* iteration cap over all restarts                 */
+    double tol;              /* target true relative residual                   */
+} qkx_solve_opts;
+
+/* U must be FP64 RECON_18.  U_inner (may be NULL -> derived internally and
+ * cached) is the gauge used by the inner operator. */
+int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+              const qkx_gauge *U_inner, const qkx_params *p,
+              const qkx_solve_opts *o, qkx_solver_stats *st);
+
+/* Relative true residual |b - D x| / |b| in FP64, full lattice, UKQCD. */
+int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,
+                      const qkx_gauge *U, const qkx_params *p);
+
+/* ---- synthetic inputs (stand-in for the LIME readers, lib/read_conf_lime.cpp) */
+/* Counter-hash RNG keyed by (seed, site, direction): thread-order independent.
+ * Gauge: U = exp(i eps H), H Gaussian Hermitian traceless, reunitarised to
+ * SU(3).  Z2: (+-1 +-i)/sqrt2 per component.  All in host reference layout. */
+int qkx_gen_gauge(double *U_aos, const int L[4], double eps, unsigned long long seed);
+int qkx_gen_z2(double *psi_aos, const int L[4], unsigned long long seed);
+int qkx_gen_gaussian(double *psi_aos, const int L[4], unsigned long long seed);
+int qkx_gen_point(double *psi_aos, const int L[4], const int x0[4], int spin, int colour);
+int qkx_gen_site(int x0[4], const int L[4], unsigned long long seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QKX_HIP_H */

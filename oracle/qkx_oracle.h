@@ -1,0 +1,74 @@
+/*
+ * qkx_oracle.h -- CPU restatement of the reference twisted-mass operator,
+ * its CG and BLAS (FP64, reference AoS layout).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker: tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg may call it; the product library
+ * (libqkx_hip.so) never links or calls it.
+ *
+ * Layout (reference include/qkxTM.h:117-134):
+ *   site   iv  = ((t*Lz + z)*Ly + y)*Lx + x                 (LEXIC)
+ *   spinor psi[iv][spin 4][colour 3]  complex, interleaved re/im
+ *   gauge  U[iv][mu 4][row 3][col 3]  complex, mu: 0=x 1=y 2=z 3=t
+ *
+ * Parity selectors follow reference include/lattice_util.h:23
+ *   enum PARITY{EO=0, OE=1, OO=2, EE=3}
+ *   hopping: EO -> even outputs computed (odd zeroed), OE -> odd outputs
+ *   twist-PC: OO -> odd outputs, EE -> even outputs
+ */
+#ifndef QKX_ORACLE_H
+#define QKX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_EO = 0, ORC_OE = 1, ORC_OO = 2, ORC_EE = 3, ORC_ALL = -1 };
+enum { ORC_UKQCD = 0, ORC_CHIRAL = 1 };
+enum { ORC_TWIST = 0, ORC_TWIST_DAG = 1, ORC_TWIST_INV = 2, ORC_TWIST_DAG_INV = 3 };
+enum { ORC_TM = 0, ORC_TM_DAG = 1, ORC_TM_DAG_TM = 2, ORC_WILSON = 3 };
+
+/* Hopping term H (dagger=0) or H^dagger (dagger=1).  parity: ORC_ALL, ORC_EO, ORC_OE. */
+void orc_hop(double *out, const double *psi, const double *U, const int L[4],
+             int basis, int dagger, int parity);
+
+/* out = -kappa*dslash + psi + i a g5 psi   (dagger: a -> -a), a = 2 kappa mu s */
+void orc_twist_add_dslash(double *out, const double *dslash, const double *psi,
+                          const int L[4], double kappa, double mu, int twistSign,
+                          int basis, int dagger);
+/* out = -kappa^2*dslash + (psi + i a g5 psi) on parity (OO/EE), zero elsewhere */
+void orc_twist_add_dslash_pc(double *out, const double *dslash, const double *psi,
+                             const int L[4], double kappa, double mu, int twistSign,
+                             int basis, int parity);
+/* out = psi - kappa*dslash */
+void orc_delta_add_dslash(double *out, const double *dslash, const double *psi,
+                          const int L[4], double kappa);
+/* site-local twist family: A, A^dag, A^-1, (A^dag)^-1 */
+void orc_twist(double *out, const double *psi, const int L[4], double kappa,
+               double mu, int twistSign, int basis, int kind);
+void orc_gamma5(double *out, const double *psi, const int L[4], int basis);
+
+/* composites */
+void orc_tm(double *out, const double *psi, const double *U, const int L[4],
+            double kappa, double mu, int twistSign, int basis, int op);
+void orc_tm_xx(double *out, const double *psi, const double *U, const int L[4],
+               double kappa, double mu, int twistSign, int basis, int parity, int normal);
+
+/* BLAS over 12*V complex numbers */
+double orc_redot(const double *x, const double *y, long V);
+void orc_axpy(double *x, double a, const double *y, long V);   /* x = x + a y */
+void orc_xpay(double *x, double a, const double *y, long V);   /* x = a x + y */
+double orc_norm2(const double *x, long V);
+
+/* Reference CG on D^dag D; returns iterations; r2_final receives the iterated r2 */
+int orc_cg(double *x, const double *b, const double *U, const int L[4],
+           double kappa, double mu, int twistSign, double tol, int maxIter,
+           double *r2_final);
+
+/* Antiperiodic (or periodic) time boundary as a link sign on the last slice */
+void orc_apply_bc(double *U, const int L[4], int boundaryT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,362 @@
+"""qkxtm_mpi_amd -- MI355X twisted-mass Wilson operator and CG solvers.
+
+Python host mirror of the C ABI in ``include/qkx_hip.h`` (libqkx_hip.so, built
+in-tree for gfx950).  The reference's operators are methods of
+``qkxTM::Vector`` acting on host arrays ``Complex M[V][4][3]``
+(reference include/lattice_util.h:80-156); here a :class:`Spinor` owns a
+device field and converts from/to that same host layout, and every operator
+keeps the reference's name and argument meaning.
+
+There is no CPU fallback: importing works without a GPU (the generators and
+the library load are host-only), but any compute call needs the HIP library
+and a device and raises :class:`QkxError` otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqkx_hip.so")
+
+# enums (include/qkx_hip.h)
+SUCCESS, NOT_CONVERGED = 0, 1
+PARITY_ALL, EO, OE, OO, EE = -1, 0, 1, 2, 3            # include/lattice_util.h:23
+UKQCD, CHIRAL = 0, 1
+DOUBLE, SINGLE, HALF, BF16 = 0, 1, 2, 3
+RECON_18, RECON_12 = 18, 12
+FULL, EVEN, ODD = 0, 1, 2
+TWIST, TWIST_DAG, TWIST_INV, TWIST_DAG_INV, GAMMA5 = 0, 1, 2, 3, 4
+XPAY_TM, XPAY_TM_DAG, XPAY_WILSON, XPAY_PC = 0, 1, 2, 3
+OP_TM, OP_TM_DAG, OP_TM_DAG_TM, OP_WILSON = 0, 1, 2, 3
+
+
+class QkxError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """Hot-path subset of the reference LatticeInfo (include/qkxTM.h:156-204)."""
+    _fields_ = [("L", ctypes.c_int * 4), ("kappa", ctypes.c_double), ("mu", ctypes.c_double),
+                ("twistSign", ctypes.c_int), ("boundaryT", ctypes.c_int), ("tol", ctypes.c_double),
+                ("maxIter", ctypes.c_int), ("reliableDelta", ctypes.c_double)]
+
+
+class SolverStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("reliable_updates", ctypes.c_int), ("converged", ctypes.c_int),
+                ("iterated_r2", ctypes.c_double), ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("eo", ctypes.c_int), ("inner_prec", ctypes.c_int), ("inner_recon", ctypes.c_int),
+                ("delta", ctypes.c_double), ("check_every", ctypes.c_int), ("maxIter", ctypes.c_int),
+                ("tol", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes (all return int unless listed in _RESTYPES)
+SIGNATURES = {
+    "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [],
+    "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
+    "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
+    "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P],
+    "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
+    "qkx_spinor_upload": [_P, _P], "qkx_spinor_download": [_P, _P], "qkx_spinor_zero": [_P],
+    "qkx_spinor_copy": [_P, _P],
+    "qkx_dslash": [_P, _P, _P, _I, _I, _I], "qkx_twist": [_P, _P, ctypes.POINTER(Params), _I, _I],
+    "qkx_twist_add_dslash": [_P, _P, _P, ctypes.POINTER(Params), _I, _I, _I],
+    "qkx_tm": [_P, _P, _P, ctypes.POINTER(Params), _I, _I],
+    "qkx_tm_xx": [_P, _P, _P, ctypes.POINTER(Params), _I, _I, _I],
+    "qkx_redot": [_DP, _P, _P], "qkx_axpy": [_P, _D, _P], "qkx_xpay": [_P, _D, _P], "qkx_norm2": [_DP, _P],
+    "qkx_cg": [_P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolverStats)],
+    "qkx_solve": [_P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolveOpts), ctypes.POINTER(SolverStats)],
+    "qkx_true_residual": [_DP, _P, _P, _P, ctypes.POINTER(Params)],
+    "qkx_gen_gauge": [_P, _IP, _D, ctypes.c_ulonglong], "qkx_gen_z2": [_P, _IP, ctypes.c_ulonglong],
+    "qkx_gen_gaussian": [_P, _IP, ctypes.c_ulonglong], "qkx_gen_point": [_P, _IP, _IP, _I, _I],
+    "qkx_gen_site": [_IP, _IP, ctypes.c_ulonglong],
+}
+_RESTYPES = {"qkx_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libqkx_hip.so (in-tree).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QkxError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C qkxtm_mpi_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def _check(rc, what=""):
+    if rc not in (SUCCESS,):
+        msg = lib().qkx_last_error().decode()
+        raise QkxError(f"{what} failed (status {rc}): {msg}")
+    return rc
+
+
+def _ivec(vals):
+    return (ctypes.c_int * len(vals))(*vals)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def volume(L):
+    return int(L[0]) * int(L[1]) * int(L[2]) * int(L[3])
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs (host, no GPU): reference layout arrays of complex128
+def gen_gauge(L, eps, seed):
+    """U[V,4,3,3] complex128, U = exp(i eps H) reunitarised to SU(3)."""
+    U = np.zeros((volume(L), 4, 3, 3), dtype=np.complex128)
+    _check(lib().qkx_gen_gauge(_ptr(U), _ivec(L), eps, seed), "qkx_gen_gauge")
+    return U
+
+
+def gen_z2(L, seed):
+    psi = np.zeros((volume(L), 4, 3), dtype=np.complex128)
+    _check(lib().qkx_gen_z2(_ptr(psi), _ivec(L), seed), "qkx_gen_z2")
+    return psi
+
+
+def gen_gaussian(L, seed):
+    psi = np.zeros((volume(L), 4, 3), dtype=np.complex128)
+    _check(lib().qkx_gen_gaussian(_ptr(psi), _ivec(L), seed), "qkx_gen_gaussian")
+    return psi
+
+
+def gen_point(L, x0, spin, colour):
+    psi = np.zeros((volume(L), 4, 3), dtype=np.complex128)
+    _check(lib().qkx_gen_point(_ptr(psi), _ivec(L), _ivec(x0), spin, colour), "qkx_gen_point")
+    return psi
+
+
+def gen_site(L, seed):
+    x0 = (ctypes.c_int * 4)()
+    _check(lib().qkx_gen_site(x0, _ivec(L), seed), "qkx_gen_site")
+    return [x0[i] for i in range(4)]
+
+
+# ---------------------------------------------------------------------------
+def init(device=0):
+    _check(lib().qkx_init(device), "qkx_init")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().qkx_device_count(ctypes.byref(n))
+    return n.value
+
+
+def sync():
+    _check(lib().qkx_sync(), "qkx_sync")
+
+
+def params(L, kappa, mu, twistSign=1, boundaryT=-1, tol=1e-10, maxIter=10000, reliableDelta=0.1):
+    p = Params()
+    for i in range(4):
+        p.L[i] = L[i]
+    p.kappa, p.mu, p.twistSign, p.boundaryT = kappa, mu, twistSign, boundaryT
+    p.tol, p.maxIter, p.reliableDelta = tol, int(maxIter), reliableDelta
+    return p
+
+
+class Gauge:
+    """Device gauge field (reference qkxTM::Gauge, include/lattice_util.h:186-212).
+
+    ``boundaryT=-1`` applies the antiperiodic sign of the last time slice inside
+    the kernels; pass ``+1`` for links that already carry it
+    (Gauge::applyBoundaryCondition, lib/lattice_util.cpp:708-724)."""
+
+    def __init__(self, L, prec=DOUBLE, recon=RECON_18, boundaryT=-1, host=None):
+        self.L = tuple(L)
+        self.prec, self.recon = prec, recon
+        h = ctypes.c_void_p()
+        _check(lib().qkx_gauge_create(ctypes.byref(h), _ivec(L), prec, recon, boundaryT), "qkx_gauge_create")
+        self.h = h
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        assert U.size == volume(self.L) * 36
+        _check(lib().qkx_gauge_upload(self.h, _ptr(U)), "qkx_gauge_upload")
+
+    def convert_from(self, src: "Gauge"):
+        _check(lib().qkx_gauge_convert(self.h, src.h), "qkx_gauge_convert")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qkx_gauge_destroy(self.h)
+            self.h = None
+
+
+class Spinor:
+    """Device spinor field (reference qkxTM::Vector, include/lattice_util.h:80-156)."""
+
+    def __init__(self, L, prec=DOUBLE, subset=FULL, host=None):
+        self.L = tuple(L)
+        self.prec, self.subset = prec, subset
+        h = ctypes.c_void_p()
+        _check(lib().qkx_spinor_create(ctypes.byref(h), _ivec(L), prec, subset), "qkx_spinor_create")
+        self.h = h
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.size == volume(self.L) * 12
+        _check(lib().qkx_spinor_upload(self.h, _ptr(psi)), "qkx_spinor_upload")
+
+    def download(self):
+        out = np.zeros((volume(self.L), 4, 3), dtype=np.complex128)
+        _check(lib().qkx_spinor_download(self.h, _ptr(out)), "qkx_spinor_download")
+        return out
+
+    def zero(self):
+        _check(lib().qkx_spinor_zero(self.h), "qkx_spinor_zero")
+
+    def copy(self, src: "Spinor"):
+        _check(lib().qkx_spinor_copy(self.h, src.h), "qkx_spinor_copy")
+
+    def norm2(self):
+        v = ctypes.c_double()
+        _check(lib().qkx_norm2(ctypes.byref(v), self.h), "qkx_norm2")
+        return v.value
+
+    # ---- operators, named after the reference's Vector methods -------------
+    def applyDslash(self, psi, u, basis=UKQCD):
+        _check(lib().qkx_dslash(self.h, psi.h, u.h, basis, 0, PARITY_ALL), "applyDslash")
+
+    def applyDslashDag(self, psi, u, basis=UKQCD):
+        _check(lib().qkx_dslash(self.h, psi.h, u.h, basis, 1, PARITY_ALL), "applyDslashDag")
+
+    def applyDslash_xx(self, psi, u, parity, basis=UKQCD):
+        _check(lib().qkx_dslash(self.h, psi.h, u.h, basis, 0, parity), "applyDslash_xx")
+
+    def twist(self, psi, p, kind, basis=UKQCD):
+        _check(lib().qkx_twist(self.h, psi.h, ctypes.byref(p), basis, kind), "qkx_twist")
+
+    def twist_add_dslash(self, dslash, psi, p, kind, basis=UKQCD, parity=PARITY_ALL):
+        _check(lib().qkx_twist_add_dslash(self.h, dslash.h, psi.h, ctypes.byref(p), basis, kind, parity),
+               "qkx_twist_add_dslash")
+
+    def tm_op(self, psi, u, p, op=OP_TM, basis=UKQCD):
+        _check(lib().qkx_tm(self.h, psi.h, u.h, ctypes.byref(p), basis, op), "qkx_tm")
+
+    def tm(self, psi, u, p, basis=UKQCD):
+        self.tm_op(psi, u, p, OP_TM, basis)
+
+    def tmDag(self, psi, u, p):
+        self.tm_op(psi, u, p, OP_TM_DAG)
+
+    def tmDagtm(self, psi, u, p):
+        self.tm_op(psi, u, p, OP_TM_DAG_TM)
+
+    def Wilson(self, psi, u, p, basis=UKQCD):
+        self.tm_op(psi, u, p, OP_WILSON, basis)
+
+    def tm_xx(self, psi, u, p, parity, basis=UKQCD):
+        _check(lib().qkx_tm_xx(self.h, psi.h, u.h, ctypes.byref(p), basis, parity, 0), "tm_xx")
+
+    def tm_xxDagTm_xx(self, psi, u, p, parity, basis=UKQCD):
+        _check(lib().qkx_tm_xx(self.h, psi.h, u.h, ctypes.byref(p), basis, parity, 1), "tm_xxDagTm_xx")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qkx_spinor_destroy(self.h)
+            self.h = None
+
+
+# ---- BLAS (include/blas_qkxTM.h:3-5) ---------------------------------------
+def reCdotXdagY(x: Spinor, y: Spinor):
+    v = ctypes.c_double()
+    _check(lib().qkx_redot(ctypes.byref(v), x.h, y.h), "reCdotXdagY")
+    return v.value
+
+
+def X_eq_X_p_aY(x: Spinor, a: float, y: Spinor):
+    _check(lib().qkx_axpy(x.h, a, y.h), "X_eq_X_p_aY")
+
+
+def X_eq_aX_p_Y(x: Spinor, a: float, y: Spinor):
+    _check(lib().qkx_xpay(x.h, a, y.h), "X_eq_aX_p_Y")
+
+
+# ---- solvers -----------------------------------------------------------------
+class CG:
+    """Reference CG (include/solvers.h:3-22, lib/solvers.cpp:8-81): solves
+    D^dag D out = in on the full lattice; ``out`` accumulates onto its value."""
+
+    def __init__(self, p: Params, gauge: Gauge):
+        self.p, self.u = p, gauge
+        self.stats = SolverStats()
+
+    def __call__(self, out: Spinor, inp: Spinor):
+        rc = lib().qkx_cg(out.h, inp.h, self.u.h, ctypes.byref(self.p), ctypes.byref(self.stats))
+        if rc == NOT_CONVERGED:
+            return self.stats  # the reference only warns (lib/solvers.cpp:68-70)
+        _check(rc, "CG")
+        return self.stats
+
+
+def solve(x: Spinor, b: Spinor, u: Gauge, p: Params, eo=True, inner_prec=DOUBLE, inner_recon=RECON_12,
+          delta=0.1, tol=1e-10, maxIter=100000, u_inner: Gauge | None = None, allow_not_converged=False):
+    """Solve D x = b to true relative residual ``tol``; returns SolverStats."""
+    o = SolveOpts(int(eo), inner_prec, inner_recon, delta, 50, int(maxIter), tol)
+    st = SolverStats()
+    rc = lib().qkx_solve(x.h, b.h, u.h, u_inner.h if u_inner else None, ctypes.byref(p), ctypes.byref(o),
+                         ctypes.byref(st))
+    if rc == NOT_CONVERGED and allow_not_converged:
+        return st
+    _check(rc, "qkx_solve")
+    return st
+
+
+def true_residual(x: Spinor, b: Spinor, u: Gauge, p: Params):
+    v = ctypes.c_double()
+    _check(lib().qkx_true_residual(ctypes.byref(v), x.h, b.h, u.h, ctypes.byref(p)), "qkx_true_residual")
+    return v.value
+
+
+class Timer:
+    """HIP events on the library stream."""
+
+    def start(self):
+        _check(lib().qkx_timer_start(), "qkx_timer_start")
+
+    def stop(self):
+        v = ctypes.c_double()
+        _check(lib().qkx_timer_stop(ctypes.byref(v)), "qkx_timer_stop")
+        return v.value
+
+
+def prof_enable(on=True):
+    _check(lib().qkx_prof_enable(int(on)), "qkx_prof_enable")
+
+
+def prof_read():
+    n, ms, by = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().qkx_prof_read(ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)), "qkx_prof_read")
+    return n.value, ms.value, by.value

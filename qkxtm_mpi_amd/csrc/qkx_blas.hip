@@ -1,0 +1,218 @@
+// qkx_blas.hip -- solver BLAS: real dot, norm, axpy/xpay and the fused CG
+// update, with single-launch reductions.
+//
+// Reference: reCdotXdagY lib/blas_qkxTM.cpp:4-28 (Re sum conj(x) y, then an
+// MPI_Allreduce), X_eq_X_p_aY :31-42, X_eq_aX_p_Y :45-56.
+//
+// Reductions: every thread accumulates in FP64, a wavefront reduces with
+// cross-lane shuffles, the workgroup through LDS, and the last workgroup to
+// finish (atomic ticket) adds the per-workgroup partials in double-double
+// (compensated) arithmetic and resets the ticket, so one launch gives the
+// final value in device memory and no memset is needed between launches.
+#include "qkx_internal.h"
+#include "qkx_core.h"
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    double err = (a - (s - bb)) + (b - bb);
+    return {s, err};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    DD s = two_sum(a.hi, b.hi);
+    double lo = s.lo + a.lo + b.lo;
+    return two_sum(s.hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Reduce NV values per thread across the grid; result[k] = sum of v[k].
+template <int NV>
+__device__ void grid_reduce(double v[NV], double *partial, unsigned int *counter, double *result)
+{
+    __shared__ double lds[NV][kRedThreads / 64];
+    __shared__ bool last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double w = wave_sum(v[k]);
+        if (lane == 0) lds[k][wid] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[k][w];
+            partial[(size_t)k * gridDim.x + blockIdx.x] = s;
+        }
+        __threadfence();
+        unsigned int t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    __shared__ double lh[NV][kRedThreads], ll[NV][kRedThreads];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        DD acc{0.0, 0.0};
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+            acc = dd_add(acc, DD{partial[(size_t)k * gridDim.x + b], 0.0});
+        lh[k][threadIdx.x] = acc.hi;
+        ll[k][threadIdx.x] = acc.lo;
+    }
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                DD a{lh[k][threadIdx.x], ll[k][threadIdx.x]};
+                DD b{lh[k][threadIdx.x + st], ll[k][threadIdx.x + st]};
+                DD c = dd_add(a, b);
+                lh[k][threadIdx.x] = c.hi;
+                ll[k][threadIdx.x] = c.lo;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) result[k] = lh[k][0] + ll[k][0];
+        *counter = 0u;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(kRedThreads) void redot_kernel(S x, S y, int n, int mode, double *partial,
+                                                            unsigned int *counter, double *result)
+{
+    double acc[1] = {0.0};
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        Cx<typename S::Real> a[12];
+        x.load(s, a);
+        if (mode == 0) {
+            Cx<typename S::Real> b[12];
+            y.load(s, b);
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                acc[0] += (double)a[k].re * (double)b[k].re + (double)a[k].im * (double)b[k].im;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                acc[0] += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
+        }
+    }
+    grid_reduce<1>(acc, partial, counter, result);
+}
+
+// x = ax*x + ay*y   (scalars from host or from device memory *sc)
+template <class S>
+__global__ __launch_bounds__(256) void axpby_kernel(S x, S y, int n, double ax, double ay, const double *sc,
+                                                    int sc_mode)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    using T = typename S::Real;
+    double a = ax, b = ay;
+    if (sc_mode == 1) b = sc[0];          // x = x + sc*y
+    else if (sc_mode == 2) a = sc[0];     // x = sc*x + y
+    else if (sc_mode == 3) b = -sc[0];    // x = x - sc*y
+    Cx<T> xv[12], yv[12];
+    x.load(s, xv);
+    y.load(s, yv);
+#pragma unroll
+    for (int k = 0; k < 12; k++) xv[k] = {T(a) * xv[k].re + T(b) * yv[k].re, T(a) * xv[k].im + T(b) * yv[k].im};
+    x.store(s, xv);
+}
+
+int check_pair(const qkx_spinor_s *x, const qkx_spinor_s *y, const char *who)
+{
+    if (!x || !y) { qkx::set_error(std::string(who) + ": null field"); return QKX_ERR_ARG; }
+    if (x->prec != y->prec || x->nsites != y->nsites || x->subset != y->subset) {
+        qkx::set_error(std::string(who) + ": fields differ in precision, subset or size");
+        return QKX_ERR_ARG;
+    }
+    return QKX_SUCCESS;
+}
+
+} // namespace
+
+namespace qkx {
+
+// device reduction to result slot 'slot' of the reduce buffer (no host sync)
+int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result)
+{
+    int rc = check_pair(x, y ? y : x, "redot");
+    if (rc) return rc;
+    int n = x->nsites;
+    int nb = std::min(kRedBlocks, (n + kRedThreads - 1) / kRedThreads);
+    int mode = y ? 0 : 1;
+    return dispatch_prec(x->prec, [&](auto tag) {
+        using S = decltype(tag);
+        S X = make_spinor<S>(x), Y = make_spinor<S>(y ? y : x);
+        hipLaunchKernelGGL((redot_kernel<S>), dim3(nb), dim3(kRedThreads), 0, g_stream, X, Y, n, mode,
+                           reduce_scratch(), reduce_counter(), result);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+}
+
+int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode)
+{
+    int rc = check_pair(x, y, "axpby");
+    if (rc) return rc;
+    int n = x->nsites;
+    return dispatch_prec(x->prec, [&](auto tag) {
+        using S = decltype(tag);
+        S X = make_spinor<S>(x), Y = make_spinor<S>(y);
+        hipLaunchKernelGGL((axpby_kernel<S>), dim3((n + 255) / 256), dim3(256), 0, g_stream, X, Y, n, a, b,
+                           sc, sc_mode);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+}
+
+} // namespace qkx
+
+extern "C" {
+
+int qkx_redot(double *res, const qkx_spinor *x, const qkx_spinor *y)
+{
+    QKX_CHECK_ARG(res, "qkx_redot: null result");
+    int rc = qkx::redot_dev(x, y, qkx::reduce_result());
+    if (rc) return rc;
+    QKX_HIP(hipMemcpyAsync(res, qkx::reduce_result(), sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+int qkx_norm2(double *res, const qkx_spinor *x)
+{
+    QKX_CHECK_ARG(res && x, "qkx_norm2: null argument");
+    int rc = qkx::redot_dev(x, nullptr, qkx::reduce_result());
+    if (rc) return rc;
+    QKX_HIP(hipMemcpyAsync(res, qkx::reduce_result(), sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+int qkx_axpy(qkx_spinor *x, double a, const qkx_spinor *y) { return qkx::axpby_dev(x, y, 1.0, a, nullptr, 0); }
+int qkx_xpay(qkx_spinor *x, double a, const qkx_spinor *y) { return qkx::axpby_dev(x, y, a, 1.0, nullptr, 0); }
+
+} // extern "C"

@@ -1,0 +1,41 @@
+// qkx_core.h -- internal entry points shared by the library's translation units.
+#pragma once
+#include "qkx_hip.h"
+
+struct qkx_spinor_s;
+struct qkx_gauge_s;
+
+namespace qkx {
+
+// hopping kernel with fused epilogue (qkx_dslash.hip):
+//   out(par_out) = cR R + gR i g5 R + cX aux + gX i g5 aux, R = H in or H^dag in
+int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
+        int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out);
+// site-local combination (qkx_dslash.hip)
+int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
+         double cR, double gR, double cX, double gX, int g5out);
+void set_hop_tuning(int xcd_remap, int block);
+
+// BLAS (qkx_blas.hip): device-resident results, no host synchronisation
+int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);
+// x = a x + b y; sc_mode 1: b = sc[0], 2: a = sc[0], 3: b = -sc[0]
+int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode);
+
+// fields (qkx_fields.hip)
+int zero_parity(qkx_spinor_s *f, int parity);
+
+// core (qkx_core.cpp)
+double *result_slot(int k);
+qkx_spinor_s *pool_get(const int L[4], int prec, int subset);
+void pool_put(qkx_spinor_s *f);
+
+// RAII temporary from the pool
+struct Temp {
+    qkx_spinor_s *f;
+    Temp(const int L[4], int prec, int subset) : f(pool_get(L, prec, subset)) {}
+    ~Temp() { if (f) pool_put(f); }
+    Temp(const Temp &) = delete;
+    Temp &operator=(const Temp &) = delete;
+};
+
+} // namespace qkx

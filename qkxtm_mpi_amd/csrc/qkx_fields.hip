@@ -1,0 +1,453 @@
+// qkx_fields.hip -- device fields: allocation, host (reference AoS layout)
+// <-> device (parity-blocked SoA) conversion, precision conversion, copies.
+//
+// Host layout of the reference: Vector::M Complex[V][4][3]
+// (include/lattice_util.h:90), Gauge::M Complex[V][4][3][3] (:196).
+#include "qkx_internal.h"
+#include "qkx_core.h"
+#include <vector>
+
+namespace {
+
+// full-lattice parity-blocked index s -> lexicographic index
+__device__ __forceinline__ int blocked_to_lex(int s, const Geom &g)
+{
+    int p = (s >= g.Vh) ? 1 : 0;
+    int cb = s - p * g.Vh;
+    int xh = cb % g.Lh0;
+    int rest = cb / g.Lh0;
+    int y = rest % g.L[1];
+    int zt = rest / g.L[1];
+    int z = zt % g.L[2];
+    int t = zt / g.L[2];
+    int x = 2 * xh + ((p + y + z + t) & 1);
+    return x + g.L[0] * rest;
+}
+
+template <class S>
+__global__ void spinor_from_aos(S dst, const double2 *src, Geom g, int par0, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s_full = par0 * g.Vh + i;   // par0 = 0 for full fields
+    int lex = blocked_to_lex(s_full, g);
+    Cx<typename S::Real> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        double2 d = src[(size_t)lex * 12 + k];
+        v[k] = {(typename S::Real)d.x, (typename S::Real)d.y};
+    }
+    dst.store(i, v);
+}
+
+template <class S>
+__global__ void spinor_to_aos(double2 *dst, S src, Geom g, int par0, int n, int zero_other)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s_full = par0 * g.Vh + i;
+    int lex = blocked_to_lex(s_full, g);
+    Cx<typename S::Real> v[12];
+    src.load(i, v);
+#pragma unroll
+    for (int k = 0; k < 12; k++) dst[(size_t)lex * 12 + k] = make_double2((double)v[k].re, (double)v[k].im);
+    if (zero_other) {
+        // the other parity's site with the same cb
+        int lex2 = blocked_to_lex((1 - par0) * g.Vh + i, g);
+#pragma unroll
+        for (int k = 0; k < 12; k++) dst[(size_t)lex2 * 12 + k] = make_double2(0.0, 0.0);
+    }
+}
+
+// dst site i (in dst storage) <- src site (i + src_shift)
+template <class SD, class SS>
+__global__ void spinor_convert(SD dst, SS src, int n, int dst_shift, int src_shift)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Cx<typename SS::Real> v[12];
+    src.load(i + src_shift, v);
+    Cx<typename SD::Real> w[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) w[k] = {(typename SD::Real)v[k].re, (typename SD::Real)v[k].im};
+    dst.store(i + dst_shift, w);
+}
+
+template <class S>
+__global__ void spinor_zero(S dst, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Cx<typename S::Real> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = {0, 0};
+    dst.store(i, v);
+}
+
+template <class S>
+__global__ void spinor_zero_range(S dst, int n, int shift)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Cx<typename S::Real> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = {0, 0};
+    dst.store(i + shift, v);
+}
+
+// ---- gauge ------------------------------------------------------------------
+// Store one link U[9] (complex) at (mu, s) in the requested format.
+template <int PREC, int RECON>
+__device__ __forceinline__ void gauge_store(void *base, int V, int mu, int s, const Cx<double> U[9])
+{
+    if constexpr (PREC == QKX_PREC_DOUBLE) {
+        double2 *p = (double2 *)base;
+        constexpr int NC = RECON == QKX_RECON_18 ? 9 : 6;
+#pragma unroll
+        for (int k = 0; k < NC; k++) p[(size_t)(mu * NC + k) * V + s] = make_double2(U[k].re, U[k].im);
+    } else if constexpr (PREC == QKX_PREC_SINGLE && RECON == QKX_RECON_18) {
+        float2 *p = (float2 *)base;
+#pragma unroll
+        for (int k = 0; k < 9; k++) p[(size_t)(mu * 9 + k) * V + s] = make_float2((float)U[k].re, (float)U[k].im);
+    } else if constexpr (PREC == QKX_PREC_SINGLE) {
+        float4 *p = (float4 *)base;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            p[(size_t)(mu * 3 + k) * V + s] = make_float4((float)U[2 * k].re, (float)U[2 * k].im,
+                                                          (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
+    } else if constexpr (PREC == QKX_PREC_HALF) {
+        uint2 *p = (uint2 *)base;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            p[(size_t)(mu * 3 + k) * V + s] = pack_h4((float)U[2 * k].re, (float)U[2 * k].im,
+                                                      (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
+    } else {
+        uint2 *p = (uint2 *)base;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            p[(size_t)(mu * 3 + k) * V + s] = pack_b4((float)U[2 * k].re, (float)U[2 * k].im,
+                                                      (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
+    }
+}
+
+template <int PREC, int RECON>
+__global__ void gauge_from_aos(void *dst, const double2 *src, Geom g)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.V) return;
+    int lex = blocked_to_lex(s, g);
+    for (int mu = 0; mu < 4; mu++) {
+        Cx<double> U[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double2 d = src[((size_t)lex * 4 + mu) * 9 + k];
+            U[k] = {d.x, d.y};
+        }
+        gauge_store<PREC, RECON>(dst, g.V, mu, s, U);
+    }
+}
+
+template <int PREC, int RECON>
+__global__ void gauge_from_d18(void *dst, const double2 *src, int V)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    for (int mu = 0; mu < 4; mu++) {
+        Cx<double> U[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double2 d = src[(size_t)(mu * 9 + k) * V + s];
+            U[k] = {d.x, d.y};
+        }
+        gauge_store<PREC, RECON>(dst, V, mu, s, U);
+    }
+}
+
+// ---- staging buffer ---------------------------------------------------------
+void *g_stage = nullptr;
+size_t g_stage_bytes = 0;
+
+void *stage(size_t bytes)
+{
+    if (bytes > g_stage_bytes) {
+        if (g_stage) (void)hipFree(g_stage);
+        g_stage = nullptr;
+        g_stage_bytes = 0;
+        if (hipMalloc(&g_stage, bytes) != hipSuccess) return nullptr;
+        g_stage_bytes = bytes;
+    }
+    return g_stage;
+}
+
+template <class F>
+int dispatch_spinor(const qkx_spinor_s *f, F fn)
+{
+    return dispatch_prec(f->prec, [&](auto tag) { return fn(make_spinor<decltype(tag)>(f)); });
+}
+
+bool valid_dims(const int L[4])
+{
+    for (int m = 0; m < 4; m++)
+        if (L[m] < 2 || (L[m] & 1)) return false;
+    long V = (long)L[0] * L[1] * L[2] * L[3];
+    return V < (1L << 28);
+}
+
+} // namespace
+
+namespace qkx {
+
+size_t spinor_elem_bytes(int prec)
+{
+    switch (prec) {
+    case QKX_PREC_DOUBLE: return 16;
+    case QKX_PREC_SINGLE: return 16;
+    default: return 8;
+    }
+}
+int spinor_nchunks(int prec) { return prec == QKX_PREC_DOUBLE ? 12 : 6; }
+
+void parity_offsets(const qkx_spinor_s *f, int off[2])
+{
+    if (f->subset == QKX_SUBSET_FULL) { off[0] = 0; off[1] = f->Vh; }
+    else if (f->subset == QKX_SUBSET_EVEN) { off[0] = 0; off[1] = 0; }
+    else { off[0] = 0; off[1] = 0; }
+}
+
+Geom make_geom(const int L[4], int bc)
+{
+    Geom g;
+    for (int m = 0; m < 4; m++) g.L[m] = L[m];
+    g.Lh0 = L[0] / 2;
+    g.V = L[0] * L[1] * L[2] * L[3];
+    g.Vh = g.V / 2;
+    g.bc = bc;
+    return g;
+}
+
+// zero the sites of one parity of a full field
+int zero_parity(qkx_spinor_s *f, int parity)
+{
+    if (f->subset != QKX_SUBSET_FULL) return QKX_SUCCESS;
+    int n = f->Vh, shift = parity * f->Vh;
+    return dispatch_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((spinor_zero_range<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, g_stream, S, n,
+                           shift);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+}
+
+} // namespace qkx
+
+extern "C" {
+
+int qkx_spinor_create(qkx_spinor **out, const int L[4], int prec, int subset)
+{
+    int rc = qkx::ensure_init();
+    if (rc) return rc;
+    QKX_CHECK_ARG(out && L, "qkx_spinor_create: null argument");
+    QKX_CHECK_ARG(valid_dims(L), "qkx_spinor_create: every extent must be even and >= 2");
+    QKX_CHECK_ARG(prec >= QKX_PREC_DOUBLE && prec <= QKX_PREC_BF16, "qkx_spinor_create: bad precision");
+    QKX_CHECK_ARG(subset >= QKX_SUBSET_FULL && subset <= QKX_SUBSET_ODD, "qkx_spinor_create: bad subset");
+    auto *f = new qkx_spinor_s();
+    for (int m = 0; m < 4; m++) f->L[m] = L[m];
+    f->V = L[0] * L[1] * L[2] * L[3];
+    f->Vh = f->V / 2;
+    f->prec = prec;
+    f->subset = subset;
+    f->nsites = subset == QKX_SUBSET_FULL ? f->V : f->Vh;
+    f->bytes = qkx::spinor_elem_bytes(prec) * qkx::spinor_nchunks(prec) * (size_t)f->nsites;
+    if (hipMalloc(&f->data, f->bytes) != hipSuccess) {
+        delete f;
+        qkx::set_error("qkx_spinor_create: out of device memory");
+        return QKX_ERR_NOMEM;
+    }
+    f->nrm = nullptr;
+    if (prec == QKX_PREC_HALF) {
+        if (hipMalloc(&f->nrm, sizeof(float) * f->nsites) != hipSuccess) {
+            (void)hipFree(f->data);
+            delete f;
+            qkx::set_error("qkx_spinor_create: out of device memory");
+            return QKX_ERR_NOMEM;
+        }
+    }
+    *out = f;
+    return qkx_spinor_zero(f);
+}
+
+int qkx_spinor_destroy(qkx_spinor *f)
+{
+    if (!f) return QKX_SUCCESS;
+    (void)hipStreamSynchronize(qkx::g_stream);
+    if (f->data) (void)hipFree(f->data);
+    if (f->nrm) (void)hipFree(f->nrm);
+    delete f;
+    return QKX_SUCCESS;
+}
+
+int qkx_spinor_zero(qkx_spinor *f)
+{
+    QKX_CHECK_ARG(f, "qkx_spinor_zero: null field");
+    int n = f->nsites;
+    return dispatch_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((spinor_zero<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream, S, n);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+}
+
+int qkx_spinor_upload(qkx_spinor *f, const double *psi)
+{
+    QKX_CHECK_ARG(f && psi, "qkx_spinor_upload: null argument");
+    size_t bytes = sizeof(double) * 24 * (size_t)f->V;
+    void *st = stage(bytes);
+    if (!st) { qkx::set_error("qkx_spinor_upload: staging allocation failed"); return QKX_ERR_NOMEM; }
+    QKX_HIP(hipMemcpyAsync(st, psi, bytes, hipMemcpyHostToDevice, qkx::g_stream));
+    Geom g = qkx::make_geom(f->L, 1);
+    int par0 = f->subset == QKX_SUBSET_ODD ? 1 : 0;
+    int n = f->nsites;
+    int rc = dispatch_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((spinor_from_aos<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
+                           S, (const double2 *)st, g, par0, n);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+    if (rc) return rc;
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+int qkx_spinor_download(const qkx_spinor *f, double *psi)
+{
+    QKX_CHECK_ARG(f && psi, "qkx_spinor_download: null argument");
+    size_t bytes = sizeof(double) * 24 * (size_t)f->V;
+    void *st = stage(bytes);
+    if (!st) { qkx::set_error("qkx_spinor_download: staging allocation failed"); return QKX_ERR_NOMEM; }
+    Geom g = qkx::make_geom(f->L, 1);
+    int par0 = f->subset == QKX_SUBSET_ODD ? 1 : 0;
+    int n = f->nsites;
+    int zero_other = f->subset != QKX_SUBSET_FULL;
+    int rc = dispatch_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((spinor_to_aos<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
+                           (double2 *)st, S, g, par0, n, zero_other);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+    if (rc) return rc;
+    QKX_HIP(hipMemcpyAsync(psi, st, bytes, hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+// Copy with precision conversion.  Full<->full and subset<->subset (same parity)
+// copy everything; full -> subset takes that parity; subset -> full writes only
+// that parity of the destination.
+int qkx_spinor_copy(qkx_spinor *dst, const qkx_spinor *src)
+{
+    QKX_CHECK_ARG(dst && src, "qkx_spinor_copy: null argument");
+    for (int m = 0; m < 4; m++) QKX_CHECK_ARG(dst->L[m] == src->L[m], "qkx_spinor_copy: lattices differ");
+    int n, dshift = 0, sshift = 0;
+    if (dst->subset == src->subset) {
+        n = dst->nsites;
+    } else if (dst->subset != QKX_SUBSET_FULL && src->subset == QKX_SUBSET_FULL) {
+        n = dst->nsites;
+        sshift = dst->subset == QKX_SUBSET_ODD ? src->Vh : 0;
+    } else if (dst->subset == QKX_SUBSET_FULL && src->subset != QKX_SUBSET_FULL) {
+        n = src->nsites;
+        dshift = src->subset == QKX_SUBSET_ODD ? dst->Vh : 0;
+    } else {
+        qkx::set_error("qkx_spinor_copy: even and odd subsets cannot be copied onto each other");
+        return QKX_ERR_ARG;
+    }
+    return dispatch_spinor(dst, [&](auto SD) {
+        return dispatch_spinor(src, [&](auto SS) {
+            hipLaunchKernelGGL((spinor_convert<decltype(SD), decltype(SS)>), dim3((n + 255) / 256), dim3(256), 0,
+                               qkx::g_stream, SD, SS, n, dshift, sshift);
+            QKX_HIP(hipGetLastError());
+            return QKX_SUCCESS;
+        });
+    });
+}
+
+int qkx_gauge_create(qkx_gauge **out, const int L[4], int prec, int recon, int boundaryT)
+{
+    int rc = qkx::ensure_init();
+    if (rc) return rc;
+    QKX_CHECK_ARG(out && L, "qkx_gauge_create: null argument");
+    QKX_CHECK_ARG(valid_dims(L), "qkx_gauge_create: every extent must be even and >= 2");
+    QKX_CHECK_ARG(recon == QKX_RECON_18 || recon == QKX_RECON_12, "qkx_gauge_create: recon must be 18 or 12");
+    QKX_CHECK_ARG(prec >= QKX_PREC_DOUBLE && prec <= QKX_PREC_BF16, "qkx_gauge_create: bad precision");
+    QKX_CHECK_ARG(!((prec == QKX_PREC_HALF || prec == QKX_PREC_BF16) && recon == QKX_RECON_18),
+                  "qkx_gauge_create: 16-bit gauge storage needs recon 12");
+    QKX_CHECK_ARG(boundaryT == 1 || boundaryT == -1, "qkx_gauge_create: boundaryT must be +1 or -1");
+    auto *g = new qkx_gauge_s();
+    for (int m = 0; m < 4; m++) g->L[m] = L[m];
+    g->V = L[0] * L[1] * L[2] * L[3];
+    g->Vh = g->V / 2;
+    g->prec = prec;
+    g->recon = recon;
+    g->bc = boundaryT;
+    g->inner = nullptr;
+    size_t per = (prec == QKX_PREC_DOUBLE ? 16 : prec == QKX_PREC_SINGLE ? 8 : 4) * (recon == 18 ? 9 : 6);
+    g->bytes = per * 4 * (size_t)g->V;
+    if (hipMalloc(&g->data, g->bytes) != hipSuccess) {
+        delete g;
+        qkx::set_error("qkx_gauge_create: out of device memory");
+        return QKX_ERR_NOMEM;
+    }
+    *out = g;
+    return QKX_SUCCESS;
+}
+
+int qkx_gauge_destroy(qkx_gauge *g)
+{
+    if (!g) return QKX_SUCCESS;
+    (void)hipStreamSynchronize(qkx::g_stream);
+    if (g->inner) qkx_gauge_destroy(g->inner);
+    if (g->data) (void)hipFree(g->data);
+    delete g;
+    return QKX_SUCCESS;
+}
+
+int qkx_gauge_upload(qkx_gauge *g, const double *U)
+{
+    QKX_CHECK_ARG(g && U, "qkx_gauge_upload: null argument");
+    size_t bytes = sizeof(double) * 72 * (size_t)g->V;
+    void *st = stage(bytes);
+    if (!st) { qkx::set_error("qkx_gauge_upload: staging allocation failed"); return QKX_ERR_NOMEM; }
+    QKX_HIP(hipMemcpyAsync(st, U, bytes, hipMemcpyHostToDevice, qkx::g_stream));
+    Geom geo = qkx::make_geom(g->L, g->bc);
+    dim3 grid((g->V + 255) / 256), blk(256);
+    const double2 *s2 = (const double2 *)st;
+    if (g->prec == QKX_PREC_DOUBLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    else if (g->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    else if (g->prec == QKX_PREC_SINGLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    else if (g->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    else if (g->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    else hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    QKX_HIP(hipGetLastError());
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src)
+{
+    QKX_CHECK_ARG(dst && src, "qkx_gauge_convert: null argument");
+    QKX_CHECK_ARG(src->prec == QKX_PREC_DOUBLE && src->recon == 18, "qkx_gauge_convert: source must be FP64 recon 18");
+    QKX_CHECK_ARG(dst->V == src->V, "qkx_gauge_convert: lattices differ");
+    dst->bc = src->bc;
+    dim3 grid((dst->V + 255) / 256), blk(256);
+    const double2 *s2 = (const double2 *)src->data;
+    int V = dst->V;
+    if (dst->prec == QKX_PREC_DOUBLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 18>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    else if (dst->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    else if (dst->prec == QKX_PREC_SINGLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 18>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    else if (dst->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    else if (dst->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    else hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
+} // extern "C"

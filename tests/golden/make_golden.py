@@ -1,0 +1,146 @@
+"""Write the golden fixtures in tests/golden/ from the reference's own code.
+
+Runs oracle/_ref/ref_harness (the reference's lib/{lattice_util,util,blas,
+blas_qkxTM,solvers}.cpp compiled by oracle/ref/Makefile) on synthetic inputs
+from the library's generator and stores, per operator output y:
+  * |y|^2,
+  * <r_k, y> for four seeded Gaussian vectors r_k (a random projection: any
+    wrong site, spin, colour or sign changes it),
+  * the spinors of 8 seeded sites,
+and, for a handful of operators on the 4^4 lattice, the whole output vector.
+
+Inputs are not stored: they are regenerated from (L, eps, seeds) by the
+library's counter-hash generator; a hash of them is kept to detect drift.
+
+usage (in the build container, after __graft_entry__.build()):
+    python tests/golden/make_golden.py
+"""
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import qkxtm_mpi_amd as q  # noqa: E402  (generator only, host code)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from golden_util import input_hash, probes, summarize  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+FULL_OPS = ["tm", "tmDag", "tm_chiral", "tm_xx_EE", "tm_xxDagTm_xx_OO", "tm_xx_chiral_OO"]
+
+# (tag, L, eps, kappa, mu, twistSign, boundaryT, gauge seed, psi seed, phi seed)
+CASES = [
+    ("a4444", (4, 4, 4, 4), 0.4, 0.156, 0.01, +1, -1, 11, 12, 13),
+    ("b4648", (4, 6, 4, 8), 0.3, 0.1632, 0.004, -1, +1, 21, 22, 23),
+    ("c4648", (4, 6, 4, 8), 0.5, 0.17, 0.05, +1, -1, 31, 32, 33),
+]
+# config-1 of the benchmark set: 8^3x16, eps 0.4, kappa 0.156, mu 0.01, Z2 source
+CG_CASE = ("cfg1", (8, 8, 8, 16), 0.4, 0.156, 0.01, +1, -1, 1, 2, 3)
+
+
+def write_input(path, L, kappa, mu, ts, bt, tol, maxit, U, psi, phi):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<4i", *L))
+        f.write(struct.pack("<dd", kappa, mu))
+        f.write(struct.pack("<ii", ts, bt))
+        f.write(struct.pack("<d", tol))
+        f.write(struct.pack("<i", maxit))
+        f.write(U.tobytes())
+        f.write(psi.tobytes())
+        f.write(phi.tobytes())
+
+
+def read_records(path):
+    b = open(path, "rb").read()
+    o, recs = 0, {}
+    while o < len(b):
+        name = b[o:o + 48].split(b"\0")[0].decode()
+        n = struct.unpack("<q", b[o + 48:o + 56])[0]
+        o += 56
+        recs[name] = np.frombuffer(b[o:o + 8 * n], dtype=np.float64).copy()
+        o += 8 * n
+    return recs
+
+
+def run_ops_case(case, tmpdir):
+    tag, L, eps, kappa, mu, ts, bt, sg, sp, sf = case
+    U = q.gen_gauge(L, eps, sg)
+    psi = q.gen_gaussian(L, sp)
+    phi = q.gen_gaussian(L, sf)
+    fin = os.path.join(tmpdir, "in.bin")
+    fout = os.path.join(tmpdir, "out.bin")
+    write_input(fin, L, kappa, mu, ts, bt, 1e-10, 100, U, psi, phi)
+    subprocess.run([HARNESS, "ops", fin, fout], check=True, stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    R, sites = probes(L)
+    data = {"L": np.array(L), "params": np.array([eps, kappa, mu, ts, bt, sg, sp, sf]),
+            "input_hash": np.array(input_hash(U, psi, phi)), "sites": sites}
+    names = []
+    for name, v in recs.items():
+        if v.size == 1:
+            data["scalar_" + name] = v
+            continue
+        y = v.view(np.complex128)
+        s = summarize(y, R, sites)
+        data["norm2_" + name] = np.array(s["norm2"])
+        data["proj_" + name] = s["proj"]
+        data["site_" + name] = s["sites"]
+        if tag == "a4444" and name in FULL_OPS:
+            data["full_" + name] = y.reshape(-1, 4, 3)
+        names.append(name)
+    data["names"] = np.array(names)
+    np.savez_compressed(os.path.join(OUT, f"ops_{tag}.npz"), **data)
+    print(f"ops_{tag}.npz: {len(names)} operator outputs")
+
+
+def run_cg_case(tmpdir):
+    tag, L, eps, kappa, mu, ts, bt, sg, sp, sf = CG_CASE
+    U = q.gen_gauge(L, eps, sg)
+    src = q.gen_z2(L, sp)
+    phi = np.zeros_like(src)
+    fin = os.path.join(tmpdir, "in.bin")
+    fout = os.path.join(tmpdir, "out.bin")
+    # ops on the config-1 field (tm of the Z2 source)
+    write_input(fin, L, kappa, mu, ts, bt, 1e-14, 20000, U, src, phi)
+    subprocess.run([HARNESS, "ops", fin, fout], check=True, stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    R, sites = probes(L)
+    data = {"L": np.array(L), "params": np.array([eps, kappa, mu, ts, bt, sg, sp, sf]),
+            "input_hash": np.array(input_hash(U, src)), "sites": sites}
+    for name in ("tm", "tmDag", "tmDagtm", "tm_xxDagTm_xx_EE"):
+        s = summarize(recs[name].view(np.complex128), R, sites)
+        data["norm2_" + name] = np.array(s["norm2"])
+        data["proj_" + name] = s["proj"]
+        data["site_" + name] = s["sites"]
+    # reference CG on D^dag D x = D^dag src, tol 1e-14 (bin/test_invert_prop.cpp:91-94)
+    subprocess.run([HARNESS, "cg", fin, fout], check=True, stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    x = recs["cg_x"].view(np.complex128)
+    Dx = recs["cg_Dx"].view(np.complex128)
+    s = summarize(x, R, sites)
+    data["norm2_cg_x"] = np.array(s["norm2"])
+    data["proj_cg_x"] = s["proj"]
+    data["site_cg_x"] = s["sites"]
+    res = np.linalg.norm(Dx - src.ravel()) / np.linalg.norm(src.ravel())
+    data["cg_true_residual"] = np.array(res)
+    np.savez_compressed(os.path.join(OUT, "cg_cfg1.npz"), **data)
+    print(f"cg_cfg1.npz: reference CG true residual {res:.3e}")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit(f"{HARNESS} missing: make -C oracle/ref (needs the reference sources)")
+    with tempfile.TemporaryDirectory() as d:
+        for c in CASES:
+            run_ops_case(c, d)
+        run_cg_case(d)
+
+
+if __name__ == "__main__":
+    main()

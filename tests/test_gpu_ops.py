@@ -1,0 +1,145 @@
+"""GPU parity: every reference operator through the C ABI against the CPU
+oracle on the same seeded inputs.  FP64 bar: relative L2 <= 1e-13."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from refops import ALL_OPS, gpu_op, oracle_op, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+
+
+def _setup(q, L, eps, seed, bt):
+    U = q.gen_gauge(L, eps, seed)
+    psi = q.gen_gaussian(L, seed + 1)
+    phi = q.gen_gaussian(L, seed + 2)
+    return U, psi, phi
+
+
+@pytest.mark.parametrize("L,eps,kappa,mu,ts,bt", [
+    ((4, 6, 4, 8), 0.4, 0.156, 0.01, +1, -1),
+    ((8, 4, 6, 4), 0.3, 0.1632, 0.004, -1, +1),
+    ((8, 8, 8, 16), 0.4, 0.156, 0.01, +1, -1),
+])
+def test_all_ops_fp64_in_kernel_bc(q, L, eps, kappa, mu, ts, bt):
+    """Gauge uploaded without the sign; the kernel applies boundaryT at the t wrap."""
+    print(f"case L={L} bt={bt} ts={ts}", flush=True)
+    U, psi, phi = _setup(q, L, eps, 5, bt)
+    Ub = orc.apply_bc(U, L, bt)
+    u = q.Gauge(L, q.DOUBLE, q.RECON_18, boundaryT=bt, host=U)
+    p = q.params(L, kappa, mu, ts, bt)
+    ps, ph = q.Spinor(L, host=psi), q.Spinor(L, host=phi)
+    for name in ALL_OPS:
+        ref = oracle_op(name, Ub, psi, phi, L, kappa, mu, ts)
+        got = gpu_op(q, name, u, ps, ph, p)
+        assert rel(got, ref) <= TOL64, (name, rel(got, ref))
+
+
+def test_ops_with_signed_links_match(q):
+    """Links that already carry the boundary sign (reference drivers) with
+    boundaryT=+1 in the kernel give the same operator."""
+    L, kappa, mu, ts = (4, 4, 6, 8), 0.156, 0.01, 1
+    U, psi, phi = _setup(q, L, 0.4, 9, -1)
+    Ub = orc.apply_bc(U, L, -1)
+    u = q.Gauge(L, boundaryT=+1, host=Ub)
+    p = q.params(L, kappa, mu, ts, -1)
+    ps, ph = q.Spinor(L, host=psi), q.Spinor(L, host=phi)
+    for name in ("tm", "tmDag", "tm_chiral", "tm_xx_EE", "tm_xxDagTm_xx_OO", "applyDslash_xx_OE"):
+        ref = oracle_op(name, Ub, psi, phi, L, kappa, mu, ts)
+        assert rel(gpu_op(q, name, u, ps, ph, p), ref) <= TOL64, name
+
+
+def test_recon12_fp64(q):
+    L, kappa, mu = (4, 6, 4, 8), 0.1632, 0.004
+    U, psi, phi = _setup(q, L, 0.3, 13, -1)
+    Ub = orc.apply_bc(U, L, -1)
+    u = q.Gauge(L, q.DOUBLE, q.RECON_12, -1, host=U)
+    p = q.params(L, kappa, mu, 1, -1)
+    ps, ph = q.Spinor(L, host=psi), q.Spinor(L, host=phi)
+    for name in ("tm", "tmDag", "tm_xxDagTm_xx_EE"):
+        ref = oracle_op(name, Ub, psi, phi, L, kappa, mu, 1)
+        assert rel(gpu_op(q, name, u, ps, ph, p), ref) <= 1e-13, name
+
+
+def test_parity_subset_fields(q):
+    """tm_xx on even-only fields equals the full-field version."""
+    L, kappa, mu = (8, 4, 4, 8), 0.16, 0.02
+    U, psi, phi = _setup(q, L, 0.4, 17, -1)
+    Ub = orc.apply_bc(U, L, -1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    p = q.params(L, kappa, mu, 1, -1)
+    for parity, sub in ((q.EE, q.EVEN), (q.OO, q.ODD)):
+        for normal in (0, 1):
+            ref = orc.tm_xx(psi, Ub, L, kappa, mu, 1, orc.UKQCD, parity, normal)
+            x = q.Spinor(L, subset=sub, host=psi)
+            y = q.Spinor(L, subset=sub)
+            (y.tm_xxDagTm_xx if normal else y.tm_xx)(x, u, p, parity)
+            assert rel(y.download(), ref) <= TOL64
+
+
+@pytest.mark.parametrize("prec,recon,tol", [
+    ("SINGLE", 18, 2e-6), ("SINGLE", 12, 2e-6), ("HALF", 12, 2e-3), ("BF16", 12, 2e-2)])
+def test_low_precision_tm(q, prec, recon, tol):
+    """Low-precision storage: tm and the Schur normal operator within the
+    precision's rounding (FP32 ~1e-7, FP16 ~5e-4, BF16 ~4e-3 per element)."""
+    L, kappa, mu = (8, 8, 4, 8), 0.1632, 0.004
+    P = getattr(q, prec)
+    U, psi, phi = _setup(q, L, 0.3, 21, -1)
+    Ub = orc.apply_bc(U, L, -1)
+    u64 = q.Gauge(L, boundaryT=-1, host=U)
+    u = q.Gauge(L, P, recon, -1)
+    u.convert_from(u64)
+    p = q.params(L, kappa, mu, 1, -1)
+    x = q.Spinor(L, P, host=psi)
+    y = q.Spinor(L, P)
+    y.tm(x, u, p)
+    assert rel(y.download(), orc.tm(psi, Ub, L, kappa, mu, 1)) <= tol
+    xe = q.Spinor(L, P, q.EVEN, host=psi)
+    ye = q.Spinor(L, P, q.EVEN)
+    ye.tm_xxDagTm_xx(xe, u, p, q.EE)
+    assert rel(ye.download(), orc.tm_xx(psi, Ub, L, kappa, mu, 1, orc.UKQCD, orc.EE, 1)) <= 4 * tol
+
+
+def test_blas(q):
+    L = (4, 4, 4, 8)
+    a = q.gen_gaussian(L, 31)
+    b = q.gen_gaussian(L, 32)
+    x, y = q.Spinor(L, host=a), q.Spinor(L, host=b)
+    assert abs(q.reCdotXdagY(x, y) - orc.redot(a, b)) <= 1e-13 * np.sqrt(orc.norm2(a) * orc.norm2(b))
+    assert abs(x.norm2() - orc.norm2(a)) <= 1e-14 * orc.norm2(a)
+    q.X_eq_X_p_aY(x, 0.37, y)
+    assert rel(x.download(), orc.axpy(a, 0.37, b)) <= 1e-15
+    x.upload(a)
+    q.X_eq_aX_p_Y(x, -1.25, y)
+    assert rel(x.download(), orc.xpay(a, -1.25, b)) <= 1e-15
+
+
+def test_upload_download_roundtrip_and_subsets(q):
+    L = (4, 6, 8, 4)
+    a = q.gen_gaussian(L, 41)
+    for prec in (q.DOUBLE,):
+        f = q.Spinor(L, prec, host=a)
+        assert np.array_equal(f.download(), a)
+    e = q.Spinor(L, q.DOUBLE, q.EVEN, host=a)
+    got = e.download()
+    V = np.prod(L)
+    x = np.arange(V)
+    c = np.stack(np.unravel_index(x, (L[3], L[2], L[1], L[0])), 1)
+    odd = (c.sum(1) & 1).astype(bool)
+    assert np.array_equal(got[~odd], a[~odd]) and not np.any(got[odd])
+
+
+def test_argument_errors(q):
+    L = (4, 4, 4, 4)
+    with pytest.raises(q.QkxError):
+        q.Spinor((3, 4, 4, 4))
+    u = q.Gauge(L, host=q.gen_gauge(L, 0.3, 1))
+    a, b = q.Spinor(L), q.Spinor(L, subset=q.EVEN)
+    with pytest.raises(q.QkxError):
+        b.applyDslash(a, u)                     # full-lattice hop needs full fields
+    with pytest.raises(q.QkxError):
+        a.tm(a, u, q.params(L, 0.1, 0.1))       # out must differ from in
+    with pytest.raises(q.QkxError):
+        a.applyDslash_xx(a.__class__(L), u, q.EE)  # EE is not a hopping parity

@@ -1,0 +1,90 @@
+"""GPU solvers: the reference CG through the C ABI against the oracle CG, and
+the true-residual solvers (full / even-odd, FP64 / FP32 / FP16 / BF16 inner
+with FP64 reliable updates) against the FP64 reference solution."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from golden_util import probes, summarize
+from refops import rel
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# config-1: 8^3x16, eps 0.4, kappa 0.156, mu 0.01, antiperiodic t, Z2 source
+L1 = (8, 8, 8, 16)
+K1, M1 = 0.156, 0.01
+
+
+@pytest.fixture(scope="module")
+def cfg1(q):
+    U = q.gen_gauge(L1, 0.4, 1)
+    src = q.gen_z2(L1, 2)
+    Ub = orc.apply_bc(U, L1, -1)
+    return U, Ub, src
+
+
+def test_reference_cg_matches_oracle(q, cfg1):
+    """CG::operator() semantics: D^dag D x = D^dag src, iterated-residual stop."""
+    U, Ub, src = cfg1
+    b = orc.tm(src, Ub, L1, K1, M1, 1, orc.UKQCD, orc.TM_DAG)
+    x_ref, k_ref, _ = orc.cg(b, Ub, L1, K1, M1, 1, 1e-10, 5000)
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1, tol=1e-10, maxIter=5000)
+    x, bd = q.Spinor(L1), q.Spinor(L1, host=b)
+    st = q.CG(p, u)(x, bd)
+    print("iterations gpu", st.iterations, "oracle", k_ref, flush=True)
+    assert st.converged and abs(st.iterations - k_ref) <= 2
+    assert rel(x.download(), x_ref) <= 1e-8
+
+
+def test_reference_cg_maxiter_status(q, cfg1):
+    U, Ub, src = cfg1
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1, tol=1e-12, maxIter=7)
+    x, bd = q.Spinor(L1), q.Spinor(L1, host=src)
+    st = q.CG(p, u)(x, bd)
+    assert st.iterations == 7 and not st.converged
+
+
+@pytest.mark.parametrize("eo,prec", [(False, "DOUBLE"), (True, "DOUBLE"), (True, "SINGLE"), (True, "HALF"),
+                                     (True, "BF16"), (False, "SINGLE")])
+def test_solve_true_residual_and_reference_solution(q, cfg1, eo, prec):
+    """True residual <= 1e-10 (recomputed by the oracle in FP64) and the
+    solution within 1e-9 of the reference CG solution at tol 1e-14."""
+    U, Ub, src = cfg1
+    g = np.load(os.path.join(GOLD, "cg_cfg1.npz"), allow_pickle=False)
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1)
+    x, b = q.Spinor(L1), q.Spinor(L1, host=src)
+    st = q.solve(x, b, u, p, eo=eo, inner_prec=getattr(q, prec), tol=1e-10, maxIter=20000)
+    xs = x.download()
+    res = rel(orc.tm(xs, Ub, L1, K1, M1, 1), src)
+    print(f"eo={eo} prec={prec} iters={st.iterations} rel_updates={st.reliable_updates} res={res:.2e}",
+          flush=True)
+    assert st.converged and res <= 1e-10 and abs(st.true_residual - res) <= 1e-3 * res + 1e-14
+    R, sites = probes(L1)
+    s = summarize(xs, R, sites)
+    n = np.sqrt(float(g["norm2_cg_x"]))
+    assert abs(np.sqrt(s["norm2"]) - n) <= 1e-9 * n
+    assert rel(s["sites"], g["site_cg_x"]) <= 1e-8
+
+
+def test_solve_nonfinite_source(q):
+    L = (4, 4, 4, 4)
+    u = q.Gauge(L, host=q.gen_gauge(L, 0.3, 3))
+    src = q.gen_z2(L, 4)
+    src[5, 1, 2] = np.nan
+    x, b = q.Spinor(L), q.Spinor(L, host=src)
+    with pytest.raises(q.QkxError):
+        q.solve(x, b, u, q.params(L, 0.15, 0.01), tol=1e-10)
+
+
+def test_solve_zero_source(q):
+    L = (4, 4, 4, 4)
+    u = q.Gauge(L, host=q.gen_gauge(L, 0.3, 3))
+    x, b = q.Spinor(L, host=q.gen_z2(L, 1)), q.Spinor(L)
+    st = q.solve(x, b, u, q.params(L, 0.15, 0.01), tol=1e-10)
+    assert st.converged and not np.any(x.download())
