@@ -112,6 +112,17 @@ int  qkx_timer_stop(double *ms);
  * hopping kernels since the last reset. */
 int  qkx_prof_enable(int on);
 int  qkx_prof_read(int *launches, double *ms, double *bytes);
+/* the same restricted to hopping kernels on spinors of one precision */
+int  qkx_prof_read_prec(int prec, int *launches, double *ms, double *bytes);
+int  qkx_prof_reset(void);
+/* Launch shape of the hopping kernel: workgroup size (64/128/256), whether
+ * workgroups are remapped so that each XCD walks a contiguous site range, and
+ * the software-pipeline depth (1 or 2 hops in flight) of the FP64 and of the
+ * FP32/FP16/BF16 kernels. */
+int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
+/* CG iterations enqueued between host reads of the solver state (default 16;
+ * 1 gives no launches past the stopping iteration). */
+int  qkx_set_cg_batch(int iterations);
 
 /* ---- fields (Gauge / Vector, include/lattice_util.h:80-212) ---------------- */
 /* Gauge::Gauge lib/lattice_util.cpp:649-678.  boundaryT = -1 multiplies the

@@ -1,0 +1,131 @@
+/*
+ * lattice_util.h -- drop-in counterpart of the reference's
+ * include/lattice_util.h (classes Vector and Gauge, hot-path methods only),
+ * backed by libqkx_hip.so.
+ *
+ * The host arrays M keep the reference layout (Vector::M Complex[V][4][3],
+ * Gauge::M Complex[V][4][3][3]) and stay authoritative at the API boundary:
+ * every operator uploads its inputs, runs on the GPU and downloads its
+ * output into this->M.  Drivers may therefore read and write M directly, as
+ * the reference drivers do (bin/test_invert.cpp:89).  The gauge is uploaded
+ * on every operator call unless Gauge::constGauge is true and it was uploaded
+ * before (the reference sets constGauge once the links are final,
+ * bin/test_invert.cpp:110).  For the fast path keep fields on the device with
+ * the C API in qkx_hip.h.
+ */
+#ifndef QKXTM_DROPIN_LATTICE_UTIL_H
+#define QKXTM_DROPIN_LATTICE_UTIL_H
+
+#include "qkxTM.h"
+
+struct qkx_spinor_s;
+struct qkx_gauge_s;
+
+namespace qkxTM {
+
+class Gauge;
+class Vector;
+
+enum PARITY { EO, OE, OO, EE };   /* reference include/lattice_util.h:23 */
+
+class LatticeGeometry {
+  public:
+    long int V4d;
+    long int V3d;
+    long int lV4d;
+    long int lV3d;
+    int L[NDIM];
+    int P[NDIM];
+    int lL[NDIM];
+    int procPos[NDIM];
+    int rank;
+    int Nprocs;
+    bool initialized;
+
+    LatticeGeometry(LatticeInfo *latInfo);
+    virtual ~LatticeGeometry() {}
+};
+
+class Vector : public LatticeGeometry {
+  public:
+    long int size;
+    long int size_total;
+    Complex (*M)[NSPINS][NCOLORS];
+
+    Vector(LatticeInfo *latInfo);
+    virtual ~Vector();
+
+    void zero();
+    void copy(Vector &src);
+    double norm2();
+
+    void applyDslash(Vector &psi, Gauge &u);
+    void applyDslash_chiral(Vector &psi, Gauge &u);
+    void applyDslash_xx(Vector &psi, Gauge &u, PARITY parity);
+    void applyDslash_xx_chiral(Vector &psi, Gauge &u, PARITY parity);
+    void applyDeltaAddDslash(Vector &dslash, Vector &psi, LatticeInfo *latInfo);
+    void applyTwistAddDslash(Vector &dslash, Vector &psi, LatticeInfo *latInfo);
+    void applyTwistAddDslash_chiral(Vector &dslash, Vector &psi, LatticeInfo *latInfo);
+    void applyTwistAddDslash_PC(Vector &dslash, Vector &psi, LatticeInfo *latInfo, PARITY parity);
+    void applyTwistAddDslash_PC_chiral(Vector &dslash, Vector &psi, LatticeInfo *latInfo, PARITY parity);
+
+    void applyDslashDag(Vector &psi, Gauge &u);
+    void applyTwistDagAddDslashDag(Vector &dslashDag, Vector &psi, LatticeInfo *latInfo);
+    void applyTwist(Vector &psi, LatticeInfo *latInfo);
+    void applyTwistDag(Vector &psi, LatticeInfo *latInfo);
+    void applyTwistInv(Vector &psi, LatticeInfo *latInfo);
+    void applyTwistInv_chiral(Vector &psi, LatticeInfo *latInfo);
+    void applyTwistDagInv(Vector &psi, LatticeInfo *latInfo);
+    void applyGamma5(Vector &psi, LatticeInfo *latInfo);
+    void applyGamma5_chiral(Vector &psi, LatticeInfo *latInfo);
+
+    void tm(Vector &psi, Gauge &u, LatticeInfo *latInfo);
+    void tm_chiral(Vector &psi, Gauge &u, LatticeInfo *latInfo);
+    void tmDag(Vector &psi, Gauge &u, LatticeInfo *latInfo);
+    void tmDagtm(Vector &psi, Vector &vec_tmp, Gauge &u, LatticeInfo *latInfo);
+    void Wilson(Vector &psi, Gauge &u, LatticeInfo *latInfo);
+
+    void tm_xx(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
+    void tm_xxDagTm_xx(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
+    void tm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
+    void tm_xxDagTm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
+
+    /* device mirror (library internal) */
+    qkx_spinor_s *dev();
+    void upload();
+    void download();
+
+  private:
+    qkx_spinor_s *dev_;
+    Vector(const Vector &);
+    Vector &operator=(const Vector &);
+};
+
+class Gauge : public LatticeGeometry {
+  public:
+    long int size;
+    long int size_total;
+    bool constGauge;
+    Complex (*M)[NDIM][NCOLORS][NCOLORS];
+
+    Gauge(LatticeInfo *latInfo);
+    virtual ~Gauge();
+
+    void zero();
+    void copy(Gauge &src);
+    double norm2();
+    void applyBoundaryCondition(LatticeInfo *latInfo);
+
+    /* device mirror, uploaded as needed (see constGauge above) */
+    qkx_gauge_s *dev();
+
+  private:
+    qkx_gauge_s *dev_;
+    bool uploaded_;
+    Gauge(const Gauge &);
+    Gauge &operator=(const Gauge &);
+};
+
+} // namespace qkxTM
+
+#endif

@@ -69,6 +69,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
+    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P],
     "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
@@ -356,7 +357,21 @@ def prof_enable(on=True):
     _check(lib().qkx_prof_enable(int(on)), "qkx_prof_enable")
 
 
-def prof_read():
+def prof_read(prec=-1):
+    """(launches, ms, bytes) of the instrumented hopping kernels since the last
+    reset, optionally only those on spinors of precision ``prec``."""
     n, ms, by = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-    _check(lib().qkx_prof_read(ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)), "qkx_prof_read")
+    _check(lib().qkx_prof_read_prec(prec, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)), "qkx_prof_read")
     return n.value, ms.value, by.value
+
+
+def prof_reset():
+    _check(lib().qkx_prof_reset(), "qkx_prof_reset")
+
+
+def set_cg_batch(iterations=16):
+    _check(lib().qkx_set_cg_batch(iterations), "qkx_set_cg_batch")
+
+
+def set_tuning(xcd_remap=True, block=256, depth64=1, depth32=2):
+    _check(lib().qkx_set_tuning(int(xcd_remap), block, depth64, depth32), "qkx_set_tuning")

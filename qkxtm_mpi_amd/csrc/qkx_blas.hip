@@ -5,10 +5,10 @@
 // MPI_Allreduce), X_eq_X_p_aY :31-42, X_eq_aX_p_Y :45-56.
 //
 // Reductions: every thread accumulates in FP64, a wavefront reduces with
-// cross-lane shuffles, the workgroup through LDS, and the last workgroup to
-// finish (atomic ticket) adds the per-workgroup partials in double-double
-// (compensated) arithmetic and resets the ticket, so one launch gives the
-// final value in device memory and no memset is needed between launches.
+// cross-lane shuffles, the workgroup through LDS and writes one partial; a
+// second one-workgroup launch adds the partials in double-double (compensated)
+// arithmetic in a fixed order.  No atomics and no device-wide fences (which
+// write back the L2 of every XCD), and the result is reproducible.
 #include "qkx_internal.h"
 #include "qkx_core.h"
 
@@ -41,68 +41,24 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Reduce NV values per thread across the grid; result[k] = sum of v[k].
-template <int NV>
-__device__ void grid_reduce(double v[NV], double *partial, unsigned int *counter, double *result)
+// Per-workgroup partial of v (wavefront shuffles, then LDS).
+__device__ void block_partial(double v, double *dst)
 {
-    __shared__ double lds[NV][kRedThreads / 64];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double w = wave_sum(v[k]);
-        if (lane == 0) lds[k][wid] = w;
-    }
+    __shared__ double lds[kRedThreads / 64];
+    double w = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[k][w];
-            partial[(size_t)k * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        unsigned int t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    __shared__ double lh[NV][kRedThreads], ll[NV][kRedThreads];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        DD acc{0.0, 0.0};
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-            acc = dd_add(acc, DD{partial[(size_t)k * gridDim.x + b], 0.0});
-        lh[k][threadIdx.x] = acc.hi;
-        ll[k][threadIdx.x] = acc.lo;
-    }
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-#pragma unroll
-            for (int k = 0; k < NV; k++) {
-                DD a{lh[k][threadIdx.x], ll[k][threadIdx.x]};
-                DD b{lh[k][threadIdx.x + st], ll[k][threadIdx.x + st]};
-                DD c = dd_add(a, b);
-                lh[k][threadIdx.x] = c.hi;
-                ll[k][threadIdx.x] = c.lo;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) result[k] = lh[k][0] + ll[k][0];
-        *counter = 0u;
+        double s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += lds[i];
+        *dst = s;
     }
 }
 
 template <class S>
-__global__ __launch_bounds__(kRedThreads) void redot_kernel(S x, S y, int n, int mode, double *partial,
-                                                            unsigned int *counter, double *result)
+__global__ __launch_bounds__(kRedThreads) void redot_kernel(S x, S y, int n, int mode, double *partial)
 {
-    double acc[1] = {0.0};
+    double acc = 0.0;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         Cx<typename S::Real> a[12];
         x.load(s, a);
@@ -111,14 +67,34 @@ __global__ __launch_bounds__(kRedThreads) void redot_kernel(S x, S y, int n, int
             y.load(s, b);
 #pragma unroll
             for (int k = 0; k < 12; k++)
-                acc[0] += (double)a[k].re * (double)b[k].re + (double)a[k].im * (double)b[k].im;
+                acc += (double)a[k].re * (double)b[k].re + (double)a[k].im * (double)b[k].im;
         } else {
 #pragma unroll
             for (int k = 0; k < 12; k++)
-                acc[0] += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
+                acc += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
         }
     }
-    grid_reduce<1>(acc, partial, counter, result);
+    block_partial(acc, partial + blockIdx.x);
+}
+
+// One workgroup: result = compensated (double-double) sum of partial[0..n).
+__global__ __launch_bounds__(kRedThreads) void finish_kernel(const double *partial, int n, double *result)
+{
+    __shared__ double lh[kRedThreads], ll[kRedThreads];
+    DD acc{0.0, 0.0};
+    for (int b = threadIdx.x; b < n; b += blockDim.x) acc = dd_add(acc, DD{partial[b], 0.0});
+    lh[threadIdx.x] = acc.hi;
+    ll[threadIdx.x] = acc.lo;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            DD c = dd_add(DD{lh[threadIdx.x], ll[threadIdx.x]}, DD{lh[threadIdx.x + st], ll[threadIdx.x + st]});
+            lh[threadIdx.x] = c.hi;
+            ll[threadIdx.x] = c.lo;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *result = lh[0] + ll[0];
 }
 
 // x = ax*x + ay*y   (scalars from host or from device memory *sc)
@@ -167,7 +143,8 @@ int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result)
         using S = decltype(tag);
         S X = make_spinor<S>(x), Y = make_spinor<S>(y ? y : x);
         hipLaunchKernelGGL((redot_kernel<S>), dim3(nb), dim3(kRedThreads), 0, g_stream, X, Y, n, mode,
-                           reduce_scratch(), reduce_counter(), result);
+                           reduce_scratch());
+        hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, g_stream, reduce_scratch(), nb, result);
         QKX_HIP(hipGetLastError());
         return QKX_SUCCESS;
     });

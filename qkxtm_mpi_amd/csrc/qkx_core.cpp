@@ -9,6 +9,7 @@
 
 namespace qkx {
 
+extern int cg_batch_size;
 hipStream_t g_stream = nullptr;
 int g_device = -1;
 int g_cu_count = 0;
@@ -26,6 +27,7 @@ bool g_prof = false;
 struct ProfRec {
     hipEvent_t a, b;
     double bytes;
+    int prec;
 };
 std::vector<ProfRec> g_prof_recs;
 size_t g_prof_used = 0;
@@ -70,11 +72,12 @@ void prof_begin()
     (void)hipEventRecord(g_prof_recs[g_prof_used].a, g_stream);
 }
 
-void prof_end(double bytes)
+void prof_end(double bytes, int prec)
 {
     if (!g_prof) return;
     (void)hipEventRecord(g_prof_recs[g_prof_used].b, g_stream);
     g_prof_recs[g_prof_used].bytes = bytes;
+    g_prof_recs[g_prof_used].prec = prec;
     g_prof_used++;
 }
 
@@ -217,19 +220,49 @@ int qkx_prof_enable(int on)
 
 int qkx_prof_read(int *launches, double *ms, double *bytes)
 {
+    return qkx_prof_read_prec(-1, launches, ms, bytes);
+}
+
+int qkx_prof_read_prec(int prec, int *launches, double *ms, double *bytes)
+{
     QKX_CHECK_ARG(launches && ms && bytes, "qkx_prof_read: null argument");
     QKX_HIP(hipStreamSynchronize(qkx::g_stream));
     double tot = 0, by = 0;
+    int n = 0;
     for (size_t i = 0; i < qkx::g_prof_used; i++) {
+        if (prec >= 0 && qkx::g_prof_recs[i].prec != prec) continue;
         float f = 0;
         QKX_HIP(hipEventElapsedTime(&f, qkx::g_prof_recs[i].a, qkx::g_prof_recs[i].b));
         tot += f;
         by += qkx::g_prof_recs[i].bytes;
+        n++;
     }
-    *launches = (int)qkx::g_prof_used;
+    *launches = n;
     *ms = tot;
     *bytes = by;
+    return QKX_SUCCESS;
+}
+
+int qkx_prof_reset(void)
+{
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
     qkx::g_prof_used = 0;
+    return QKX_SUCCESS;
+}
+
+int qkx_set_cg_batch(int iterations)
+{
+    QKX_CHECK_ARG(iterations >= 1 && iterations <= 1024, "qkx_set_cg_batch: 1..1024");
+    qkx::cg_batch_size = iterations;
+    return QKX_SUCCESS;
+}
+
+int qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32)
+{
+    QKX_CHECK_ARG(block == 64 || block == 128 || block == 256, "qkx_set_tuning: block must be 64, 128 or 256");
+    QKX_CHECK_ARG((depth64 == 1 || depth64 == 2) && (depth32 == 1 || depth32 == 2),
+                  "qkx_set_tuning: pipeline depth must be 1 or 2");
+    qkx::set_hop_tuning(xcd_remap != 0, block, depth64, depth32);
     return QKX_SUCCESS;
 }
 

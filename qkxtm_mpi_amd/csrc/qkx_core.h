@@ -9,12 +9,21 @@ namespace qkx {
 
 // hopping kernel with fused epilogue (qkx_dslash.hip):
 //   out(par_out) = cR R + gR i g5 R + cX aux + gX i g5 aux, R = H in or H^dag in
+//   norm_partial (optional): per-workgroup partial sums of |out|^2, count in *nblocks_out
 int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
-        int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out);
+        int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out,
+        double *norm_partial = nullptr, int *nblocks_out = nullptr);
 // site-local combination (qkx_dslash.hip)
 int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
          double cR, double gR, double cX, double gX, int g5out);
-void set_hop_tuning(int xcd_remap, int block);
+void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
+void set_hop_skip(const int *flag);
+// even-odd Schur operator on parity pe (qkx_ops.cpp):
+//   out_pe = [A in - kappa^2 H A^-1 H in]_pe (then g5 if g5out); the optional
+//   norm_partial collects |out|^2 per workgroup of the second hop
+int schur(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, double kappa, double a, double nrm,
+          int basis, int pe, int g5out, qkx_spinor_s *tmp_other, double *norm_partial = nullptr,
+          int *nblocks = nullptr);
 
 // BLAS (qkx_blas.hip): device-resident results, no host synchronisation
 int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);

@@ -21,6 +21,21 @@ namespace {
 
 enum : int { K0 = 0, KP1, KM1, KPI, KMI, K2 };
 
+// Pin the accumulator: the empty asm reads and writes every element of R and
+// clobbers memory, so one direction's arithmetic completes before the next
+// direction's gathers are issued.  Without it the compiler issues all sixteen
+// gathers first (~240 registers of in-flight data, spills for FP64).
+template <class T> __device__ __forceinline__ void pin_acc(Cx<T> R[12])
+{
+    asm volatile("" : "+v"(R[0].re), "+v"(R[0].im), "+v"(R[1].re), "+v"(R[1].im), "+v"(R[2].re),
+                      "+v"(R[2].im), "+v"(R[3].re), "+v"(R[3].im), "+v"(R[4].re), "+v"(R[4].im),
+                      "+v"(R[5].re), "+v"(R[5].im)::"memory");
+    asm volatile("" : "+v"(R[6].re), "+v"(R[6].im), "+v"(R[7].re), "+v"(R[7].im), "+v"(R[8].re),
+                      "+v"(R[8].im), "+v"(R[9].re), "+v"(R[9].im), "+v"(R[10].re), "+v"(R[10].im),
+                      "+v"(R[11].re), "+v"(R[11].im)::"memory");
+}
+#define QKX_SCHED_FENCE() pin_acc(R)
+
 struct Proj {
     int sa[2], a[2], sb[2], b[2];   // h_k = a_k psi[sa_k] + b_k psi[sb_k]
     int kq[4], q[4];                // R_s += q_s xi[kq_s]
@@ -114,19 +129,38 @@ __device__ __forceinline__ void link_mul(const Cx<T> U[9], const Cx<T> h[6], Cx<
         }
 }
 
-// One direction, one sign: load neighbour spins, project, link, collect.
+// Raw operands of one hop (the neighbour spin pairs the projector reads + link).
+template <class SIn, class G> struct HopOps {
+    typename SIn::Raw psi;
+    typename G::Raw link;
+};
+
+template <int B, int MU, int KIND> constexpr int spin_mask()
+{
+    return (uses_spin_pair<B, MU, KIND>(0) ? 1 : 0) | (uses_spin_pair<B, MU, KIND>(1) ? 2 : 0);
+}
+
+template <int B, int MU, int KIND, class SIn, class G>
+__device__ __forceinline__ HopOps<SIn, G> fetch_dir(const SIn &in, int s_nb, const G &U, int s_link)
+{
+    HopOps<SIn, G> o;
+    o.psi = in.template fetch<spin_mask<B, MU, KIND>()>(s_nb);
+    o.link = U.fetch(MU, s_link);
+    return o;
+}
+
+// Project, multiply by the link (or its adjoint), reconstruct, accumulate.
 template <int B, int MU, int KIND, bool DAGL, class SIn, class G, class T>
-__device__ __forceinline__ void hop_dir(Cx<T> R[12], const SIn &in, int s_nb, const G &U,
-                                        int s_link, T sign)
+__device__ __forceinline__ void apply_dir(Cx<T> R[12], const SIn &in, const G &U, const HopOps<SIn, G> &o,
+                                          T sign)
 {
     Cx<T> psi[12];
-    if constexpr (uses_spin_pair<B, MU, KIND>(0)) in.template load_half<0>(s_nb, psi);
-    if constexpr (uses_spin_pair<B, MU, KIND>(1)) in.template load_half<1>(s_nb, psi);
+    in.template unpack<spin_mask<B, MU, KIND>()>(o.psi, psi);
     Cx<T> h[6];
     project_k<B, MU, KIND, 0>(psi, h);
     project_k<B, MU, KIND, 1>(psi, h);
     Cx<T> Ul[9];
-    U.load(MU, s_link, Ul);
+    U.unpack(o.link, Ul);
     if (sign != T(1)) {
 #pragma unroll
         for (int k = 0; k < 6; k++) h[k] = sign * h[k];
@@ -200,8 +234,24 @@ struct HopArgs {
     int nout;
     int nblk8;     // grid size / 8 when XCD remapping is on, else 0
     const int *skip;   // when non-null and *skip != 0 the launch does nothing
+    double *norm_partial;  // when non-null: per-workgroup sum of |out|^2
     Epi epi;
 };
+
+// Sum v over the workgroup; thread 0 writes the result to dst.
+__device__ __forceinline__ void block_partial(double v, double *dst)
+{
+    __shared__ double lds[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[w];
+        *dst = s;
+    }
+}
 
 // Hardware workgroup IDs are dealt round-robin to the 8 XCDs; remap them so
 // that each XCD walks one contiguous range of sites and its private L2 sees
@@ -213,22 +263,30 @@ __device__ __forceinline__ int logical_block(int nblk8)
     return b;
 }
 
-template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX>
+template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH>
 __global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
 {
     using T = typename SOut::Real;
-    if (a.skip && *a.skip) return;
-    const int o = logical_block(a.nblk8) * blockDim.x + threadIdx.x;
-    if (o >= a.nout) return;
+    if (a.skip && *a.skip) return;   // uniform over the grid
     const Geom &g = a.g;
     int p, cb;
-    if (a.par_out < 0) {
+    if (a.par_out >= 0) {
+        p = a.par_out;
+        cb = logical_block(a.nblk8) * blockDim.x + threadIdx.x;
+    } else if (a.nblk8 > 0) {
+        // full lattice: XCD k owns one slab of checkerboard indices and
+        // alternates between the two parities, so the links and spinors of a
+        // slab (both parities) are fetched into one L2 only
+        const int b = blockIdx.x, i = b >> 3;
+        p = i & 1;
+        cb = ((b & 7) * a.nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
+    } else {
+        const int o = blockIdx.x * blockDim.x + threadIdx.x;
         p = (o >= g.Vh) ? 1 : 0;
         cb = o - p * g.Vh;
-    } else {
-        p = a.par_out;
-        cb = o;
     }
+    double nrm2 = 0.0;
+    if (cb < g.Vh) {
     // coordinates: cb = xh + Lh0 * rest,  rest = y + Ly*(z + Lz*t)
     const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
     const int xh = cb % g.Lh0;
@@ -265,14 +323,55 @@ __global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
 
     constexpr int FW = DAG ? 1 : 0;   // projector of the forward hop
     constexpr int BW = DAG ? 0 : 1;   // projector of the backward hop
-    hop_dir<B, 0, FW, false>(R, a.in, in_q + cb_xp, a.U, poff + cb, T(1));
-    hop_dir<B, 0, BW, true>(R, a.in, in_q + cb_xm, a.U, qoff + cb_xm, T(1));
-    hop_dir<B, 1, FW, false>(R, a.in, in_q + cb_yp, a.U, poff + cb, T(1));
-    hop_dir<B, 1, BW, true>(R, a.in, in_q + cb_ym, a.U, qoff + cb_ym, T(1));
-    hop_dir<B, 2, FW, false>(R, a.in, in_q + cb_zp, a.U, poff + cb, T(1));
-    hop_dir<B, 2, BW, true>(R, a.in, in_q + cb_zm, a.U, qoff + cb_zm, T(1));
-    hop_dir<B, 3, FW, false>(R, a.in, in_q + cb_tp, a.U, poff + cb, tw_p ? bc : T(1));
-    hop_dir<B, 3, BW, true>(R, a.in, in_q + cb_tm, a.U, qoff + cb_tm, tw_m ? bc : T(1));
+    // Software pipeline over the eight hops.  DEPTH 2: the operands of hop
+    // d+1 are in flight while hop d is computed; DEPTH 1: one hop at a time
+    // (fewer registers, used where occupancy matters more).  pin_acc()
+    // orders the next gathers after the arithmetic of the current hop.
+    const SIn &in = a.in;
+    const G &U = a.U;
+    if constexpr (DEPTH == 2) {
+        auto o0 = fetch_dir<B, 0, FW>(in, in_q + cb_xp, U, poff + cb);
+        auto o1 = fetch_dir<B, 0, BW>(in, in_q + cb_xm, U, qoff + cb_xm);
+        apply_dir<B, 0, FW, false>(R, in, U, o0, T(1));
+        QKX_SCHED_FENCE();
+        auto o2 = fetch_dir<B, 1, FW>(in, in_q + cb_yp, U, poff + cb);
+        apply_dir<B, 0, BW, true>(R, in, U, o1, T(1));
+        QKX_SCHED_FENCE();
+        auto o3 = fetch_dir<B, 1, BW>(in, in_q + cb_ym, U, qoff + cb_ym);
+        apply_dir<B, 1, FW, false>(R, in, U, o2, T(1));
+        QKX_SCHED_FENCE();
+        auto o4 = fetch_dir<B, 2, FW>(in, in_q + cb_zp, U, poff + cb);
+        apply_dir<B, 1, BW, true>(R, in, U, o3, T(1));
+        QKX_SCHED_FENCE();
+        auto o5 = fetch_dir<B, 2, BW>(in, in_q + cb_zm, U, qoff + cb_zm);
+        apply_dir<B, 2, FW, false>(R, in, U, o4, T(1));
+        QKX_SCHED_FENCE();
+        auto o6 = fetch_dir<B, 3, FW>(in, in_q + cb_tp, U, poff + cb);
+        apply_dir<B, 2, BW, true>(R, in, U, o5, T(1));
+        QKX_SCHED_FENCE();
+        auto o7 = fetch_dir<B, 3, BW>(in, in_q + cb_tm, U, qoff + cb_tm);
+        apply_dir<B, 3, FW, false>(R, in, U, o6, tw_p ? bc : T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 3, BW, true>(R, in, U, o7, tw_m ? bc : T(1));
+    } else {
+        apply_dir<B, 0, FW, false>(R, in, U, fetch_dir<B, 0, FW>(in, in_q + cb_xp, U, poff + cb), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 0, BW, true>(R, in, U, fetch_dir<B, 0, BW>(in, in_q + cb_xm, U, qoff + cb_xm), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 1, FW, false>(R, in, U, fetch_dir<B, 1, FW>(in, in_q + cb_yp, U, poff + cb), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 1, BW, true>(R, in, U, fetch_dir<B, 1, BW>(in, in_q + cb_ym, U, qoff + cb_ym), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 2, FW, false>(R, in, U, fetch_dir<B, 2, FW>(in, in_q + cb_zp, U, poff + cb), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 2, BW, true>(R, in, U, fetch_dir<B, 2, BW>(in, in_q + cb_zm, U, qoff + cb_zm), T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 3, FW, false>(R, in, U, fetch_dir<B, 3, FW>(in, in_q + cb_tp, U, poff + cb),
+                                   tw_p ? bc : T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 3, BW, true>(R, in, U, fetch_dir<B, 3, BW>(in, in_q + cb_tm, U, qoff + cb_tm),
+                                  tw_m ? bc : T(1));
+    }
 
     Cx<T> res[12];
     if constexpr (AUX) {
@@ -283,6 +382,12 @@ __global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
         epilogue<B, T>(a.epi, R, nullptr, res);
     }
     a.out.store(a.out_off[p] + cb, res);
+    if (a.norm_partial) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+    }
+    }
+    if (a.norm_partial) block_partial(nrm2, a.norm_partial + blockIdx.x);
 }
 
 // Site-local combination: out = cR y + gR i g5 y + cX x + gX i g5 x (g5out),
@@ -316,11 +421,14 @@ __global__ __launch_bounds__(256) void site_kernel(S out, S y, S x, int V, int V
 // host side
 int g_xcd_remap = 1;
 int g_block = 256;
+int g_depth64 = 1;   // pipeline depth of the FP64 hop
+int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 const int *g_skip = nullptr;
 
 template <class S, class G, int B>
 int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
-                 const qkx_gauge_s *U, int dagger, int par_out, const Epi &e)
+                 const qkx_gauge_s *U, int dagger, int par_out, const Epi &e, double *norm_partial,
+                 int *nblocks_out)
 {
     HopArgs<S, S, S, G> a;
     a.g = qkx::make_geom(U->L, U->bc);
@@ -336,13 +444,20 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     a.nout = par_out < 0 ? a.g.V : a.g.Vh;
     a.epi = e;
     a.skip = g_skip;
+    a.norm_partial = norm_partial;
     int nb = (a.nout + g_block - 1) / g_block;
+    a.nblk8 = 0;
     if (g_xcd_remap && nb >= 64) {
-        nb = (nb + 7) / 8 * 8;
-        a.nblk8 = nb / 8;
-    } else {
-        a.nblk8 = 0;
+        if (par_out >= 0) {
+            nb = (nb + 7) / 8 * 8;
+            a.nblk8 = nb / 8;
+        } else {
+            int nbh = ((a.g.Vh + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
+            a.nblk8 = nbh / 8;
+            nb = 2 * nbh;
+        }
     }
+    if (nblocks_out) *nblocks_out = nb;
     // bytes moved per launch (compulsory traffic) for the instrumentation
     const int nout = a.nout;
     double ws = (double)qkx::spinor_elem_bytes(in->prec) * qkx::spinor_nchunks(in->prec) +
@@ -351,16 +466,21 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
                 (U->recon == QKX_RECON_18 ? 9 : 6);
     double bytes = par_out < 0 ? (double)nout * (4 * wg + 2 * ws)
                                : (double)nout * (8 * wg + 2 * ws);
-    if (aux) bytes += (double)nout * ws;
+    if (aux && aux != in) bytes += (double)nout * ws;
     qkx::prof_begin();
-    if (dagger) {
-        if (aux) hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, true, true>), dim3(nb), dim3(g_block), 0, qkx::g_stream, a);
-        else     hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, true, false>), dim3(nb), dim3(g_block), 0, qkx::g_stream, a);
+    const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
+    const dim3 grid(nb), blk(g_block);
+    hipStream_t st = qkx::g_stream;
+#define QKX_LAUNCH_HOP(DG, AX, DP) hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, DG, AX, DP>), grid, blk, 0, st, a)
+    if (depth == 2) {
+        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else QKX_LAUNCH_HOP(true, false, 2); }
+        else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else QKX_LAUNCH_HOP(false, false, 2); }
     } else {
-        if (aux) hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, false, true>), dim3(nb), dim3(g_block), 0, qkx::g_stream, a);
-        else     hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, false, false>), dim3(nb), dim3(g_block), 0, qkx::g_stream, a);
+        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 1); else QKX_LAUNCH_HOP(true, false, 1); }
+        else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else QKX_LAUNCH_HOP(false, false, 1); }
     }
-    qkx::prof_end(bytes);
+#undef QKX_LAUNCH_HOP
+    qkx::prof_end(bytes, in->prec);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -385,15 +505,18 @@ namespace qkx {
 
 void set_hop_skip(const int *flag) { g_skip = flag; }
 
-void set_hop_tuning(int xcd_remap, int block)
+void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
     g_xcd_remap = xcd_remap;
     g_block = block;
+    g_depth64 = depth64;
+    g_depth32 = depth32;
 }
 
 // Hop kernel: out(parity par_out or all) = epilogue(H_or_Hdag in, aux)
 int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
-        int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out)
+        int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out,
+        double *norm_partial, int *nblocks_out)
 {
     Epi e{cR, gR, cX, gX, g5out};
     const int sp = in->prec;
@@ -408,28 +531,28 @@ int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, cons
     if (sp == QKX_PREC_DOUBLE) {
         if (U->prec != QKX_PREC_DOUBLE) { set_error("hop: FP64 spinor needs FP64 gauge"); return QKX_ERR_ARG; }
         if (U->recon == QKX_RECON_18) {
-            if (basis == QKX_BASIS_UKQCD) return launch_hop_t<SpinorD, GaugeD18, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
-            return launch_hop_t<SpinorD, GaugeD18, QKX_BASIS_CHIRAL>(out, in, aux, U, dagger, par_out, e);
+            if (basis == QKX_BASIS_UKQCD) return launch_hop_t<SpinorD, GaugeD18, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
+            return launch_hop_t<SpinorD, GaugeD18, QKX_BASIS_CHIRAL>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
         } else {
-            if (basis == QKX_BASIS_UKQCD) return launch_hop_t<SpinorD, GaugeD12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
-            return launch_hop_t<SpinorD, GaugeD12, QKX_BASIS_CHIRAL>(out, in, aux, U, dagger, par_out, e);
+            if (basis == QKX_BASIS_UKQCD) return launch_hop_t<SpinorD, GaugeD12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
+            return launch_hop_t<SpinorD, GaugeD12, QKX_BASIS_CHIRAL>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
         }
     }
     if (sp == QKX_PREC_SINGLE) {
         if (U->prec == QKX_PREC_SINGLE && U->recon == QKX_RECON_18)
-            return launch_hop_t<SpinorF, GaugeF18, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
+            return launch_hop_t<SpinorF, GaugeF18, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
         if (U->prec == QKX_PREC_SINGLE && U->recon == QKX_RECON_12)
-            return launch_hop_t<SpinorF, GaugeF12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
+            return launch_hop_t<SpinorF, GaugeF12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
     }
     if (sp == QKX_PREC_HALF) {
         if (U->prec == QKX_PREC_HALF && U->recon == QKX_RECON_12)
-            return launch_hop_t<SpinorH, GaugeH12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
+            return launch_hop_t<SpinorH, GaugeH12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
         if (U->prec == QKX_PREC_SINGLE && U->recon == QKX_RECON_12)
-            return launch_hop_t<SpinorH, GaugeF12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
+            return launch_hop_t<SpinorH, GaugeF12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
     }
     if (sp == QKX_PREC_BF16) {
         if (U->prec == QKX_PREC_BF16 && U->recon == QKX_RECON_12)
-            return launch_hop_t<SpinorB, GaugeB12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e);
+            return launch_hop_t<SpinorB, GaugeB12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
     }
     set_error("hop: unsupported spinor/gauge precision combination");
     return QKX_ERR_ARG;

@@ -115,18 +115,23 @@ __device__ __forceinline__ void gauge_store(void *base, int V, int mu, int s, co
         for (int k = 0; k < 3; k++)
             p[(size_t)(mu * 3 + k) * V + s] = make_float4((float)U[2 * k].re, (float)U[2 * k].im,
                                                           (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
-    } else if constexpr (PREC == QKX_PREC_HALF) {
-        uint2 *p = (uint2 *)base;
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            p[(size_t)(mu * 3 + k) * V + s] = pack_h4((float)U[2 * k].re, (float)U[2 * k].im,
-                                                      (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
     } else {
-        uint2 *p = (uint2 *)base;
+        // 16-bit: U0..U3 in a uint4 array [4][V], U4, U5 in a uint2 array [4][V]
+        uint4 *p4 = (uint4 *)base;
+        uint2 *p2 = (uint2 *)((char *)base + (size_t)16 * 4 * V);
+        float f[8];
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-            p[(size_t)(mu * 3 + k) * V + s] = pack_b4((float)U[2 * k].re, (float)U[2 * k].im,
-                                                      (float)U[2 * k + 1].re, (float)U[2 * k + 1].im);
+        for (int k = 0; k < 4; k++) {
+            f[2 * k] = (float)U[k].re;
+            f[2 * k + 1] = (float)U[k].im;
+        }
+        if constexpr (PREC == QKX_PREC_HALF) {
+            p4[(size_t)mu * V + s] = pack_h8(f);
+            p2[(size_t)mu * V + s] = pack_h4((float)U[4].re, (float)U[4].im, (float)U[5].re, (float)U[5].im);
+        } else {
+            p4[(size_t)mu * V + s] = pack_b8(f);
+            p2[(size_t)mu * V + s] = pack_b4((float)U[4].re, (float)U[4].im, (float)U[5].re, (float)U[5].im);
+        }
     }
 }
 
@@ -197,15 +202,9 @@ bool valid_dims(const int L[4])
 
 namespace qkx {
 
-size_t spinor_elem_bytes(int prec)
-{
-    switch (prec) {
-    case QKX_PREC_DOUBLE: return 16;
-    case QKX_PREC_SINGLE: return 16;
-    default: return 8;
-    }
-}
-int spinor_nchunks(int prec) { return prec == QKX_PREC_DOUBLE ? 12 : 6; }
+// every storage chunk is 16 bytes: DOUBLE 12 per site, SINGLE 6, 16-bit 3
+size_t spinor_elem_bytes(int) { return 16; }
+int spinor_nchunks(int prec) { return prec == QKX_PREC_DOUBLE ? 12 : prec == QKX_PREC_SINGLE ? 6 : 3; }
 
 void parity_offsets(const qkx_spinor_s *f, int off[2])
 {

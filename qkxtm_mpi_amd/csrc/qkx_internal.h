@@ -8,12 +8,12 @@
 //   spinor      component-major: component k of site s at  base[k*stride + s]
 //               DOUBLE: 12 x double2 (one complex each)
 //               SINGLE:  6 x float4  (two complex each, k = spin*3+colour order)
-//               HALF  :  6 x 4 fp16  (two complex each) + one float scale per site
-//               BF16  :  6 x 4 bf16  (two complex each)
+//               HALF  :  3 x 8 fp16  (four complex each) + one float scale per site
+//               BF16  :  3 x 8 bf16  (four complex each)
 //   gauge       direction-major, link (mu, s) element k at base[(mu*NC + k)*V + s]
 //               DOUBLE/18: 9 double2      DOUBLE/12: 6 double2
 //               SINGLE/18: 9 float2       SINGLE/12: 3 float4
-//               HALF/12  : 3 x 4 fp16     BF16/12  : 3 x 4 bf16
+//               HALF/12, BF16/12: 8 x 16-bit ([4][V] uint4) then 4 x 16-bit ([4][V] uint2)
 // Consecutive threads take consecutive s, so every load of a wavefront is one
 // contiguous 512 B or 1 KiB run.
 #pragma once
@@ -93,123 +93,219 @@ __device__ __forceinline__ uint2 pack_b4(float a, float b, float c, float d)
 }
 
 // ---------------------------------------------------------------------------
-// spinor storage accessors.  load_half<H>: H=0 -> spins 0,1 (components 0..5),
-// H=1 -> spins 2,3 (components 6..11).
+// Loads/stores as (uniform base pointer, 32-bit byte offset): the base of each
+// component lives in scalar registers and every site costs one 32-bit vector
+// offset, which maps onto the saddr form of global_load/global_store.
+template <class E> __device__ __forceinline__ E ld(const E *base, uint32_t idx)
+{
+    return *reinterpret_cast<const E *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(E));
+}
+template <class E> __device__ __forceinline__ void st(E *base, uint32_t idx, E v)
+{
+    *reinterpret_cast<E *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(E)) = v;
+}
+
+// 8 x 16-bit in a uint4
+__device__ __forceinline__ void unpack_h8(uint4 w, float f[8])
+{
+    unpack_h4(make_uint2(w.x, w.y), f);
+    unpack_h4(make_uint2(w.z, w.w), f + 4);
+}
+__device__ __forceinline__ uint4 pack_h8(const float f[8])
+{
+    uint2 a = pack_h4(f[0], f[1], f[2], f[3]), b = pack_h4(f[4], f[5], f[6], f[7]);
+    return make_uint4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ void unpack_b8(uint4 w, float f[8])
+{
+    unpack_b4(make_uint2(w.x, w.y), f);
+    unpack_b4(make_uint2(w.z, w.w), f + 4);
+}
+__device__ __forceinline__ uint4 pack_b8(const float f[8])
+{
+    uint2 a = pack_b4(f[0], f[1], f[2], f[3]), b = pack_b4(f[4], f[5], f[6], f[7]);
+    return make_uint4(a.x, a.y, b.x, b.y);
+}
+
+// ---------------------------------------------------------------------------
+// spinor storage accessors.  Components are k = spin*3 + colour.
+//   fetch<MASK>(s) returns the raw (unconverted) words, unpack<MASK> converts;
+//   MASK bit 0 = spins 0,1 (components 0..5), bit 1 = spins 2,3 (6..11).
+//   Splitting fetch from unpack lets the stencil keep one hop's loads in
+//   flight while it computes the previous hop.
 struct SpinorD {
     using Real = double;
     double2 *p;
     int stride;
-    template <int H> __device__ __forceinline__ void load_half(int s, Cx<double> v[12]) const
+    struct Raw {
+        double2 c[12];
+    };
+    template <int MASK> __device__ __forceinline__ Raw fetch(int s) const
+    {
+        Raw r;
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            if ((MASK >> (k / 6)) & 1) r.c[k] = ld(p + (size_t)k * stride, (uint32_t)s);
+        return r;
+    }
+    template <int MASK> __device__ __forceinline__ void unpack(const Raw &r, Cx<double> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            double2 d = p[(size_t)(6 * H + k) * stride + s];
-            v[6 * H + k] = {d.x, d.y};
-        }
+        for (int k = 0; k < 12; k++)
+            if ((MASK >> (k / 6)) & 1) v[k] = {r.c[k].x, r.c[k].y};
     }
-    __device__ __forceinline__ void load(int s, Cx<double> v[12]) const
-    {
-        load_half<0>(s, v);
-        load_half<1>(s, v);
-    }
+    __device__ __forceinline__ void load(int s, Cx<double> v[12]) const { unpack<3>(fetch<3>(s), v); }
     __device__ __forceinline__ void store(int s, const Cx<double> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 12; k++) p[(size_t)k * stride + s] = make_double2(v[k].re, v[k].im);
+        for (int k = 0; k < 12; k++) st(p + (size_t)k * stride, (uint32_t)s, make_double2(v[k].re, v[k].im));
     }
 };
 
+// FP32: 6 x float4, chunk j = components 2j, 2j+1
 struct SpinorF {
     using Real = float;
     float4 *p;
     int stride;
-    template <int H> __device__ __forceinline__ void load_half(int s, Cx<float> v[12]) const
+    struct Raw {
+        float4 c[6];
+    };
+    template <int MASK> __device__ __forceinline__ Raw fetch(int s) const
+    {
+        Raw r;
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+            if ((MASK >> (j / 3)) & 1) r.c[j] = ld(p + (size_t)j * stride, (uint32_t)s);
+        return r;
+    }
+    template <int MASK> __device__ __forceinline__ void unpack(const Raw &r, Cx<float> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float4 d = p[(size_t)(3 * H + k) * stride + s];
-            v[6 * H + 2 * k] = {d.x, d.y};
-            v[6 * H + 2 * k + 1] = {d.z, d.w};
-        }
+        for (int j = 0; j < 6; j++)
+            if ((MASK >> (j / 3)) & 1) {
+                v[2 * j] = {r.c[j].x, r.c[j].y};
+                v[2 * j + 1] = {r.c[j].z, r.c[j].w};
+            }
     }
-    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const
-    {
-        load_half<0>(s, v);
-        load_half<1>(s, v);
-    }
+    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const { unpack<3>(fetch<3>(s), v); }
     __device__ __forceinline__ void store(int s, const Cx<float> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 6; k++)
-            p[(size_t)k * stride + s] = make_float4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im);
+        for (int j = 0; j < 6; j++)
+            st(p + (size_t)j * stride, (uint32_t)s,
+               make_float4(v[2 * j].re, v[2 * j].im, v[2 * j + 1].re, v[2 * j + 1].im));
     }
 };
 
-// FP16 storage with one FP32 scale per site (max |component|).
+// 16-bit storage: 3 x uint4, chunk j = components 4j..4j+3.  Spins 0,1 need
+// chunks 0,1; spins 2,3 need chunks 1,2.
+template <int MASK> __device__ __forceinline__ constexpr bool need_chunk16(int j)
+{
+    return (j == 0 && (MASK & 1)) || (j == 1) || (j == 2 && (MASK & 2));
+}
+
+// FP16 with one FP32 scale per site (max |component|)
 struct SpinorH {
     using Real = float;
-    uint2 *p;
+    uint4 *p;
     float *nrm;
     int stride;
-    template <int H> __device__ __forceinline__ void load_half(int s, Cx<float> v[12]) const
+    struct Raw {
+        uint4 c[3];
+        float n;
+    };
+    template <int MASK> __device__ __forceinline__ Raw fetch(int s) const
     {
-        float n = nrm[s];
+        Raw r;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float f[4];
-            unpack_h4(p[(size_t)(3 * H + k) * stride + s], f);
-            v[6 * H + 2 * k] = {f[0] * n, f[1] * n};
-            v[6 * H + 2 * k + 1] = {f[2] * n, f[3] * n};
-        }
+        for (int j = 0; j < 3; j++)
+            if (need_chunk16<MASK>(j)) r.c[j] = ld(p + (size_t)j * stride, (uint32_t)s);
+        r.n = ld(nrm, (uint32_t)s);
+        return r;
     }
-    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const
+    template <int MASK> __device__ __forceinline__ void unpack(const Raw &r, Cx<float> v[12]) const
     {
-        load_half<0>(s, v);
-        load_half<1>(s, v);
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (need_chunk16<MASK>(j)) {
+                float f[8];
+                unpack_h8(r.c[j], f);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int k = 4 * j + i;
+                    if ((MASK >> (k / 6)) & 1) v[k] = {f[2 * i] * r.n, f[2 * i + 1] * r.n};
+                }
+            }
     }
+    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const { unpack<3>(fetch<3>(s), v); }
     __device__ __forceinline__ void store(int s, const Cx<float> v[12]) const
     {
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
-        float inv = m > 0.f ? 1.0f / m : 0.f;
-        nrm[s] = m;
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        st(nrm, (uint32_t)s, m);
 #pragma unroll
-        for (int k = 0; k < 6; k++)
-            p[(size_t)k * stride + s] = pack_h4(v[2 * k].re * inv, v[2 * k].im * inv,
-                                                v[2 * k + 1].re * inv, v[2 * k + 1].im * inv);
+        for (int j = 0; j < 3; j++) {
+            float f[8];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                f[2 * i] = v[4 * j + i].re * inv;
+                f[2 * i + 1] = v[4 * j + i].im * inv;
+            }
+            st(p + (size_t)j * stride, (uint32_t)s, pack_h8(f));
+        }
     }
 };
 
+// BF16 (no scale: bf16 has the FP32 exponent range)
 struct SpinorB {
     using Real = float;
-    uint2 *p;
+    uint4 *p;
     int stride;
-    template <int H> __device__ __forceinline__ void load_half(int s, Cx<float> v[12]) const
+    struct Raw {
+        uint4 c[3];
+    };
+    template <int MASK> __device__ __forceinline__ Raw fetch(int s) const
+    {
+        Raw r;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (need_chunk16<MASK>(j)) r.c[j] = ld(p + (size_t)j * stride, (uint32_t)s);
+        return r;
+    }
+    template <int MASK> __device__ __forceinline__ void unpack(const Raw &r, Cx<float> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float f[4];
-            unpack_b4(p[(size_t)(3 * H + k) * stride + s], f);
-            v[6 * H + 2 * k] = {f[0], f[1]};
-            v[6 * H + 2 * k + 1] = {f[2], f[3]};
-        }
+        for (int j = 0; j < 3; j++)
+            if (need_chunk16<MASK>(j)) {
+                float f[8];
+                unpack_b8(r.c[j], f);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int k = 4 * j + i;
+                    if ((MASK >> (k / 6)) & 1) v[k] = {f[2 * i], f[2 * i + 1]};
+                }
+            }
     }
-    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const
-    {
-        load_half<0>(s, v);
-        load_half<1>(s, v);
-    }
+    __device__ __forceinline__ void load(int s, Cx<float> v[12]) const { unpack<3>(fetch<3>(s), v); }
     __device__ __forceinline__ void store(int s, const Cx<float> v[12]) const
     {
 #pragma unroll
-        for (int k = 0; k < 6; k++)
-            p[(size_t)k * stride + s] = pack_b4(v[2 * k].re, v[2 * k].im, v[2 * k + 1].re, v[2 * k + 1].im);
+        for (int j = 0; j < 3; j++) {
+            float f[8];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                f[2 * i] = v[4 * j + i].re;
+                f[2 * i + 1] = v[4 * j + i].im;
+            }
+            st(p + (size_t)j * stride, (uint32_t)s, pack_b8(f));
+        }
     }
 };
 
 // ---------------------------------------------------------------------------
-// gauge storage accessors: load(mu, s, U) fills the 3x3 link U[row*3+col].
+// gauge storage accessors: fetch(mu, s) raw words, unpack -> U[row*3+col].
 template <class T> __device__ __forceinline__ void recon_row2(Cx<T> U[9])
 {
     // row2 = conj(row0 x row1)  (SU(3))
@@ -221,93 +317,101 @@ template <class T> __device__ __forceinline__ void recon_row2(Cx<T> U[9])
     U[8] = {c.re, -c.im};
 }
 
-struct GaugeD18 {
-    using Real = double;
-    const double2 *p;
+template <class E, int NC, class R> struct GaugeVec {
+    using Real = R;
+    const E *p;
     int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<double> U[9]) const
+    struct Raw {
+        E c[NC];
+    };
+    __device__ __forceinline__ Raw fetch(int mu, int s) const
     {
+        Raw r;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            double2 d = p[(size_t)(mu * 9 + k) * V + s];
-            U[k] = {d.x, d.y};
-        }
+        for (int k = 0; k < NC; k++) r.c[k] = ld(p + (size_t)(mu * NC + k) * V, (uint32_t)s);
+        return r;
     }
 };
-struct GaugeD12 {
-    using Real = double;
-    const double2 *p;
-    int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<double> U[9]) const
+
+// FP64, 18 reals: 9 double2
+struct GaugeD18 : GaugeVec<double2, 9, double> {
+    __device__ __forceinline__ void unpack(const Raw &r, Cx<double> U[9]) const
     {
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            double2 d = p[(size_t)(mu * 6 + k) * V + s];
-            U[k] = {d.x, d.y};
-        }
+        for (int k = 0; k < 9; k++) U[k] = {r.c[k].x, r.c[k].y};
+    }
+    __device__ __forceinline__ void load(int mu, int s, Cx<double> U[9]) const { unpack(fetch(mu, s), U); }
+};
+// FP64, 12 reals: 6 double2, row 2 reconstructed
+struct GaugeD12 : GaugeVec<double2, 6, double> {
+    __device__ __forceinline__ void unpack(const Raw &r, Cx<double> U[9]) const
+    {
+#pragma unroll
+        for (int k = 0; k < 6; k++) U[k] = {r.c[k].x, r.c[k].y};
         recon_row2(U);
     }
+    __device__ __forceinline__ void load(int mu, int s, Cx<double> U[9]) const { unpack(fetch(mu, s), U); }
 };
-struct GaugeF18 {
-    using Real = float;
-    const float2 *p;
-    int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const
+// FP32, 18 reals: 9 float2
+struct GaugeF18 : GaugeVec<float2, 9, float> {
+    __device__ __forceinline__ void unpack(const Raw &r, Cx<float> U[9]) const
     {
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            float2 d = p[(size_t)(mu * 9 + k) * V + s];
-            U[k] = {d.x, d.y};
-        }
+        for (int k = 0; k < 9; k++) U[k] = {r.c[k].x, r.c[k].y};
     }
+    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const { unpack(fetch(mu, s), U); }
 };
-struct GaugeF12 {
-    using Real = float;
-    const float4 *p;
-    int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const
+// FP32, 12 reals: 3 float4
+struct GaugeF12 : GaugeVec<float4, 3, float> {
+    __device__ __forceinline__ void unpack(const Raw &r, Cx<float> U[9]) const
     {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            float4 d = p[(size_t)(mu * 3 + k) * V + s];
-            U[2 * k] = {d.x, d.y};
-            U[2 * k + 1] = {d.z, d.w};
+            U[2 * k] = {r.c[k].x, r.c[k].y};
+            U[2 * k + 1] = {r.c[k].z, r.c[k].w};
         }
         recon_row2(U);
     }
+    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const { unpack(fetch(mu, s), U); }
 };
-struct GaugeH12 {
+// 16-bit, 12 reals: a uint4 (U0..U3) array [4][V] followed by a uint2 (U4, U5)
+// array [4][V]; links are unitary so no scale is needed
+template <bool BF> struct Gauge16 {
     using Real = float;
-    const uint2 *p;
+    const uint4 *p4;
+    const uint2 *p2;
     int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const
+    struct Raw {
+        uint4 a;
+        uint2 b;
+    };
+    __device__ __forceinline__ Raw fetch(int mu, int s) const
     {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float f[4];
-            unpack_h4(p[(size_t)(mu * 3 + k) * V + s], f);
-            U[2 * k] = {f[0], f[1]};
-            U[2 * k + 1] = {f[2], f[3]};
+        Raw r;
+        r.a = ld(p4 + (size_t)mu * V, (uint32_t)s);
+        r.b = ld(p2 + (size_t)mu * V, (uint32_t)s);
+        return r;
+    }
+    __device__ __forceinline__ void unpack(const Raw &r, Cx<float> U[9]) const
+    {
+        float f[8], g[4];
+        if constexpr (BF) {
+            unpack_b8(r.a, f);
+            unpack_b4(r.b, g);
+        } else {
+            unpack_h8(r.a, f);
+            unpack_h4(r.b, g);
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[k] = {f[2 * k], f[2 * k + 1]};
+        U[4] = {g[0], g[1]};
+        U[5] = {g[2], g[3]};
         recon_row2(U);
     }
+    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const { unpack(fetch(mu, s), U); }
 };
-struct GaugeB12 {
-    using Real = float;
-    const uint2 *p;
-    int V;
-    __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const
-    {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float f[4];
-            unpack_b4(p[(size_t)(mu * 3 + k) * V + s], f);
-            U[2 * k] = {f[0], f[1]};
-            U[2 * k + 1] = {f[2], f[3]};
-        }
-        recon_row2(U);
-    }
-};
+using GaugeH12 = Gauge16<false>;
+using GaugeB12 = Gauge16<true>;
 
 // ---------------------------------------------------------------------------
 // host-side handle types (declared before the accessor factories below)
@@ -353,7 +457,7 @@ unsigned int *reduce_counter();
 double *reduce_result();
 // profiling of hopping kernels
 void prof_begin();
-void prof_end(double bytes);
+void prof_end(double bytes, int prec);
 } // namespace qkx
 
 #define QKX_CHECK_ARG(cond, msg)           \
@@ -375,16 +479,23 @@ void prof_end(double bytes);
 template <class S> inline S make_spinor(const qkx_spinor_s *f);
 template <> inline SpinorD make_spinor<SpinorD>(const qkx_spinor_s *f) { return SpinorD{(double2 *)f->data, f->nsites}; }
 template <> inline SpinorF make_spinor<SpinorF>(const qkx_spinor_s *f) { return SpinorF{(float4 *)f->data, f->nsites}; }
-template <> inline SpinorH make_spinor<SpinorH>(const qkx_spinor_s *f) { return SpinorH{(uint2 *)f->data, f->nrm, f->nsites}; }
-template <> inline SpinorB make_spinor<SpinorB>(const qkx_spinor_s *f) { return SpinorB{(uint2 *)f->data, f->nsites}; }
+template <> inline SpinorH make_spinor<SpinorH>(const qkx_spinor_s *f) { return SpinorH{(uint4 *)f->data, f->nrm, f->nsites}; }
+template <> inline SpinorB make_spinor<SpinorB>(const qkx_spinor_s *f) { return SpinorB{(uint4 *)f->data, f->nsites}; }
 
 template <class G> inline G make_gauge(const qkx_gauge_s *g);
-template <> inline GaugeD18 make_gauge<GaugeD18>(const qkx_gauge_s *g) { return GaugeD18{(const double2 *)g->data, g->V}; }
-template <> inline GaugeD12 make_gauge<GaugeD12>(const qkx_gauge_s *g) { return GaugeD12{(const double2 *)g->data, g->V}; }
-template <> inline GaugeF18 make_gauge<GaugeF18>(const qkx_gauge_s *g) { return GaugeF18{(const float2 *)g->data, g->V}; }
-template <> inline GaugeF12 make_gauge<GaugeF12>(const qkx_gauge_s *g) { return GaugeF12{(const float4 *)g->data, g->V}; }
-template <> inline GaugeH12 make_gauge<GaugeH12>(const qkx_gauge_s *g) { return GaugeH12{(const uint2 *)g->data, g->V}; }
-template <> inline GaugeB12 make_gauge<GaugeB12>(const qkx_gauge_s *g) { return GaugeB12{(const uint2 *)g->data, g->V}; }
+template <> inline GaugeD18 make_gauge<GaugeD18>(const qkx_gauge_s *g) { GaugeD18 r; r.p = (const double2 *)g->data; r.V = g->V; return r; }
+template <> inline GaugeD12 make_gauge<GaugeD12>(const qkx_gauge_s *g) { GaugeD12 r; r.p = (const double2 *)g->data; r.V = g->V; return r; }
+template <> inline GaugeF18 make_gauge<GaugeF18>(const qkx_gauge_s *g) { GaugeF18 r; r.p = (const float2 *)g->data; r.V = g->V; return r; }
+template <> inline GaugeF12 make_gauge<GaugeF12>(const qkx_gauge_s *g) { GaugeF12 r; r.p = (const float4 *)g->data; r.V = g->V; return r; }
+// 16-bit links: uint4 part [4][V], then uint2 part [4][V]
+template <> inline GaugeH12 make_gauge<GaugeH12>(const qkx_gauge_s *g)
+{
+    return GaugeH12{(const uint4 *)g->data, (const uint2 *)((const char *)g->data + (size_t)16 * 4 * g->V), g->V};
+}
+template <> inline GaugeB12 make_gauge<GaugeB12>(const qkx_gauge_s *g)
+{
+    return GaugeB12{(const uint4 *)g->data, (const uint2 *)((const char *)g->data + (size_t)16 * 4 * g->V), g->V};
+}
 
 // fn(tag) is called with a default-constructed accessor of the field's type.
 template <class F> inline int dispatch_prec(int prec, F fn)

@@ -45,12 +45,12 @@ namespace qkx {
 //   out_pe = [A in - kappa^2 H A^-1 H in]_pe  (then g5 if g5out)
 // reference tm_xx lib/lattice_util.cpp:2001-2022.
 int schur(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, double kappa, double a, double nrm,
-          int basis, int pe, int g5out, qkx_spinor_s *tmp_other)
+          int basis, int pe, int g5out, qkx_spinor_s *tmp_other, double *norm_partial, int *nblocks)
 {
     int po = 1 - pe;
     int rc = hop(tmp_other, in, nullptr, U, basis, 0, po, 1.0 / nrm, -a / nrm, 0.0, 0.0, 0);
     if (rc) return rc;
-    return hop(out, tmp_other, in, U, basis, 0, pe, -kappa * kappa, 0.0, 1.0, a, g5out);
+    return hop(out, tmp_other, in, U, basis, 0, pe, -kappa * kappa, 0.0, 1.0, a, g5out, norm_partial, nblocks);
 }
 
 } // namespace qkx

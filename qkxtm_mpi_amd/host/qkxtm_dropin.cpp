@@ -1,0 +1,344 @@
+// qkxtm_dropin.cpp -- the reference's Vector / Gauge / CG / BLAS interface
+// (include/lattice_util.h, include/solvers.h, include/blas_qkxTM.h) over the
+// C ABI of libqkx_hip.so.  C++98 (gnu++98), like the reference.
+//
+// Each operator uploads its inputs from the host arrays M (reference layout),
+// runs on the GPU and downloads the result into this->M, so drivers that read
+// or write M directly keep working.  Errors that the reference turns into
+// errorQKXTM (bad parity, bad process grid) abort the same way.
+#include "qkxtm/qkxTM.h"
+#include "qkxtm/lattice_util.h"
+#include "qkxtm/solvers.h"
+#include "qkxtm/blas_qkxTM.h"
+#include "qkx_hip.h"
+
+using namespace qkxTM;
+
+namespace {
+
+void check(int rc, const char *what)
+{
+    if (rc != QKX_SUCCESS) errorQKXTM("%s: %s\n", what, qkx_last_error());
+}
+
+qkx_params params_of(const LatticeInfo *li)
+{
+    qkx_params p;
+    for (int m = 0; m < 4; m++) p.L[m] = li->L[m];
+    p.kappa = li->kappa;
+    p.mu = li->mu;
+    p.twistSign = li->twistSign;
+    p.boundaryT = li->boundaryT;
+    p.tol = li->tol;
+    p.maxIter = (int)li->maxIter;
+    p.reliableDelta = li->reliableDelta;
+    return p;
+}
+
+int parity_code(PARITY p)
+{
+    switch (p) {
+    case EO: return QKX_PARITY_EO;
+    case OE: return QKX_PARITY_OE;
+    case OO: return QKX_PARITY_OO;
+    case EE: return QKX_PARITY_EE;
+    }
+    return -99;
+}
+
+} // namespace
+
+// ---- LatticeGeometry: lib/lattice_util.cpp:44-66 (the single-process part) --
+LatticeGeometry::LatticeGeometry(LatticeInfo *latInfo)
+{
+    rank = latInfo->rank < 0 ? 0 : latInfo->rank;
+    for (int i = 0; i < NDIM; i++) {
+        L[i] = latInfo->L[i];
+        P[i] = latInfo->P[i];
+    }
+    if (P[0] * P[1] * P[2] * P[3] != 1 || latInfo->Nprocs > 1)
+        errorQKXTM("Error one GPU holds the whole lattice: the process grid must be 1,1,1,1\n");
+    for (int i = 0; i < NDIM; i++) {
+        if (L[i] <= 0 || (L[i] & 1)) errorQKXTM("Error every lattice extent must be even and positive\n");
+        lL[i] = L[i];
+        procPos[i] = 0;
+    }
+    V4d = (long)L[0] * L[1] * L[2] * L[3];
+    V3d = (long)L[0] * L[1] * L[2];
+    lV4d = V4d;
+    lV3d = V3d;
+    Nprocs = 1;
+    initialized = true;
+}
+
+// ---- Vector: lib/lattice_util.cpp:382-477 ----------------------------------
+Vector::Vector(LatticeInfo *latInfo) : LatticeGeometry(latInfo), dev_(0)
+{
+    size = lV4d;
+    size_total = lV4d;
+    M = (Complex(*)[NSPINS][NCOLORS])malloc(NSPINOR * size_total * sizeof(double));
+    if (!M) errorQKXTM("Error out of memory");
+}
+
+Vector::~Vector()
+{
+    if (dev_) qkx_spinor_destroy(dev_);
+    free(M);
+}
+
+qkx_spinor_s *Vector::dev()
+{
+    if (!dev_) check(qkx_spinor_create(&dev_, L, QKX_PREC_DOUBLE, QKX_SUBSET_FULL), "Vector");
+    return dev_;
+}
+void Vector::upload() { check(qkx_spinor_upload(dev(), &(M[0][0][0].real())), "Vector upload"); }
+void Vector::download() { check(qkx_spinor_download(dev(), &(M[0][0][0].real())), "Vector download"); }
+
+void Vector::zero() { memset(&(M[0][0][0].real()), 0, NSPINOR * size_total * sizeof(double)); }
+void Vector::copy(Vector &src) { memcpy(&(M[0][0][0].real()), &(src.M[0][0][0].real()), lV4d * NSPINOR * sizeof(double)); }
+
+double Vector::norm2()
+{
+    upload();
+    double r = 0;
+    check(qkx_norm2(&r, dev()), "norm2");
+    return r;
+}
+
+// hopping terms: applyDslash :1158, _chiral :1215, _xx :1272, _xx_chiral :1346, Dag :1643
+void Vector::applyDslash(Vector &psi, Gauge &u)
+{
+    psi.upload();
+    check(qkx_dslash(dev(), psi.dev(), u.dev(), QKX_BASIS_UKQCD, 0, QKX_PARITY_ALL), "applyDslash");
+    download();
+}
+void Vector::applyDslash_chiral(Vector &psi, Gauge &u)
+{
+    psi.upload();
+    check(qkx_dslash(dev(), psi.dev(), u.dev(), QKX_BASIS_CHIRAL, 0, QKX_PARITY_ALL), "applyDslash_chiral");
+    download();
+}
+void Vector::applyDslash_xx(Vector &psi, Gauge &u, PARITY parity)
+{
+    psi.upload();
+    check(qkx_dslash(dev(), psi.dev(), u.dev(), QKX_BASIS_UKQCD, 0, parity_code(parity)), "applyDslash_xx");
+    download();
+}
+void Vector::applyDslash_xx_chiral(Vector &psi, Gauge &u, PARITY parity)
+{
+    psi.upload();
+    check(qkx_dslash(dev(), psi.dev(), u.dev(), QKX_BASIS_CHIRAL, 0, parity_code(parity)), "applyDslash_xx_chiral");
+    download();
+}
+void Vector::applyDslashDag(Vector &psi, Gauge &u)
+{
+    psi.upload();
+    check(qkx_dslash(dev(), psi.dev(), u.dev(), QKX_BASIS_UKQCD, 1, QKX_PARITY_ALL), "applyDslashDag");
+    download();
+}
+
+// site terms added to a hop: :1420, :1444, :1468, :1510, :1552, :1701
+static void xpay(Vector &out, Vector &d, Vector &psi, LatticeInfo *li, int basis, int kind, int parity,
+                 const char *what)
+{
+    d.upload();
+    if (&psi != &d) psi.upload();
+    qkx_params p = params_of(li);
+    check(qkx_twist_add_dslash(out.dev(), d.dev(), psi.dev(), &p, basis, kind, parity), what);
+    out.download();
+}
+void Vector::applyDeltaAddDslash(Vector &d, Vector &psi, LatticeInfo *li)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_UKQCD, QKX_XPAY_WILSON, QKX_PARITY_ALL, "applyDeltaAddDslash");
+}
+void Vector::applyTwistAddDslash(Vector &d, Vector &psi, LatticeInfo *li)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_UKQCD, QKX_XPAY_TM, QKX_PARITY_ALL, "applyTwistAddDslash");
+}
+void Vector::applyTwistAddDslash_chiral(Vector &d, Vector &psi, LatticeInfo *li)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_CHIRAL, QKX_XPAY_TM, QKX_PARITY_ALL, "applyTwistAddDslash_chiral");
+}
+void Vector::applyTwistAddDslash_PC(Vector &d, Vector &psi, LatticeInfo *li, PARITY parity)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_UKQCD, QKX_XPAY_PC, parity_code(parity), "applyTwistAddDslash_PC");
+}
+void Vector::applyTwistAddDslash_PC_chiral(Vector &d, Vector &psi, LatticeInfo *li, PARITY parity)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_CHIRAL, QKX_XPAY_PC, parity_code(parity), "applyTwistAddDslash_PC_chiral");
+}
+void Vector::applyTwistDagAddDslashDag(Vector &d, Vector &psi, LatticeInfo *li)
+{
+    xpay(*this, d, psi, li, QKX_BASIS_UKQCD, QKX_XPAY_TM_DAG, QKX_PARITY_ALL, "applyTwistDagAddDslashDag");
+}
+
+// site-local twist family: :1563, :1726, :1588, :1615, :1750, :1778, :1800
+static void twist(Vector &out, Vector &psi, LatticeInfo *li, int basis, int kind, const char *what)
+{
+    psi.upload();
+    qkx_params p = params_of(li);
+    check(qkx_twist(out.dev(), psi.dev(), &p, basis, kind), what);
+    out.download();
+}
+void Vector::applyTwist(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_UKQCD, QKX_TWIST, "applyTwist"); }
+void Vector::applyTwistDag(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_UKQCD, QKX_TWIST_DAG, "applyTwistDag"); }
+void Vector::applyTwistInv(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_UKQCD, QKX_TWIST_INV, "applyTwistInv"); }
+void Vector::applyTwistInv_chiral(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_CHIRAL, QKX_TWIST_INV, "applyTwistInv_chiral"); }
+void Vector::applyTwistDagInv(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_UKQCD, QKX_TWIST_DAG_INV, "applyTwistDagInv"); }
+void Vector::applyGamma5(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_UKQCD, QKX_GAMMA5, "applyGamma5"); }
+void Vector::applyGamma5_chiral(Vector &psi, LatticeInfo *li) { twist(*this, psi, li, QKX_BASIS_CHIRAL, QKX_GAMMA5, "applyGamma5_chiral"); }
+
+// composites: tm :1925, tm_chiral :1930, tmDag :1935, tmDagtm :1940, Wilson :1949
+static void tm_op(Vector &out, Vector &psi, Gauge &u, LatticeInfo *li, int basis, int op, const char *what)
+{
+    psi.upload();
+    qkx_params p = params_of(li);
+    check(qkx_tm(out.dev(), psi.dev(), u.dev(), &p, basis, op), what);
+    out.download();
+}
+void Vector::tm(Vector &psi, Gauge &u, LatticeInfo *li) { tm_op(*this, psi, u, li, QKX_BASIS_UKQCD, QKX_OP_TM, "tm"); }
+void Vector::tm_chiral(Vector &psi, Gauge &u, LatticeInfo *li) { tm_op(*this, psi, u, li, QKX_BASIS_CHIRAL, QKX_OP_TM, "tm_chiral"); }
+void Vector::tmDag(Vector &psi, Gauge &u, LatticeInfo *li) { tm_op(*this, psi, u, li, QKX_BASIS_UKQCD, QKX_OP_TM_DAG, "tmDag"); }
+void Vector::Wilson(Vector &psi, Gauge &u, LatticeInfo *li) { tm_op(*this, psi, u, li, QKX_BASIS_UKQCD, QKX_OP_WILSON, "Wilson"); }
+void Vector::tmDagtm(Vector &psi, Vector &vec_tmp, Gauge &u, LatticeInfo *li)
+{
+    // as in the reference, vec_tmp ends up holding D psi
+    psi.upload();
+    qkx_params p = params_of(li);
+    check(qkx_tm(vec_tmp.dev(), psi.dev(), u.dev(), &p, QKX_BASIS_UKQCD, QKX_OP_TM), "tmDagtm");
+    check(qkx_tm(dev(), vec_tmp.dev(), u.dev(), &p, QKX_BASIS_UKQCD, QKX_OP_TM_DAG), "tmDagtm");
+    vec_tmp.download();
+    download();
+}
+
+// even-odd Schur operator: tm_xx :2001, tm_xxDagTm_xx :2024, chiral :2037, :2060
+static void tm_xx_op(Vector &out, Vector &psi, Gauge &u, LatticeInfo *li, int basis, PARITY parity, int normal,
+                     const char *what)
+{
+    psi.upload();
+    qkx_params p = params_of(li);
+    check(qkx_tm_xx(out.dev(), psi.dev(), u.dev(), &p, basis, parity_code(parity), normal), what);
+    out.download();
+}
+void Vector::tm_xx(Vector &psi, Gauge &u, LatticeInfo *li, PARITY parity)
+{
+    tm_xx_op(*this, psi, u, li, QKX_BASIS_UKQCD, parity, 0, "tm_xx");
+}
+void Vector::tm_xxDagTm_xx(Vector &psi, Gauge &u, LatticeInfo *li, PARITY parity)
+{
+    tm_xx_op(*this, psi, u, li, QKX_BASIS_UKQCD, parity, 1, "tm_xxDagTm_xx");
+}
+void Vector::tm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *li, PARITY parity)
+{
+    tm_xx_op(*this, psi, u, li, QKX_BASIS_CHIRAL, parity, 0, "tm_xx_chiral");
+}
+void Vector::tm_xxDagTm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *li, PARITY parity)
+{
+    tm_xx_op(*this, psi, u, li, QKX_BASIS_CHIRAL, parity, 1, "tm_xxDagTm_xx_chiral");
+}
+
+// ---- Gauge: lib/lattice_util.cpp:649-724 -----------------------------------
+Gauge::Gauge(LatticeInfo *latInfo) : LatticeGeometry(latInfo), constGauge(false), dev_(0), uploaded_(false)
+{
+    size = lV4d;
+    size_total = lV4d;
+    M = (Complex(*)[NDIM][NCOLORS][NCOLORS])malloc(NLINKS * size_total * sizeof(double));
+    if (!M) errorQKXTM("Error out of memory");
+}
+
+Gauge::~Gauge()
+{
+    if (dev_) qkx_gauge_destroy(dev_);
+    free(M);
+}
+
+qkx_gauge_s *Gauge::dev()
+{
+    // the host links carry the boundary sign (applyBoundaryCondition), so the
+    // kernels apply none of their own
+    if (!dev_) check(qkx_gauge_create(&dev_, L, QKX_PREC_DOUBLE, QKX_RECON_18, +1), "Gauge");
+    if (!uploaded_ || !constGauge) {
+        check(qkx_gauge_upload(dev_, &(M[0][0][0][0].real())), "Gauge upload");
+        uploaded_ = true;
+    }
+    return dev_;
+}
+
+void Gauge::zero()
+{
+    memset(&(M[0][0][0][0].real()), 0, NLINKS * size_total * sizeof(double));
+    uploaded_ = false;
+}
+void Gauge::copy(Gauge &src)
+{
+    memcpy(&(M[0][0][0][0].real()), &(src.M[0][0][0][0].real()), lV4d * NLINKS * sizeof(double));
+    uploaded_ = false;
+}
+double Gauge::norm2()
+{
+    double s = 0;
+    for (long iv = 0; iv < lV4d; iv++)
+        for (int d = 0; d < NDIM; d++)
+            for (int a = 0; a < NCOLORS; a++)
+                for (int b = 0; b < NCOLORS; b++) s += std::norm(M[iv][d][a][b]);
+    return s;
+}
+// Gauge::applyBoundaryCondition :708-724: t-links of the last slice * -1
+void Gauge::applyBoundaryCondition(LatticeInfo *latInfo)
+{
+    if (latInfo->boundaryT != -1) return;
+    printfQKXTM("Flip sign in gauge link\n");
+    for (long iv = lL[0] * lL[1] * lL[2] * (lL[3] - 1); iv < lV4d; iv++)
+        for (int a = 0; a < NCOLORS; a++)
+            for (int b = 0; b < NCOLORS; b++) M[iv][3][a][b] = -M[iv][3][a][b];
+    uploaded_ = false;
+}
+
+// ---- CG: lib/solvers.cpp:8-81 ----------------------------------------------
+CG::CG(LatticeInfo *param, qkxTM::Gauge &gauge) : u(gauge), lastIterations(0)
+{
+    maxIter = (int)param->maxIter;
+    tol = param->tol;
+    reliableDelta = param->reliableDelta;
+}
+
+void CG::operator()(qkxTM::Vector &out, qkxTM::Vector &in, LatticeInfo *param)
+{
+    int rank = out.rank;
+    out.upload();
+    in.upload();
+    qkx_params p = params_of(param);
+    p.tol = tol;
+    p.maxIter = maxIter;
+    qkx_solver_stats st;
+    int rc = qkx_cg(out.dev(), in.dev(), u.dev(), &p, &st);
+    if (rc != QKX_SUCCESS && rc != QKX_NOT_CONVERGED) check(rc, "CG");
+    lastIterations = st.iterations;
+    out.download();
+    if (rc == QKX_NOT_CONVERGED) printfQKXTM("Warning solver reached number of max iterations \n");
+    else printfQKXTM("CG converged after iteration %d\n", st.iterations);
+}
+
+// ---- BLAS: lib/blas_qkxTM.cpp:4-56 -----------------------------------------
+double reCdotXdagY(qkxTM::Vector &x, qkxTM::Vector &y)
+{
+    x.upload();
+    if (&y != &x) y.upload();
+    double r = 0;
+    check(qkx_redot(&r, x.dev(), y.dev()), "reCdotXdagY");
+    return r;
+}
+void X_eq_X_p_aY(qkxTM::Vector &x, double a, qkxTM::Vector &y)
+{
+    x.upload();
+    if (&y != &x) y.upload();
+    check(qkx_axpy(x.dev(), a, y.dev()), "X_eq_X_p_aY");
+    x.download();
+}
+void X_eq_aX_p_Y(qkxTM::Vector &x, double a, qkxTM::Vector &y)
+{
+    x.upload();
+    if (&y != &x) y.upload();
+    check(qkx_xpay(x.dev(), a, y.dev()), "X_eq_aX_p_Y");
+    x.download();
+}

@@ -1,0 +1,49 @@
+"""GPU: the reference-driver counterparts, built on the C++98 drop-in classes,
+against the CPU oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from refops import rel
+
+pytestmark = pytest.mark.gpu
+HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qkxtm_mpi_amd", "host")
+
+
+def test_driver_test_invert(q):
+    """bin/test_invert counterpart: CG through qkxTM::Vector / CG on config-1."""
+    print("driver test_invert", flush=True)
+    r = subprocess.run([os.path.join(HOST, "test_invert")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    norm = float(re.search(r"the norm of vector is (\S+)", r.stdout).group(1))
+    res = float(re.search(r"true residual (\S+)", r.stdout).group(1))
+    iters = int(re.search(r"iterations (\d+)", r.stdout).group(1))
+    L = (8, 8, 8, 16)
+    U = orc.apply_bc(q.gen_gauge(L, 0.4, 1), L, -1)
+    src = q.gen_z2(L, 2)
+    b = orc.tm(src, U, L, 0.156, 0.01, 1, orc.UKQCD, orc.TM_DAG)
+    x, k, _ = orc.cg(b, U, L, 0.156, 0.01, 1, 1e-10, 5000)
+    assert abs(iters - k) <= 2
+    assert abs(norm - orc.norm2(x)) <= 1e-8 * orc.norm2(x)
+    assert res <= 1e-9
+
+
+def test_driver_test_applyTM(q, tmp_path):
+    """bin/test_applyTM counterpart: tm_xxDagTm_xx(OO) through qkxTM::Vector."""
+    print("driver test_applyTM", flush=True)
+    dump = tmp_path / "dump.txt"
+    r = subprocess.run([os.path.join(HOST, "test_applyTM"), str(dump)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    a = np.loadtxt(dump)
+    L = (8, 8, 8, 8)
+    V = int(np.prod(L))
+    vin = (a[:, 4] + 1j * a[:, 5]).reshape(V, 4, 3)
+    vout = (a[:, 6] + 1j * a[:, 7]).reshape(V, 4, 3)
+    assert np.array_equal(vin, q.gen_z2(L, 12))
+    U = q.gen_gauge(L, 0.4, 11)       # boundaryT = +1 in this driver
+    ref = orc.tm_xx(np.ascontiguousarray(vin), U, L, 0.156, 0.1, 1, orc.UKQCD, orc.OO, 1)
+    assert rel(vout, ref) <= 1e-13
