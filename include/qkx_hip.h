@@ -139,11 +139,21 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src);
 
 /* Vector::Vector lib/lattice_util.cpp:382-413 */
 int qkx_spinor_create(qkx_spinor **s, const int L[4], int prec, int subset);
+/* A batch of nrhs (<= QKX_MAX_RHS) fields on one lattice, stored with the
+ * right-hand side innermost so one stencil launch serves all of them and each
+ * link is read once per site (no reference counterpart: the reference solves
+ * the 12 propagator columns one after another, bin/test_invert_prop.cpp:83-97).
+ * Every single-field call also accepts a batch and acts on all its members. */
+#define QKX_MAX_RHS 16
+int qkx_spinor_create_batch(qkx_spinor **s, const int L[4], int prec, int subset, int nrhs);
 int qkx_spinor_destroy(qkx_spinor *s);
 /* Host AoS Complex[V][4][3] <-> device.  A parity subset takes its own sites
  * from the full host array; download writes zeros on the other parity. */
 int qkx_spinor_upload(qkx_spinor *s, const double *psi_aos);
 int qkx_spinor_download(const qkx_spinor *s, double *psi_aos);
+/* one member of a batch */
+int qkx_spinor_upload_rhs(qkx_spinor *s, int rhs, const double *psi_aos);
+int qkx_spinor_download_rhs(const qkx_spinor *s, int rhs, double *psi_aos);
 /* Vector::zero :443-447, Vector::copy :449-453 (copy also converts precision
  * and moves a parity between full and subset fields). */
 int qkx_spinor_zero(qkx_spinor *s);
@@ -224,6 +234,20 @@ typedef struct {
 int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
               const qkx_gauge *U_inner, const qkx_params *p,
               const qkx_solve_opts *o, qkx_solver_stats *st);
+
+/* Solve D_r x_r = b_r for every right-hand side r of a batch (x, b from
+ * qkx_spinor_create_batch, FP64, full lattice) in lockstep: one stencil launch
+ * serves the whole batch and reads each link once per site.  This replaces the
+ * reference's loop of single solves over the 12 spin-colour sources of a
+ * propagator (bin/test_invert_prop.cpp:83-97).  mu (may be NULL -> p->mu for
+ * all) gives each right-hand side its own twisted mass on the shared gauge
+ * field.  Right-hand sides that meet tol are frozen while the others go on;
+ * reliable updates are taken for the whole batch.  st (may be NULL) receives
+ * nrhs entries; iterations is the batch's iteration count. */
+int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+                    const qkx_gauge *U_inner, const qkx_params *p,
+                    const double *mu, const qkx_solve_opts *o,
+                    qkx_solver_stats *st);
 
 /* Relative true residual |b - D x| / |b| in FP64, full lattice, UKQCD. */
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,

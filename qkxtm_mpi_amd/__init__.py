@@ -32,6 +32,7 @@ FULL, EVEN, ODD = 0, 1, 2
 TWIST, TWIST_DAG, TWIST_INV, TWIST_DAG_INV, GAMMA5 = 0, 1, 2, 3, 4
 XPAY_TM, XPAY_TM_DAG, XPAY_WILSON, XPAY_PC = 0, 1, 2, 3
 OP_TM, OP_TM_DAG, OP_TM_DAG_TM, OP_WILSON = 0, 1, 2, 3
+MAX_RHS = 16                                            # QKX_MAX_RHS
 
 
 class QkxError(RuntimeError):
@@ -73,6 +74,8 @@ SIGNATURES = {
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P],
     "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
+    "qkx_spinor_create_batch": [ctypes.POINTER(_P), _IP, _I, _I, _I],
+    "qkx_spinor_upload_rhs": [_P, _I, _P], "qkx_spinor_download_rhs": [_P, _I, _P],
     "qkx_spinor_upload": [_P, _P], "qkx_spinor_download": [_P, _P], "qkx_spinor_zero": [_P],
     "qkx_spinor_copy": [_P, _P],
     "qkx_dslash": [_P, _P, _P, _I, _I, _I], "qkx_twist": [_P, _P, ctypes.POINTER(Params), _I, _I],
@@ -82,6 +85,8 @@ SIGNATURES = {
     "qkx_redot": [_DP, _P, _P], "qkx_axpy": [_P, _D, _P], "qkx_xpay": [_P, _D, _P], "qkx_norm2": [_DP, _P],
     "qkx_cg": [_P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolverStats)],
     "qkx_solve": [_P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolveOpts), ctypes.POINTER(SolverStats)],
+    "qkx_solve_batch": [_P, _P, _P, _P, ctypes.POINTER(Params), _DP, ctypes.POINTER(SolveOpts),
+                        ctypes.POINTER(SolverStats)],
     "qkx_true_residual": [_DP, _P, _P, _P, ctypes.POINTER(Params)],
     "qkx_gen_gauge": [_P, _IP, _D, ctypes.c_ulonglong], "qkx_gen_z2": [_P, _IP, ctypes.c_ulonglong],
     "qkx_gen_gaussian": [_P, _IP, ctypes.c_ulonglong], "qkx_gen_point": [_P, _IP, _IP, _I, _I],
@@ -216,23 +221,45 @@ class Gauge:
 class Spinor:
     """Device spinor field (reference qkxTM::Vector, include/lattice_util.h:80-156)."""
 
-    def __init__(self, L, prec=DOUBLE, subset=FULL, host=None):
+    def __init__(self, L, prec=DOUBLE, subset=FULL, host=None, nrhs=1):
         self.L = tuple(L)
-        self.prec, self.subset = prec, subset
+        self.prec, self.subset, self.nrhs = prec, subset, int(nrhs)
         h = ctypes.c_void_p()
-        _check(lib().qkx_spinor_create(ctypes.byref(h), _ivec(L), prec, subset), "qkx_spinor_create")
+        if self.nrhs == 1:
+            _check(lib().qkx_spinor_create(ctypes.byref(h), _ivec(L), prec, subset), "qkx_spinor_create")
+        else:
+            _check(lib().qkx_spinor_create_batch(ctypes.byref(h), _ivec(L), prec, subset, self.nrhs),
+                   "qkx_spinor_create_batch")
         self.h = h
         if host is not None:
             self.upload(host)
 
     def upload(self, psi):
+        """Whole field from (V, 4, 3) complex -- or (nrhs, V, 4, 3) for a batch."""
         psi = np.ascontiguousarray(psi, dtype=np.complex128)
-        assert psi.size == volume(self.L) * 12
-        _check(lib().qkx_spinor_upload(self.h, _ptr(psi)), "qkx_spinor_upload")
+        if self.nrhs == 1:
+            assert psi.size == volume(self.L) * 12
+            _check(lib().qkx_spinor_upload(self.h, _ptr(psi)), "qkx_spinor_upload")
+        else:
+            psi = psi.reshape(self.nrhs, volume(self.L) * 12)
+            for r in range(self.nrhs):
+                self.upload_rhs(r, psi[r])
 
     def download(self):
+        if self.nrhs > 1:
+            return np.stack([self.download_rhs(r) for r in range(self.nrhs)])
         out = np.zeros((volume(self.L), 4, 3), dtype=np.complex128)
         _check(lib().qkx_spinor_download(self.h, _ptr(out)), "qkx_spinor_download")
+        return out
+
+    def upload_rhs(self, r, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.size == volume(self.L) * 12
+        _check(lib().qkx_spinor_upload_rhs(self.h, int(r), _ptr(psi)), "qkx_spinor_upload_rhs")
+
+    def download_rhs(self, r):
+        out = np.zeros((volume(self.L), 4, 3), dtype=np.complex128)
+        _check(lib().qkx_spinor_download_rhs(self.h, int(r), _ptr(out)), "qkx_spinor_download_rhs")
         return out
 
     def zero(self):
@@ -335,6 +362,25 @@ def solve(x: Spinor, b: Spinor, u: Gauge, p: Params, eo=True, inner_prec=DOUBLE,
     return st
 
 
+def solve_batch(x: Spinor, b: Spinor, u: Gauge, p: Params, mu=None, eo=True, inner_prec=DOUBLE,
+                inner_recon=RECON_12, delta=0.1, tol=1e-10, maxIter=100000, u_inner: Gauge | None = None,
+                allow_not_converged=False):
+    """Solve D_r x_r = b_r for every right-hand side of a batch in lockstep
+    (per-rhs twisted mass ``mu`` optional); returns a list of SolverStats."""
+    o = SolveOpts(int(eo), inner_prec, inner_recon, delta, 50, int(maxIter), tol)
+    st = (SolverStats * b.nrhs)()
+    mu_arr = None
+    if mu is not None:
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        assert mu.size == b.nrhs
+        mu_arr = mu.ctypes.data_as(_DP)
+    rc = lib().qkx_solve_batch(x.h, b.h, u.h, u_inner.h if u_inner else None, ctypes.byref(p), mu_arr,
+                               ctypes.byref(o), st)
+    if not (rc == NOT_CONVERGED and allow_not_converged):
+        _check(rc, "qkx_solve_batch")
+    return list(st)
+
+
 def true_residual(x: Spinor, b: Spinor, u: Gauge, p: Params):
     v = ctypes.c_double()
     _check(lib().qkx_true_residual(ctypes.byref(v), x.h, b.h, u.h, ctypes.byref(p)), "qkx_true_residual")
@@ -373,5 +419,5 @@ def set_cg_batch(iterations=16):
     _check(lib().qkx_set_cg_batch(iterations), "qkx_set_cg_batch")
 
 
-def set_tuning(xcd_remap=True, block=256, depth64=1, depth32=2):
+def set_tuning(xcd_remap=True, block=256, depth64=2, depth32=2):
     _check(lib().qkx_set_tuning(int(xcd_remap), block, depth64, depth32), "qkx_set_tuning")

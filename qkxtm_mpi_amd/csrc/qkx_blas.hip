@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void axpby_kernel(S x, S y, int n, double ax, 
 int check_pair(const qkx_spinor_s *x, const qkx_spinor_s *y, const char *who)
 {
     if (!x || !y) { qkx::set_error(std::string(who) + ": null field"); return QKX_ERR_ARG; }
-    if (x->prec != y->prec || x->nsites != y->nsites || x->subset != y->subset) {
+    if (x->prec != y->prec || x->nsites != y->nsites || x->nrhs != y->nrhs || x->subset != y->subset) {
         qkx::set_error(std::string(who) + ": fields differ in precision, subset or size");
         return QKX_ERR_ARG;
     }
@@ -136,7 +136,7 @@ int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result)
 {
     int rc = check_pair(x, y ? y : x, "redot");
     if (rc) return rc;
-    int n = x->nsites;
+    int n = entries(x);
     int nb = std::min(kRedBlocks, (n + kRedThreads - 1) / kRedThreads);
     int mode = y ? 0 : 1;
     return dispatch_prec(x->prec, [&](auto tag) {
@@ -154,7 +154,7 @@ int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const 
 {
     int rc = check_pair(x, y, "axpby");
     if (rc) return rc;
-    int n = x->nsites;
+    int n = entries(x);
     return dispatch_prec(x->prec, [&](auto tag) {
         using S = decltype(tag);
         S X = make_spinor<S>(x), Y = make_spinor<S>(y);

@@ -81,20 +81,22 @@ void prof_end(double bytes, int prec)
     g_prof_used++;
 }
 
-qkx_spinor_s *pool_get(const int L[4], int prec, int subset)
+qkx_spinor_s *pool_get_batch(const int L[4], int prec, int subset, int nrhs)
 {
     for (auto &e : g_pool) {
-        if (!e.busy && e.f->prec == prec && e.f->subset == subset && e.f->L[0] == L[0] && e.f->L[1] == L[1] &&
-            e.f->L[2] == L[2] && e.f->L[3] == L[3]) {
+        if (!e.busy && e.f->prec == prec && e.f->subset == subset && e.f->nrhs == nrhs && e.f->L[0] == L[0] &&
+            e.f->L[1] == L[1] && e.f->L[2] == L[2] && e.f->L[3] == L[3]) {
             e.busy = true;
             return e.f;
         }
     }
     qkx_spinor_s *f = nullptr;
-    if (qkx_spinor_create(&f, L, prec, subset) != QKX_SUCCESS) return nullptr;
+    if (qkx_spinor_create_batch(&f, L, prec, subset, nrhs) != QKX_SUCCESS) return nullptr;
     g_pool.push_back({f, true});
     return f;
 }
+
+qkx_spinor_s *pool_get(const int L[4], int prec, int subset) { return pool_get_batch(L, prec, subset, 1); }
 
 void pool_put(qkx_spinor_s *f)
 {

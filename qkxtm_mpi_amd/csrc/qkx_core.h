@@ -13,9 +13,15 @@ namespace qkx {
 int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
         int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out,
         double *norm_partial = nullptr, int *nblocks_out = nullptr);
+// the same with per-right-hand-side coefficients coef[r] = {cR, gR, cX, gX}
+int hop_batch(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
+              int basis, int dagger, int par_out, const double coef[][4], int g5out,
+              double *norm_partial = nullptr, int *nblocks_out = nullptr);
 // site-local combination (qkx_dslash.hip)
 int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
          double cR, double gR, double cX, double gX, int g5out);
+int site_batch(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
+               const double coef[][4], int g5out);
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 // even-odd Schur operator on parity pe (qkx_ops.cpp):
@@ -36,12 +42,13 @@ int zero_parity(qkx_spinor_s *f, int parity);
 // core (qkx_core.cpp)
 double *result_slot(int k);
 qkx_spinor_s *pool_get(const int L[4], int prec, int subset);
+qkx_spinor_s *pool_get_batch(const int L[4], int prec, int subset, int nrhs);
 void pool_put(qkx_spinor_s *f);
 
 // RAII temporary from the pool
 struct Temp {
     qkx_spinor_s *f;
-    Temp(const int L[4], int prec, int subset) : f(pool_get(L, prec, subset)) {}
+    Temp(const int L[4], int prec, int subset, int nrhs = 1) : f(pool_get_batch(L, prec, subset, nrhs)) {}
     ~Temp() { if (f) pool_put(f); }
     Temp(const Temp &) = delete;
     Temp &operator=(const Temp &) = delete;

@@ -231,11 +231,12 @@ struct HopArgs {
     G U;
     int in_off[2], out_off[2], aux_off[2];
     int par_out;   // -1: all sites; 0/1: that parity only
-    int nout;
+    int nout;      // output entries (sites x right-hand sides)
+    int nrhs;      // right-hand sides; thread unit u -> site u / nrhs, rhs u % nrhs
     int nblk8;     // grid size / 8 when XCD remapping is on, else 0
     const int *skip;   // when non-null and *skip != 0 the launch does nothing
-    double *norm_partial;  // when non-null: per-workgroup sum of |out|^2
-    Epi epi;
+    double *norm_partial;  // when non-null: per-workgroup (and per-rhs) sum of |out|^2
+    Epi epi[QKX_MAX_RHS];  // epilogue coefficients per right-hand side
 };
 
 // Sum v over the workgroup; thread 0 writes the result to dst.
@@ -269,22 +270,26 @@ __global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
     using T = typename SOut::Real;
     if (a.skip && *a.skip) return;   // uniform over the grid
     const Geom &g = a.g;
-    int p, cb;
+    const int nr = a.nrhs;
+    int p, u;   // parity and unit (site x rhs) index within it
     if (a.par_out >= 0) {
         p = a.par_out;
-        cb = logical_block(a.nblk8) * blockDim.x + threadIdx.x;
+        u = logical_block(a.nblk8) * blockDim.x + threadIdx.x;
     } else if (a.nblk8 > 0) {
         // full lattice: XCD k owns one slab of checkerboard indices and
         // alternates between the two parities, so the links and spinors of a
         // slab (both parities) are fetched into one L2 only
         const int b = blockIdx.x, i = b >> 3;
         p = i & 1;
-        cb = ((b & 7) * a.nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
+        u = ((b & 7) * a.nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
     } else {
         const int o = blockIdx.x * blockDim.x + threadIdx.x;
-        p = (o >= g.Vh) ? 1 : 0;
-        cb = o - p * g.Vh;
+        const int vhn = g.Vh * nr;
+        p = (o >= vhn) ? 1 : 0;
+        u = o - p * vhn;
     }
+    const int cb = nr == 1 ? u : u / nr;
+    const int rr = u - cb * nr;
     double nrm2 = 0.0;
     if (cb < g.Vh) {
     // coordinates: cb = xh + Lh0 * rest,  rest = y + Ly*(z + Lz*t)
@@ -330,75 +335,95 @@ __global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
     const SIn &in = a.in;
     const G &U = a.U;
     if constexpr (DEPTH == 2) {
-        auto o0 = fetch_dir<B, 0, FW>(in, in_q + cb_xp, U, poff + cb);
-        auto o1 = fetch_dir<B, 0, BW>(in, in_q + cb_xm, U, qoff + cb_xm);
+        auto o0 = fetch_dir<B, 0, FW>(in, in_q + cb_xp * nr + rr, U, poff + cb);
+        auto o1 = fetch_dir<B, 0, BW>(in, in_q + cb_xm * nr + rr, U, qoff + cb_xm);
         apply_dir<B, 0, FW, false>(R, in, U, o0, T(1));
         QKX_SCHED_FENCE();
-        auto o2 = fetch_dir<B, 1, FW>(in, in_q + cb_yp, U, poff + cb);
+        auto o2 = fetch_dir<B, 1, FW>(in, in_q + cb_yp * nr + rr, U, poff + cb);
         apply_dir<B, 0, BW, true>(R, in, U, o1, T(1));
         QKX_SCHED_FENCE();
-        auto o3 = fetch_dir<B, 1, BW>(in, in_q + cb_ym, U, qoff + cb_ym);
+        auto o3 = fetch_dir<B, 1, BW>(in, in_q + cb_ym * nr + rr, U, qoff + cb_ym);
         apply_dir<B, 1, FW, false>(R, in, U, o2, T(1));
         QKX_SCHED_FENCE();
-        auto o4 = fetch_dir<B, 2, FW>(in, in_q + cb_zp, U, poff + cb);
+        auto o4 = fetch_dir<B, 2, FW>(in, in_q + cb_zp * nr + rr, U, poff + cb);
         apply_dir<B, 1, BW, true>(R, in, U, o3, T(1));
         QKX_SCHED_FENCE();
-        auto o5 = fetch_dir<B, 2, BW>(in, in_q + cb_zm, U, qoff + cb_zm);
+        auto o5 = fetch_dir<B, 2, BW>(in, in_q + cb_zm * nr + rr, U, qoff + cb_zm);
         apply_dir<B, 2, FW, false>(R, in, U, o4, T(1));
         QKX_SCHED_FENCE();
-        auto o6 = fetch_dir<B, 3, FW>(in, in_q + cb_tp, U, poff + cb);
+        auto o6 = fetch_dir<B, 3, FW>(in, in_q + cb_tp * nr + rr, U, poff + cb);
         apply_dir<B, 2, BW, true>(R, in, U, o5, T(1));
         QKX_SCHED_FENCE();
-        auto o7 = fetch_dir<B, 3, BW>(in, in_q + cb_tm, U, qoff + cb_tm);
+        auto o7 = fetch_dir<B, 3, BW>(in, in_q + cb_tm * nr + rr, U, qoff + cb_tm);
         apply_dir<B, 3, FW, false>(R, in, U, o6, tw_p ? bc : T(1));
         QKX_SCHED_FENCE();
         apply_dir<B, 3, BW, true>(R, in, U, o7, tw_m ? bc : T(1));
     } else {
-        apply_dir<B, 0, FW, false>(R, in, U, fetch_dir<B, 0, FW>(in, in_q + cb_xp, U, poff + cb), T(1));
+        apply_dir<B, 0, FW, false>(R, in, U, fetch_dir<B, 0, FW>(in, in_q + cb_xp * nr + rr, U, poff + cb), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 0, BW, true>(R, in, U, fetch_dir<B, 0, BW>(in, in_q + cb_xm, U, qoff + cb_xm), T(1));
+        apply_dir<B, 0, BW, true>(R, in, U, fetch_dir<B, 0, BW>(in, in_q + cb_xm * nr + rr, U, qoff + cb_xm), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 1, FW, false>(R, in, U, fetch_dir<B, 1, FW>(in, in_q + cb_yp, U, poff + cb), T(1));
+        apply_dir<B, 1, FW, false>(R, in, U, fetch_dir<B, 1, FW>(in, in_q + cb_yp * nr + rr, U, poff + cb), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 1, BW, true>(R, in, U, fetch_dir<B, 1, BW>(in, in_q + cb_ym, U, qoff + cb_ym), T(1));
+        apply_dir<B, 1, BW, true>(R, in, U, fetch_dir<B, 1, BW>(in, in_q + cb_ym * nr + rr, U, qoff + cb_ym), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 2, FW, false>(R, in, U, fetch_dir<B, 2, FW>(in, in_q + cb_zp, U, poff + cb), T(1));
+        apply_dir<B, 2, FW, false>(R, in, U, fetch_dir<B, 2, FW>(in, in_q + cb_zp * nr + rr, U, poff + cb), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 2, BW, true>(R, in, U, fetch_dir<B, 2, BW>(in, in_q + cb_zm, U, qoff + cb_zm), T(1));
+        apply_dir<B, 2, BW, true>(R, in, U, fetch_dir<B, 2, BW>(in, in_q + cb_zm * nr + rr, U, qoff + cb_zm), T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 3, FW, false>(R, in, U, fetch_dir<B, 3, FW>(in, in_q + cb_tp, U, poff + cb),
+        apply_dir<B, 3, FW, false>(R, in, U, fetch_dir<B, 3, FW>(in, in_q + cb_tp * nr + rr, U, poff + cb),
                                    tw_p ? bc : T(1));
         QKX_SCHED_FENCE();
-        apply_dir<B, 3, BW, true>(R, in, U, fetch_dir<B, 3, BW>(in, in_q + cb_tm, U, qoff + cb_tm),
+        apply_dir<B, 3, BW, true>(R, in, U, fetch_dir<B, 3, BW>(in, in_q + cb_tm * nr + rr, U, qoff + cb_tm),
                                   tw_m ? bc : T(1));
     }
 
     Cx<T> res[12];
+    Epi ep = a.epi[0];                 // uniform (scalar) load for a single rhs
+    if (nr > 1) ep = a.epi[rr];
     if constexpr (AUX) {
         Cx<T> xv[12];
-        a.aux.load(a.aux_off[p] + cb, xv);
-        epilogue<B>(a.epi, R, xv, res);
+        a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
+        epilogue<B>(ep, R, xv, res);
     } else {
-        epilogue<B, T>(a.epi, R, nullptr, res);
+        epilogue<B, T>(ep, R, nullptr, res);
     }
-    a.out.store(a.out_off[p] + cb, res);
+    a.out.store(a.out_off[p] + cb * nr + rr, res);
     if (a.norm_partial) {
 #pragma unroll
         for (int k = 0; k < 12; k++) nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
     }
     }
-    if (a.norm_partial) block_partial(nrm2, a.norm_partial + blockIdx.x);
+    if (a.norm_partial) {
+        if (nr == 1) {
+            block_partial(nrm2, a.norm_partial + blockIdx.x);
+        } else {
+            // per right-hand side: LDS accumulation, then one partial per rhs
+            __shared__ double acc[QKX_MAX_RHS];
+            if (threadIdx.x < QKX_MAX_RHS) acc[threadIdx.x] = 0.0;
+            __syncthreads();
+            if (cb < g.Vh) atomicAdd(&acc[rr], nrm2);
+            __syncthreads();
+            if ((int)threadIdx.x < nr) a.norm_partial[blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+        }
+    }
 }
 
 // Site-local combination: out = cR y + gR i g5 y + cX x + gX i g5 x (g5out),
 // optionally restricted to one parity with zeros written on the other.
+struct EpiSet {
+    Epi e[QKX_MAX_RHS];
+};
+
 template <class S, int B>
 __global__ __launch_bounds__(256) void site_kernel(S out, S y, S x, int V, int Vh, int has_x,
-                                                   int keep_par, Epi e)
+                                                   int keep_par, int nrhs, EpiSet es)
 {
     using T = typename S::Real;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
+    Epi e = es.e[0];
+    if (nrhs > 1) e = es.e[s % nrhs];
     Cx<T> res[12];
     if (keep_par >= 0 && ((s >= Vh) ? 1 : 0) != keep_par) {
 #pragma unroll
@@ -421,13 +446,13 @@ __global__ __launch_bounds__(256) void site_kernel(S out, S y, S x, int V, int V
 // host side
 int g_xcd_remap = 1;
 int g_block = 256;
-int g_depth64 = 1;   // pipeline depth of the FP64 hop
+int g_depth64 = 2;   // pipeline depth of the FP64 hop (2 measured faster than 1 at 24^3x48)
 int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 const int *g_skip = nullptr;
 
 template <class S, class G, int B>
 int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
-                 const qkx_gauge_s *U, int dagger, int par_out, const Epi &e, double *norm_partial,
+                 const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, double *norm_partial,
                  int *nblocks_out)
 {
     HopArgs<S, S, S, G> a;
@@ -441,8 +466,9 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     if (aux) qkx::parity_offsets(aux, a.aux_off);
     else a.aux_off[0] = a.aux_off[1] = 0;
     a.par_out = par_out;
-    a.nout = par_out < 0 ? a.g.V : a.g.Vh;
-    a.epi = e;
+    a.nrhs = in->nrhs;
+    a.nout = (par_out < 0 ? a.g.V : a.g.Vh) * a.nrhs;
+    for (int r = 0; r < a.nrhs; r++) a.epi[r] = e[r];
     a.skip = g_skip;
     a.norm_partial = norm_partial;
     int nb = (a.nout + g_block - 1) / g_block;
@@ -452,7 +478,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
             nb = (nb + 7) / 8 * 8;
             a.nblk8 = nb / 8;
         } else {
-            int nbh = ((a.g.Vh + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
+            int nbh = ((a.g.Vh * a.nrhs + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
             a.nblk8 = nbh / 8;
             nb = 2 * nbh;
         }
@@ -486,13 +512,14 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
 }
 
 template <class S, int B>
-int launch_site_t(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int keep_par, const Epi &e)
+int launch_site_t(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int keep_par, const EpiSet &e)
 {
-    int n = out->nsites;
+    int n = entries(out);
     int nb = (n + 255) / 256;
     S so = make_spinor<S>(out), sy = make_spinor<S>(y), sx = x ? make_spinor<S>(x) : so;
     hipLaunchKernelGGL((site_kernel<S, B>), dim3(nb), dim3(256), 0, qkx::g_stream, so, sy, sx, n,
-                       out->subset == QKX_SUBSET_FULL ? out->Vh : n, x ? 1 : 0, keep_par, e);
+                       out->subset == QKX_SUBSET_FULL ? out->Vh * out->nrhs : n, x ? 1 : 0, keep_par, out->nrhs,
+                       e);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -518,7 +545,26 @@ int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, cons
         int basis, int dagger, int par_out, double cR, double gR, double cX, double gX, int g5out,
         double *norm_partial, int *nblocks_out)
 {
-    Epi e{cR, gR, cX, gX, g5out};
+    double coef[QKX_MAX_RHS][4];
+    for (int r = 0; r < QKX_MAX_RHS; r++) {
+        coef[r][0] = cR;
+        coef[r][1] = gR;
+        coef[r][2] = cX;
+        coef[r][3] = gX;
+    }
+    return hop_batch(out, in, aux, U, basis, dagger, par_out, coef, g5out, norm_partial, nblocks_out);
+}
+
+int hop_batch(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
+              int basis, int dagger, int par_out, const double coef[][4], int g5out, double *norm_partial,
+              int *nblocks_out)
+{
+    if (out->nrhs != in->nrhs || (aux && aux->nrhs != in->nrhs)) {
+        set_error("hop: fields hold different numbers of right-hand sides");
+        return QKX_ERR_ARG;
+    }
+    Epi e[QKX_MAX_RHS];
+    for (int r = 0; r < in->nrhs; r++) e[r] = Epi{coef[r][0], coef[r][1], coef[r][2], coef[r][3], g5out};
     const int sp = in->prec;
     if (out->prec != sp || (aux && aux->prec != sp)) {
         set_error("hop: spinor precisions differ");
@@ -563,9 +609,24 @@ int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, cons
 int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
          double cR, double gR, double cX, double gX, int g5out)
 {
-    Epi e{cR, gR, cX, gX, g5out};
-    if (out->prec != y->prec || (x && x->prec != y->prec) || out->nsites != y->nsites ||
-        (x && x->nsites != y->nsites)) {
+    double coef[QKX_MAX_RHS][4];
+    for (int r = 0; r < QKX_MAX_RHS; r++) {
+        coef[r][0] = cR;
+        coef[r][1] = gR;
+        coef[r][2] = cX;
+        coef[r][3] = gX;
+    }
+    return site_batch(out, y, x, basis, keep_par, coef, g5out);
+}
+
+int site_batch(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
+               const double coef[][4], int g5out)
+{
+    EpiSet e;
+    for (int r = 0; r < QKX_MAX_RHS; r++) e.e[r] = Epi{coef[r][0], coef[r][1], coef[r][2], coef[r][3], g5out};
+    if (out->prec != y->prec || (x && x->prec != y->prec) || entries(out) != entries(y) ||
+        (x && entries(x) != entries(y)) || out->nrhs != y->nrhs || (x && x->nrhs != y->nrhs) ||
+        out->subset != y->subset || (x && x->subset != y->subset)) {
         set_error("site op: fields differ in precision or size");
         return QKX_ERR_ARG;
     }

@@ -25,7 +25,7 @@ __device__ __forceinline__ int blocked_to_lex(int s, const Geom &g)
 }
 
 template <class S>
-__global__ void spinor_from_aos(S dst, const double2 *src, Geom g, int par0, int n)
+__global__ void spinor_from_aos(S dst, const double2 *src, Geom g, int par0, int n, int nrhs, int rhs)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -37,18 +37,18 @@ __global__ void spinor_from_aos(S dst, const double2 *src, Geom g, int par0, int
         double2 d = src[(size_t)lex * 12 + k];
         v[k] = {(typename S::Real)d.x, (typename S::Real)d.y};
     }
-    dst.store(i, v);
+    dst.store(i * nrhs + rhs, v);
 }
 
 template <class S>
-__global__ void spinor_to_aos(double2 *dst, S src, Geom g, int par0, int n, int zero_other)
+__global__ void spinor_to_aos(double2 *dst, S src, Geom g, int par0, int n, int zero_other, int nrhs, int rhs)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int s_full = par0 * g.Vh + i;
     int lex = blocked_to_lex(s_full, g);
     Cx<typename S::Real> v[12];
-    src.load(i, v);
+    src.load(i * nrhs + rhs, v);
 #pragma unroll
     for (int k = 0; k < 12; k++) dst[(size_t)lex * 12 + k] = make_double2((double)v[k].re, (double)v[k].im);
     if (zero_other) {
@@ -206,11 +206,11 @@ namespace qkx {
 size_t spinor_elem_bytes(int) { return 16; }
 int spinor_nchunks(int prec) { return prec == QKX_PREC_DOUBLE ? 12 : prec == QKX_PREC_SINGLE ? 6 : 3; }
 
+// storage offset (in entries) of each parity's first site
 void parity_offsets(const qkx_spinor_s *f, int off[2])
 {
-    if (f->subset == QKX_SUBSET_FULL) { off[0] = 0; off[1] = f->Vh; }
-    else if (f->subset == QKX_SUBSET_EVEN) { off[0] = 0; off[1] = 0; }
-    else { off[0] = 0; off[1] = 0; }
+    off[0] = 0;
+    off[1] = f->subset == QKX_SUBSET_FULL ? f->Vh * f->nrhs : 0;
 }
 
 Geom make_geom(const int L[4], int bc)
@@ -228,7 +228,7 @@ Geom make_geom(const int L[4], int bc)
 int zero_parity(qkx_spinor_s *f, int parity)
 {
     if (f->subset != QKX_SUBSET_FULL) return QKX_SUCCESS;
-    int n = f->Vh, shift = parity * f->Vh;
+    int n = f->Vh * f->nrhs, shift = parity * f->Vh * f->nrhs;
     return dispatch_spinor(f, [&](auto S) {
         hipLaunchKernelGGL((spinor_zero_range<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, g_stream, S, n,
                            shift);
@@ -243,12 +243,18 @@ extern "C" {
 
 int qkx_spinor_create(qkx_spinor **out, const int L[4], int prec, int subset)
 {
+    return qkx_spinor_create_batch(out, L, prec, subset, 1);
+}
+
+int qkx_spinor_create_batch(qkx_spinor **out, const int L[4], int prec, int subset, int nrhs)
+{
     int rc = qkx::ensure_init();
     if (rc) return rc;
     QKX_CHECK_ARG(out && L, "qkx_spinor_create: null argument");
     QKX_CHECK_ARG(valid_dims(L), "qkx_spinor_create: every extent must be even and >= 2");
     QKX_CHECK_ARG(prec >= QKX_PREC_DOUBLE && prec <= QKX_PREC_BF16, "qkx_spinor_create: bad precision");
     QKX_CHECK_ARG(subset >= QKX_SUBSET_FULL && subset <= QKX_SUBSET_ODD, "qkx_spinor_create: bad subset");
+    QKX_CHECK_ARG(nrhs >= 1 && nrhs <= QKX_MAX_RHS, "qkx_spinor_create: nrhs out of range");
     auto *f = new qkx_spinor_s();
     for (int m = 0; m < 4; m++) f->L[m] = L[m];
     f->V = L[0] * L[1] * L[2] * L[3];
@@ -256,7 +262,8 @@ int qkx_spinor_create(qkx_spinor **out, const int L[4], int prec, int subset)
     f->prec = prec;
     f->subset = subset;
     f->nsites = subset == QKX_SUBSET_FULL ? f->V : f->Vh;
-    f->bytes = qkx::spinor_elem_bytes(prec) * qkx::spinor_nchunks(prec) * (size_t)f->nsites;
+    f->nrhs = nrhs;
+    f->bytes = qkx::spinor_elem_bytes(prec) * qkx::spinor_nchunks(prec) * (size_t)f->nsites * nrhs;
     if (hipMalloc(&f->data, f->bytes) != hipSuccess) {
         delete f;
         qkx::set_error("qkx_spinor_create: out of device memory");
@@ -264,7 +271,7 @@ int qkx_spinor_create(qkx_spinor **out, const int L[4], int prec, int subset)
     }
     f->nrm = nullptr;
     if (prec == QKX_PREC_HALF) {
-        if (hipMalloc(&f->nrm, sizeof(float) * f->nsites) != hipSuccess) {
+        if (hipMalloc(&f->nrm, sizeof(float) * (size_t)f->nsites * nrhs) != hipSuccess) {
             (void)hipFree(f->data);
             delete f;
             qkx::set_error("qkx_spinor_create: out of device memory");
@@ -288,7 +295,7 @@ int qkx_spinor_destroy(qkx_spinor *f)
 int qkx_spinor_zero(qkx_spinor *f)
 {
     QKX_CHECK_ARG(f, "qkx_spinor_zero: null field");
-    int n = f->nsites;
+    int n = entries(f);
     return dispatch_spinor(f, [&](auto S) {
         hipLaunchKernelGGL((spinor_zero<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream, S, n);
         QKX_HIP(hipGetLastError());
@@ -296,9 +303,13 @@ int qkx_spinor_zero(qkx_spinor *f)
     });
 }
 
-int qkx_spinor_upload(qkx_spinor *f, const double *psi)
+int qkx_spinor_upload(qkx_spinor *f, const double *psi) { return qkx_spinor_upload_rhs(f, 0, psi); }
+int qkx_spinor_download(const qkx_spinor *f, double *psi) { return qkx_spinor_download_rhs(f, 0, psi); }
+
+int qkx_spinor_upload_rhs(qkx_spinor *f, int rhs, const double *psi)
 {
     QKX_CHECK_ARG(f && psi, "qkx_spinor_upload: null argument");
+    QKX_CHECK_ARG(rhs >= 0 && rhs < f->nrhs, "qkx_spinor_upload: rhs out of range");
     size_t bytes = sizeof(double) * 24 * (size_t)f->V;
     void *st = stage(bytes);
     if (!st) { qkx::set_error("qkx_spinor_upload: staging allocation failed"); return QKX_ERR_NOMEM; }
@@ -308,7 +319,7 @@ int qkx_spinor_upload(qkx_spinor *f, const double *psi)
     int n = f->nsites;
     int rc = dispatch_spinor(f, [&](auto S) {
         hipLaunchKernelGGL((spinor_from_aos<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
-                           S, (const double2 *)st, g, par0, n);
+                           S, (const double2 *)st, g, par0, n, f->nrhs, rhs);
         QKX_HIP(hipGetLastError());
         return QKX_SUCCESS;
     });
@@ -317,9 +328,10 @@ int qkx_spinor_upload(qkx_spinor *f, const double *psi)
     return QKX_SUCCESS;
 }
 
-int qkx_spinor_download(const qkx_spinor *f, double *psi)
+int qkx_spinor_download_rhs(const qkx_spinor *f, int rhs, double *psi)
 {
     QKX_CHECK_ARG(f && psi, "qkx_spinor_download: null argument");
+    QKX_CHECK_ARG(rhs >= 0 && rhs < f->nrhs, "qkx_spinor_download: rhs out of range");
     size_t bytes = sizeof(double) * 24 * (size_t)f->V;
     void *st = stage(bytes);
     if (!st) { qkx::set_error("qkx_spinor_download: staging allocation failed"); return QKX_ERR_NOMEM; }
@@ -329,7 +341,7 @@ int qkx_spinor_download(const qkx_spinor *f, double *psi)
     int zero_other = f->subset != QKX_SUBSET_FULL;
     int rc = dispatch_spinor(f, [&](auto S) {
         hipLaunchKernelGGL((spinor_to_aos<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
-                           (double2 *)st, S, g, par0, n, zero_other);
+                           (double2 *)st, S, g, par0, n, zero_other, f->nrhs, rhs);
         QKX_HIP(hipGetLastError());
         return QKX_SUCCESS;
     });
@@ -346,15 +358,17 @@ int qkx_spinor_copy(qkx_spinor *dst, const qkx_spinor *src)
 {
     QKX_CHECK_ARG(dst && src, "qkx_spinor_copy: null argument");
     for (int m = 0; m < 4; m++) QKX_CHECK_ARG(dst->L[m] == src->L[m], "qkx_spinor_copy: lattices differ");
+    QKX_CHECK_ARG(dst->nrhs == src->nrhs, "qkx_spinor_copy: numbers of right-hand sides differ");
+    const int nr = dst->nrhs;
     int n, dshift = 0, sshift = 0;
     if (dst->subset == src->subset) {
-        n = dst->nsites;
+        n = entries(dst);
     } else if (dst->subset != QKX_SUBSET_FULL && src->subset == QKX_SUBSET_FULL) {
-        n = dst->nsites;
-        sshift = dst->subset == QKX_SUBSET_ODD ? src->Vh : 0;
+        n = entries(dst);
+        sshift = dst->subset == QKX_SUBSET_ODD ? src->Vh * nr : 0;
     } else if (dst->subset == QKX_SUBSET_FULL && src->subset != QKX_SUBSET_FULL) {
-        n = src->nsites;
-        dshift = src->subset == QKX_SUBSET_ODD ? dst->Vh : 0;
+        n = entries(src);
+        dshift = src->subset == QKX_SUBSET_ODD ? dst->Vh * nr : 0;
     } else {
         qkx::set_error("qkx_spinor_copy: even and odd subsets cannot be copied onto each other");
         return QKX_ERR_ARG;

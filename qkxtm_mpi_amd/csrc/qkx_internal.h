@@ -421,6 +421,7 @@ struct qkx_spinor_s {
     int prec;     // QKX_PREC_*
     int subset;   // QKX_SUBSET_*
     int nsites;   // V or Vh
+    int nrhs;     // right-hand sides stored innermost: entry (site, rhs) = site*nrhs + rhs
     void *data;   // component-major storage
     float *nrm;   // HALF only
     size_t bytes;
@@ -477,10 +478,12 @@ void prof_end(double bytes, int prec);
 // ---------------------------------------------------------------------------
 // accessor factories and precision dispatch
 template <class S> inline S make_spinor(const qkx_spinor_s *f);
-template <> inline SpinorD make_spinor<SpinorD>(const qkx_spinor_s *f) { return SpinorD{(double2 *)f->data, f->nsites}; }
-template <> inline SpinorF make_spinor<SpinorF>(const qkx_spinor_s *f) { return SpinorF{(float4 *)f->data, f->nsites}; }
-template <> inline SpinorH make_spinor<SpinorH>(const qkx_spinor_s *f) { return SpinorH{(uint4 *)f->data, f->nrm, f->nsites}; }
-template <> inline SpinorB make_spinor<SpinorB>(const qkx_spinor_s *f) { return SpinorB{(uint4 *)f->data, f->nsites}; }
+// storage entries of a field: sites x right-hand sides
+inline int entries(const qkx_spinor_s *f) { return f->nsites * f->nrhs; }
+template <> inline SpinorD make_spinor<SpinorD>(const qkx_spinor_s *f) { return SpinorD{(double2 *)f->data, entries(f)}; }
+template <> inline SpinorF make_spinor<SpinorF>(const qkx_spinor_s *f) { return SpinorF{(float4 *)f->data, entries(f)}; }
+template <> inline SpinorH make_spinor<SpinorH>(const qkx_spinor_s *f) { return SpinorH{(uint4 *)f->data, f->nrm, entries(f)}; }
+template <> inline SpinorB make_spinor<SpinorB>(const qkx_spinor_s *f) { return SpinorB{(uint4 *)f->data, entries(f)}; }
 
 template <class G> inline G make_gauge(const qkx_gauge_s *g);
 template <> inline GaugeD18 make_gauge<GaugeD18>(const qkx_gauge_s *g) { GaugeD18 r; r.p = (const double2 *)g->data; r.V = g->V; return r; }

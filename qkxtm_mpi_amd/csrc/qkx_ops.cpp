@@ -127,7 +127,7 @@ int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, const qkx_
     case QKX_OP_TM_DAG: return qkx::hop(out, in, in, U, basis, 1, -1, -k, 0, 1.0, -a, 0);
     case QKX_OP_WILSON: return qkx::hop(out, in, in, U, basis, 0, -1, -k, 0, 1.0, 0, 0);
     case QKX_OP_TM_DAG_TM: {
-        qkx::Temp t(in->L, in->prec, QKX_SUBSET_FULL);
+        qkx::Temp t(in->L, in->prec, QKX_SUBSET_FULL, in->nrhs);
         if (!t.f) return QKX_ERR_NOMEM;
         int rc = qkx::hop(t.f, in, in, U, basis, 0, -1, -k, 0, 1.0, a, 0);
         if (rc) return rc;
@@ -149,14 +149,14 @@ int qkx_tm_xx(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, const q
     QKX_CHECK_ARG(holds_parity(out, pe) && holds_parity(in, pe), "qkx_tm_xx: field subsets do not match parity");
     double k = p->kappa, a = twist_a(p), n = twist_norm(p);
     int sub_other = pe == 0 ? QKX_SUBSET_ODD : QKX_SUBSET_EVEN;
-    qkx::Temp t(in->L, in->prec, sub_other);
+    qkx::Temp t(in->L, in->prec, sub_other, in->nrhs);
     if (!t.f) return QKX_ERR_NOMEM;
     int rc;
     if (!normal) {
         rc = qkx::schur(out, in, U, k, a, n, basis, pe, 0, t.f);
     } else {
         // tm_xxDagTm_xx :2024-2035: g5 M(-s) g5 M(s)
-        qkx::Temp y(in->L, in->prec, pe == 0 ? QKX_SUBSET_EVEN : QKX_SUBSET_ODD);
+        qkx::Temp y(in->L, in->prec, pe == 0 ? QKX_SUBSET_EVEN : QKX_SUBSET_ODD, in->nrhs);
         if (!y.f) return QKX_ERR_NOMEM;
         rc = qkx::schur(y.f, in, U, k, a, n, basis, pe, 1, t.f);
         if (rc) return rc;

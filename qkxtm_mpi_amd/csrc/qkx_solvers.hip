@@ -260,7 +260,7 @@ struct Engine {
 template <class SX, class SR, class S>
 int enqueue_blas(Engine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap, int npap)
 {
-    int n = p->nsites;
+    int n = entries(p);
     SX X = make_spinor<SX>(x);
     SR R = make_spinor<SR>(r);
     S P = make_spinor<S>(p), AP = make_spinor<S>(Ap);
@@ -303,7 +303,7 @@ int enqueue_iteration(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_
 // beta = |r|^2 / r2_old of the current slot.
 int restart(Engine &E, qkx_spinor_s *p, qkx_spinor_s *r, int mode)
 {
-    int n = r->nsites;
+    int n = entries(r);
     int rc = with_spinor(r, [&](auto R) {
         hipLaunchKernelGGL((norm_partial_kernel<decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, R, n,
                            g_part + g_part_cap);
@@ -345,7 +345,7 @@ int write_slot(const Engine &E, const CGSlot &h)
 
 int launch_accum(qkx_spinor_s *xh, const qkx_spinor_s *xl)
 {
-    int n = xh->nsites;
+    int n = entries(xh);
     return with_spinor(xl, [&](auto S) {
         hipLaunchKernelGGL((accum_kernel<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
                            make_spinor<SpinorD>(xh), S, n);

@@ -1,0 +1,626 @@
+// qkx_solvers_batch.hip -- CG for a batch of right-hand sides in lockstep.
+//
+// The reference solves the twelve spin-colour columns of a propagator one
+// after another with the same gauge field (bin/test_invert_prop.cpp:83-97).
+// Here the columns (or several twisted masses mu on one gauge field) are one
+// batched field with the right-hand side innermost, so every stencil launch
+// serves all of them and reads each link once per site, and every CG launch
+// updates all of them.  Each right-hand side keeps its own CG scalars; one
+// that meets its stopping test is frozen (alpha = 0) while the others go on.
+// Reliable updates and true-residual checks are done for the whole batch.
+//
+// Same launch structure and reduction scheme as qkx_solvers.hip: per-workgroup
+// partials (one per right-hand side), added by every workgroup of the consuming
+// launch in a fixed order, so all workgroups take identical decisions.
+#include "qkx_internal.h"
+#include "qkx_core.h"
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int NR = QKX_MAX_RHS;
+
+struct BSlot {
+    double r2[NR];      // iterated |r|^2
+    double r2_old[NR];
+    double pAp[NR];
+    double maxr2[NR];   // |r|^2 at the last reliable update
+    double stop2[NR];   // iterated stopping threshold per right-hand side
+    int active[NR];     // 0: converged (true residual met), frozen
+    int flag;           // 0 run, 1 reliable update due, 2 all below stop2, 3 breakdown
+    int k;
+};
+
+BSlot *g_bslots = nullptr, *g_bslots_host = nullptr;
+double *g_bpart = nullptr;
+int g_bpart_cap = 0;
+double *g_bres = nullptr, *g_bres_host = nullptr;
+
+int ensure_bstate(int nblocks)
+{
+    if (!g_bslots) {
+        QKX_HIP(hipMalloc(&g_bslots, 2 * sizeof(BSlot)));
+        QKX_HIP(hipHostMalloc((void **)&g_bslots_host, sizeof(BSlot), hipHostMallocDefault));
+        QKX_HIP(hipMalloc(&g_bres, NR * sizeof(double)));
+        QKX_HIP(hipHostMalloc((void **)&g_bres_host, NR * sizeof(double), hipHostMallocDefault));
+    }
+    size_t need = (size_t)nblocks * NR;
+    if ((int)need > g_bpart_cap) {
+        if (g_bpart) (void)hipFree(g_bpart);
+        g_bpart = nullptr;
+        QKX_HIP(hipMalloc(&g_bpart, sizeof(double) * 2 * need));
+        g_bpart_cap = (int)need;
+    }
+    return QKX_SUCCESS;
+}
+
+// sums[r] = sum_b part[b*nr + r] for r < nr, in a fixed order; every thread
+// of the workgroup sees the same values.
+__device__ void block_sums_rhs(const double *part, int nblocks, int nr, double *sums)
+{
+    __shared__ double lds[kT];
+    const int groups = blockDim.x / nr;
+    const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+    double s = 0;
+    if (g < groups)
+        for (int b = g; b < nblocks; b += groups) s += part[(size_t)b * nr + r];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+        double t = 0;
+        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
+        sums[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// per-workgroup, per-rhs partial of v (rr = this thread's right-hand side)
+__device__ void block_partial_rhs(double v, int rr, bool valid, int nr, double *dst)
+{
+    __shared__ double acc[NR];
+    if (threadIdx.x < NR) acc[threadIdx.x] = 0.0;
+    __syncthreads();
+    if (valid) atomicAdd(&acc[rr], v);
+    __syncthreads();
+    if ((int)threadIdx.x < nr) dst[(size_t)blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+}
+
+__device__ __forceinline__ bool frozen(const BSlot *c, int r) { return !c->active[r] || c->r2[r] <= c->stop2[r]; }
+
+template <class SX, class SR, class S>
+__global__ __launch_bounds__(kT) void cgb_update_kernel(SX x, SR r, S p, S Ap, int n, int nr, const BSlot *cur,
+                                                        const double *pap_part, int npap, double *r2_part)
+{
+    if (cur->flag) return;
+    __shared__ double pap[NR];
+    block_sums_rhs(pap_part, npap, nr, pap);
+    using T = typename SX::Real;
+    using TR = typename SR::Real;
+    using TP = typename S::Real;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rr = u % nr;
+    const double alpha = frozen(cur, rr) ? 0.0 : cur->r2[rr] / pap[rr];
+    double acc = 0;
+    const bool valid = u < n;
+    if (valid) {
+        const T a = T(alpha);
+        const TR ar = TR(alpha);
+        Cx<T> xv[12];
+        Cx<TR> rv[12];
+        Cx<TP> pv[12], av[12];
+        x.load(u, xv);
+        p.load(u, pv);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + a * T(pv[k].re), xv[k].im + a * T(pv[k].im)};
+        x.store(u, xv);
+        r.load(u, rv);
+        Ap.load(u, av);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            rv[k] = {rv[k].re - ar * TR(av[k].re), rv[k].im - ar * TR(av[k].im)};
+            acc += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+        }
+        r.store(u, rv);
+    }
+    block_partial_rhs(acc, rr, valid, nr, r2_part);
+}
+
+// mode 0: iteration; mode 1: restart after a reliable update (beta from r2_old)
+template <class S, class SR>
+__global__ __launch_bounds__(kT) void cgb_dir_kernel(S p, SR r, int n, int nr, const BSlot *cur, BSlot *next,
+                                                     const double *r2_part, int nr2, const double *pap_part, int npap,
+                                                     double delta2, int mode)
+{
+    if (mode == 0 && cur->flag) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *next = *cur;
+        return;
+    }
+    __shared__ double r2n[NR], pap[NR];
+    __shared__ int flag_s;
+    block_sums_rhs(r2_part, nr2, nr, r2n);
+    if (mode == 0) block_sums_rhs(pap_part, npap, nr, pap);
+    if (threadIdx.x == 0) {
+        int flag = 0;
+        if (mode == 0) {
+            bool all_frozen = true, any_rel = false, bad = false;
+            for (int q = 0; q < nr; q++) {
+                if (frozen(cur, q)) continue;
+                if (!(pap[q] > 0.0) || !isfinite(r2n[q])) bad = true;
+                if (cur->active[q] && r2n[q] > cur->stop2[q]) all_frozen = false;
+                if (r2n[q] < delta2 * cur->maxr2[q]) any_rel = true;
+            }
+            flag = bad ? 3 : all_frozen ? 2 : any_rel ? 1 : 0;
+        } else {
+            for (int q = 0; q < nr; q++)
+                if (!isfinite(r2n[q])) flag = 3;
+        }
+        flag_s = flag;
+    }
+    __syncthreads();
+    const int flag = flag_s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BSlot nx = *cur;
+        for (int q = 0; q < nr; q++) {
+            nx.r2[q] = r2n[q];
+            if (mode == 0) {
+                nx.r2_old[q] = frozen(cur, q) ? 0.0 : cur->r2[q];   // 0: restart from p = r
+                nx.pAp[q] = pap[q];
+            } else {
+                nx.maxr2[q] = r2n[q];
+            }
+        }
+        nx.flag = flag;
+        if (mode == 0) nx.k = cur->k + 1;
+        *next = nx;
+    }
+    if (flag) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int rr = u % nr;
+    double beta;
+    if (mode == 0) beta = frozen(cur, rr) ? 0.0 : r2n[rr] / cur->r2[rr];
+    else beta = cur->r2_old[rr] > 0 ? r2n[rr] / cur->r2_old[rr] : 0.0;
+    using T = typename SR::Real;
+    const T b = T(beta);
+    Cx<typename S::Real> pv[12];
+    Cx<T> rv[12];
+    p.load(u, pv);
+    r.load(u, rv);
+    Cx<typename S::Real> out[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        out[k] = {(typename S::Real)(rv[k].re + b * T(pv[k].re)), (typename S::Real)(rv[k].im + b * T(pv[k].im))};
+    p.store(u, out);
+}
+
+template <class S>
+__global__ __launch_bounds__(kT) void normb_partial_kernel(S r, int n, int nr, double *part)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0;
+    const bool valid = u < n;
+    if (valid) {
+        Cx<typename S::Real> a[12];
+        r.load(u, a);
+#pragma unroll
+        for (int k = 0; k < 12; k++) acc += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
+    }
+    block_partial_rhs(acc, u % nr, valid, nr, part);
+}
+
+__global__ __launch_bounds__(kT) void normb_finish_kernel(const double *part, int nblocks, int nr, double *res)
+{
+    __shared__ double s[NR];
+    block_sums_rhs(part, nblocks, nr, s);
+    if ((int)threadIdx.x < nr) res[threadIdx.x] = s[threadIdx.x];
+}
+
+template <class SL>
+__global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    Cx<double> a[12];
+    Cx<typename SL::Real> b[12];
+    xh.load(s, a);
+    xl.load(s, b);
+#pragma unroll
+    for (int k = 0; k < 12; k++) a[k] = {a[k].re + (double)b[k].re, a[k].im + (double)b[k].im};
+    xh.store(s, a);
+}
+
+int nblk(int n) { return (n + kT - 1) / kT; }
+
+template <class F> int with_spinor(const qkx_spinor_s *f, F fn)
+{
+    return dispatch_prec(f->prec, [&](auto tag) { return fn(make_spinor<decltype(tag)>(f)); });
+}
+
+// per-rhs squared norms of a batched field, to the host
+int norms(const qkx_spinor_s *f, std::vector<double> &out)
+{
+    const int n = entries(f), nr = f->nrhs, nb = nblk(n);
+    int rc = with_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((normb_partial_kernel<decltype(S)>), dim3(nb), dim3(kT), 0, qkx::g_stream, S, n, nr,
+                           g_bpart + g_bpart_cap);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(normb_finish_kernel, dim3(1), dim3(kT), 0, qkx::g_stream, g_bpart + g_bpart_cap, nb, nr, g_bres);
+    QKX_HIP(hipGetLastError());
+    QKX_HIP(hipMemcpyAsync(g_bres_host, g_bres, nr * sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    out.assign(g_bres_host, g_bres_host + nr);
+    return QKX_SUCCESS;
+}
+
+// Batched operator with per-rhs twist a_r and A-normalisation n_r
+struct BatchOp {
+    const qkx_gauge_s *U;
+    double kappa;
+    double a[NR], nrm[NR];
+    int nr;
+    bool eo;
+    qkx_spinor_s *tmp_full, *tmp_odd, *tmp_even;
+
+    // Schur operator (tm_xx, lib/lattice_util.cpp:2001-2022) with twist sign sg
+    int schur(qkx_spinor_s *out, const qkx_spinor_s *in, double sg, int g5out, double *part = nullptr,
+              int *np = nullptr) const
+    {
+        double c1[NR][4], c2[NR][4];
+        for (int r = 0; r < NR; r++) {
+            const int q = r < nr ? r : 0;
+            const double ar = sg * a[q];
+            c1[r][0] = 1.0 / nrm[q];
+            c1[r][1] = -ar / nrm[q];
+            c1[r][2] = 0;
+            c1[r][3] = 0;
+            c2[r][0] = -kappa * kappa;
+            c2[r][1] = 0;
+            c2[r][2] = 1.0;
+            c2[r][3] = ar;
+        }
+        int rc = qkx::hop_batch(tmp_odd, in, nullptr, U, QKX_BASIS_UKQCD, 0, 1, c1, 0);
+        if (rc) return rc;
+        return qkx::hop_batch(out, tmp_odd, in, U, QKX_BASIS_UKQCD, 0, 0, c2, g5out, part, np);
+    }
+    // full-lattice D (dag = 0) or D^dag (dag = 1), aux = in
+    int dirac(qkx_spinor_s *out, const qkx_spinor_s *in, int dag, double *part = nullptr, int *np = nullptr) const
+    {
+        double c[NR][4];
+        for (int r = 0; r < NR; r++) {
+            const int q = r < nr ? r : 0;
+            c[r][0] = -kappa;
+            c[r][1] = 0;
+            c[r][2] = 1.0;
+            c[r][3] = dag ? -a[q] : a[q];
+        }
+        return qkx::hop_batch(out, in, in, U, QKX_BASIS_UKQCD, dag, -1, c, 0, part, np);
+    }
+    int apply(qkx_spinor_s *out, const qkx_spinor_s *in, double *part, int *np) const
+    {
+        if (!eo) {
+            int rc = dirac(tmp_full, in, 0, part, np);
+            if (rc) return rc;
+            return dirac(out, tmp_full, 1);
+        }
+        int rc = schur(tmp_even, in, +1, 1, part, np);
+        if (rc) return rc;
+        return schur(out, tmp_even, -1, 1);
+    }
+    int apply_op(qkx_spinor_s *out, const qkx_spinor_s *in) const
+    {
+        return eo ? schur(out, in, +1, 0) : dirac(out, in, 0);
+    }
+    int apply_dag(qkx_spinor_s *out, const qkx_spinor_s *in) const
+    {
+        if (!eo) return dirac(out, in, 1);
+        int rc = qkx::site(tmp_even, in, nullptr, QKX_BASIS_UKQCD, -1, 1.0, 0, 0, 0, 1);
+        if (rc) return rc;
+        return schur(out, tmp_even, -1, 1);
+    }
+};
+
+struct BEngine {
+    int j = 0;
+    double delta2 = 0;
+    BSlot *cur() const { return g_bslots + (j & 1); }
+    BSlot *next() const { return g_bslots + ((j + 1) & 1); }
+};
+
+int read_bslot(const BEngine &E, BSlot &h)
+{
+    QKX_HIP(hipMemcpyAsync(g_bslots_host, E.cur(), sizeof(BSlot), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    h = *g_bslots_host;
+    return QKX_SUCCESS;
+}
+
+int write_bslot(const BEngine &E, const BSlot &h)
+{
+    *g_bslots_host = h;
+    QKX_HIP(hipMemcpyAsync(E.cur(), g_bslots_host, sizeof(BSlot), hipMemcpyHostToDevice, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
+template <class SX, class SR, class S>
+int enqueue_bblas(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap, int npap)
+{
+    const int n = entries(p), nr = p->nrhs;
+    double *pap_part = g_bpart, *r2_part = g_bpart + g_bpart_cap;
+    SX X = make_spinor<SX>(x);
+    SR R = make_spinor<SR>(r);
+    S P = make_spinor<S>(p), AP = make_spinor<S>(Ap);
+    hipLaunchKernelGGL((cgb_update_kernel<SX, SR, S>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, X, R, P, AP, n, nr,
+                       E.cur(), pap_part, npap, r2_part);
+    hipLaunchKernelGGL((cgb_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, P, R, n, nr, E.cur(),
+                       E.next(), r2_part, nblk(n), pap_part, npap, E.delta2, 0);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
+int enqueue_biter(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
+                  qkx_spinor_s *Ap)
+{
+    qkx::set_hop_skip(&E.cur()->flag);
+    int npap = 0;
+    int rc = A.apply(Ap, p, g_bpart, &npap);
+    qkx::set_hop_skip(nullptr);
+    if (rc) return rc;
+    const int px = x->prec, pr = r->prec, pp = p->prec;
+    if (px == QKX_PREC_DOUBLE && pr == QKX_PREC_DOUBLE && pp == QKX_PREC_DOUBLE)
+        rc = enqueue_bblas<SpinorD, SpinorD, SpinorD>(E, x, r, p, Ap, npap);
+    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_SINGLE)
+        rc = enqueue_bblas<SpinorF, SpinorF, SpinorF>(E, x, r, p, Ap, npap);
+    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_HALF && pp == QKX_PREC_HALF)
+        rc = enqueue_bblas<SpinorF, SpinorH, SpinorH>(E, x, r, p, Ap, npap);
+    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_BF16)
+        rc = enqueue_bblas<SpinorF, SpinorF, SpinorB>(E, x, r, p, Ap, npap);
+    else {
+        qkx::set_error("batched cg: unsupported precision combination");
+        return QKX_ERR_ARG;
+    }
+    E.j++;
+    return rc;
+}
+
+int brestart(BEngine &E, qkx_spinor_s *p, qkx_spinor_s *r, int initial)
+{
+    const int n = entries(r), nr = r->nrhs;
+    int rc = with_spinor(r, [&](auto R) {
+        hipLaunchKernelGGL((normb_partial_kernel<decltype(R)>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, R, n, nr,
+                           g_bpart + g_bpart_cap);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+    if (rc) return rc;
+    if (initial && (rc = qkx_spinor_copy(p, r))) return rc;
+    rc = with_spinor(p, [&](auto P) {
+        return with_spinor(r, [&](auto R) {
+            hipLaunchKernelGGL((cgb_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream,
+                               P, R, initial ? 0 : n, nr, E.cur(), E.next(), g_bpart + g_bpart_cap, nblk(n), g_bpart,
+                               0, 0.0, 1);
+            QKX_HIP(hipGetLastError());
+            return QKX_SUCCESS;
+        });
+    });
+    E.j++;
+    return rc;
+}
+
+} // namespace
+
+namespace qkx {
+extern int cg_batch_size;
+}
+
+extern "C" {
+
+int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_gauge *U_inner_arg,
+                    const qkx_params *p, const double *mu, const qkx_solve_opts *o, qkx_solver_stats *st)
+{
+    QKX_CHECK_ARG(x && b && U && p && o, "qkx_solve_batch: null argument");
+    QKX_CHECK_ARG(x->prec == QKX_PREC_DOUBLE && b->prec == QKX_PREC_DOUBLE && x->subset == QKX_SUBSET_FULL &&
+                      b->subset == QKX_SUBSET_FULL && x->nrhs == b->nrhs,
+                  "qkx_solve_batch: x and b must be full FP64 batches of the same size");
+    QKX_CHECK_ARG(U->prec == QKX_PREC_DOUBLE && U->recon == QKX_RECON_18, "qkx_solve_batch: U must be FP64 recon 18");
+    QKX_CHECK_ARG(o->tol > 0 && o->maxIter > 0, "qkx_solve_batch: bad tol/maxIter");
+    const int nr = b->nrhs;
+    const int *L = b->L;
+    if (int rc = ensure_bstate(b->V * nr / kT + 64)) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    const bool mixed = o->inner_prec != QKX_PREC_DOUBLE;
+    const int lp = o->inner_prec;
+    const int sub = o->eo ? QKX_SUBSET_EVEN : QKX_SUBSET_FULL;
+    int rc;
+
+    const qkx_gauge_s *Ui = U;
+    if (mixed) {
+        if (U_inner_arg) {
+            Ui = U_inner_arg;
+        } else {
+            qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
+            int want_recon = o->inner_recon == QKX_RECON_18 ? QKX_RECON_18 : QKX_RECON_12;
+            if (lp == QKX_PREC_HALF || lp == QKX_PREC_BF16) want_recon = QKX_RECON_12;
+            if (!Uc->inner || Uc->inner->prec != lp || Uc->inner->recon != want_recon) {
+                if (Uc->inner) qkx_gauge_destroy(Uc->inner);
+                Uc->inner = nullptr;
+                if ((rc = qkx_gauge_create(&Uc->inner, L, lp, want_recon, U->bc))) return rc;
+                if ((rc = qkx_gauge_convert(Uc->inner, U))) return rc;
+            }
+            Ui = Uc->inner;
+        }
+    }
+
+    BatchOp Ah{}, Al{};
+    Ah.U = U;
+    Al.U = Ui;
+    Ah.kappa = Al.kappa = p->kappa;
+    Ah.nr = Al.nr = nr;
+    Ah.eo = Al.eo = o->eo != 0;
+    for (int r = 0; r < NR; r++) {
+        const double m = (mu && r < nr) ? mu[r] : p->mu;
+        Ah.a[r] = Al.a[r] = 2.0 * p->kappa * m * p->twistSign;
+        Ah.nrm[r] = Al.nrm[r] = 1.0 + 4.0 * p->kappa * p->kappa * m * m;
+    }
+    auto T = [&](int prec, int subset) { return qkx::pool_get_batch(L, prec, subset, nr); };
+    qkx_spinor_s *hf = T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL), *ho = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
+                 *he = T(QKX_PREC_DOUBLE, QKX_SUBSET_EVEN), *lf = T(lp, QKX_SUBSET_FULL),
+                 *lo = T(lp, QKX_SUBSET_ODD), *le = T(lp, QKX_SUBSET_EVEN);
+    const int xp = lp == QKX_PREC_DOUBLE ? QKX_PREC_DOUBLE : QKX_PREC_SINGLE;
+    const int rp = lp == QKX_PREC_BF16 ? QKX_PREC_SINGLE : lp;
+    qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub), *xs = T(QKX_PREC_DOUBLE, sub),
+                 *s = T(QKX_PREC_DOUBLE, sub), *rh = T(QKX_PREC_DOUBLE, sub), *bo = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
+                 *to = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD), *r = T(rp, sub), *pp = T(lp, sub), *Ap = T(lp, sub),
+                 *xl = T(xp, sub), *tf = T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL);
+    std::vector<qkx_spinor_s *> temps = {hf, ho, he, lf, lo, le, bh, bn, xs, s, rh, bo, to, r, pp, Ap, xl, tf};
+    struct Release {
+        std::vector<qkx_spinor_s *> &t;
+        ~Release()
+        {
+            for (auto f : t)
+                if (f) qkx::pool_put(f);
+        }
+    } release{temps};
+    for (auto f : temps)
+        if (!f) return QKX_ERR_NOMEM;
+    Ah.tmp_full = hf; Ah.tmp_odd = ho; Ah.tmp_even = he;
+    Al.tmp_full = lf; Al.tmp_odd = lo; Al.tmp_even = le;
+
+    std::vector<double> b2, bn2, s2;
+    if ((rc = norms(b, b2))) return rc;
+    for (int q = 0; q < nr; q++)
+        if (!std::isfinite(b2[q])) { qkx::set_error("qkx_solve_batch: non-finite source"); return QKX_ERR_NONFINITE; }
+
+    if (o->eo) {
+        double c[NR][4];
+        for (int q = 0; q < NR; q++) {
+            const int z = q < nr ? q : 0;
+            c[q][0] = 1.0 / Ah.nrm[z];
+            c[q][1] = -Ah.a[z] / Ah.nrm[z];
+            c[q][2] = c[q][3] = 0;
+        }
+        if ((rc = qkx_spinor_copy(bh, b)) || (rc = qkx_spinor_copy(bo, b))) return rc;
+        if ((rc = qkx::site_batch(to, bo, nullptr, QKX_BASIS_UKQCD, -1, c, 0))) return rc;
+        if ((rc = qkx::hop(bh, to, bh, U, QKX_BASIS_UKQCD, 0, 0, p->kappa, 0, 1.0, 0, 0))) return rc;
+    } else {
+        if ((rc = qkx_spinor_copy(bh, b))) return rc;
+    }
+    if ((rc = Ah.apply_dag(bn, bh))) return rc;
+    if ((rc = qkx_spinor_zero(xs)) || (rc = qkx_spinor_zero(xl))) return rc;
+    if ((rc = qkx_spinor_copy(r, bn))) return rc;
+    if ((rc = norms(bn, bn2))) return rc;
+
+    const double tol2 = o->tol * o->tol;
+    BEngine E;
+    E.delta2 = mixed ? o->delta * o->delta : 0.0;
+    BSlot h{};
+    for (int q = 0; q < nr; q++) {
+        h.active[q] = b2[q] > 0 ? 1 : 0;
+        h.stop2[q] = tol2 * bn2[q];
+    }
+    if ((rc = write_bslot(E, h))) return rc;
+    if ((rc = brestart(E, pp, r, 1))) return rc;
+    if ((rc = read_bslot(E, h))) return rc;
+
+    int status = QKX_NOT_CONVERGED;
+    int nrel = 0;
+    std::vector<double> true_res(nr, -1.0);
+    qkx_spinor_s *xiter = mixed ? xl : xs;
+    auto all_done = [&]() {
+        for (int q = 0; q < nr; q++)
+            if (h.active[q]) return false;
+        return true;
+    };
+    if (all_done()) status = QKX_SUCCESS;
+    while (status != QKX_SUCCESS && h.k < o->maxIter) {
+        int batch = std::min(qkx::cg_batch_size, o->maxIter - h.k);
+        for (int i = 0; i < batch; i++)
+            if ((rc = enqueue_biter(E, Al, xiter, r, pp, Ap))) return rc;
+        if ((rc = read_bslot(E, h))) return rc;
+        if (h.flag == 0) continue;
+        if (h.flag == 3) {
+            qkx::set_error("qkx_solve_batch: breakdown (p.Ap <= 0 or non-finite residual)");
+            status = QKX_ERR_BREAKDOWN;
+            break;
+        }
+        if (mixed) {
+            const int n = entries(xs);
+            rc = with_spinor(xl, [&](auto S) {
+                hipLaunchKernelGGL((accumb_kernel<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
+                                   make_spinor<SpinorD>(xs), S, n);
+                QKX_HIP(hipGetLastError());
+                return QKX_SUCCESS;
+            });
+            if (rc || (rc = qkx_spinor_zero(xl))) return rc;
+        }
+        if ((rc = Ah.apply_op(s, xs))) return rc;
+        if ((rc = qkx::axpby_dev(s, bh, -1.0, 1.0, nullptr, 0))) return rc;
+        if ((rc = norms(s, s2))) return rc;
+        bool nonfinite = false;
+        for (int q = 0; q < nr; q++) {
+            if (!std::isfinite(s2[q])) nonfinite = true;
+            true_res[q] = std::sqrt(s2[q] / b2[q]);
+            if (h.active[q] && s2[q] <= tol2 * b2[q]) h.active[q] = 0;
+            else if (h.active[q] && h.r2[q] <= h.stop2[q])
+                h.stop2[q] = h.stop2[q] * std::max(1e-4, 0.5 * tol2 * b2[q] / s2[q]);
+        }
+        if (nonfinite) { status = QKX_ERR_NONFINITE; qkx::set_error("qkx_solve_batch: non-finite residual"); break; }
+        if (all_done()) { status = QKX_SUCCESS; break; }
+        h.flag = 0;
+        if ((rc = write_bslot(E, h))) return rc;
+        if (mixed) {
+            nrel++;
+            if ((rc = Ah.apply_dag(rh, s))) return rc;
+            if ((rc = qkx_spinor_copy(r, rh))) return rc;
+        }
+        if ((rc = brestart(E, pp, r, 0))) return rc;
+        if ((rc = read_bslot(E, h))) return rc;
+    }
+    if (status == QKX_NOT_CONVERGED) qkx::set_error("qkx_solve_batch: maxIter reached");
+
+    // assemble: x_o = A^-1 (b_o + kappa H_oe x_e), per right-hand side
+    if (o->eo) {
+        double c[NR][4];
+        for (int q = 0; q < NR; q++) {
+            const int z = q < nr ? q : 0;
+            const double ar = Ah.a[z], nz = Ah.nrm[z];
+            c[q][0] = p->kappa / nz;
+            c[q][1] = -ar * p->kappa / nz;
+            c[q][2] = 1.0 / nz;
+            c[q][3] = -ar / nz;
+        }
+        if ((rc = qkx::hop_batch(to, xs, bo, U, QKX_BASIS_UKQCD, 0, 1, c, 0))) return rc;
+        if ((rc = qkx_spinor_copy(x, xs)) || (rc = qkx_spinor_copy(x, to))) return rc;
+    } else {
+        if ((rc = qkx_spinor_copy(x, xs))) return rc;
+    }
+    // true residuals of the full system, FP64
+    {
+        BatchOp F = Ah;
+        F.eo = false;
+        if ((rc = F.dirac(tf, x, 0))) return rc;
+        if ((rc = qkx::axpby_dev(tf, b, -1.0, 1.0, nullptr, 0))) return rc;
+        std::vector<double> t2;
+        if ((rc = norms(tf, t2))) return rc;
+        for (int q = 0; q < nr; q++) true_res[q] = b2[q] > 0 ? std::sqrt(t2[q] / b2[q]) : 0.0;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st)
+        for (int q = 0; q < nr; q++) {
+            st[q].iterations = h.k;
+            st[q].reliable_updates = nrel;
+            st[q].converged = status == QKX_SUCCESS;
+            st[q].iterated_r2 = h.r2[q];
+            st[q].true_residual = true_res[q];
+            st[q].seconds = secs;
+        }
+    return status;
+}
+
+} // extern "C"

@@ -1,0 +1,175 @@
+"""GPU batches of right-hand sides: every operator applied to a batch field
+against the oracle applied to each right-hand side, batch results against the
+same operator on single fields, and the lockstep batched solver (per-rhs
+twisted mass, frozen converged right-hand sides, zero sources) against the
+oracle's true residual and the reference CG solution."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from golden_util import probes, summarize
+from refops import rel
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+LS = (4, 4, 4, 8)
+KS, MS = 0.153, 0.07
+
+
+def _srcs(q, L, n, seed=100):
+    return np.stack([q.gen_gaussian(L, seed + r) for r in range(n)])
+
+
+@pytest.fixture(scope="module")
+def small(q):
+    U = q.gen_gauge(LS, 0.4, 21)
+    return U, orc.apply_bc(U, LS, -1)
+
+
+@pytest.mark.parametrize("nrhs", [2, 5, 16])
+def test_batch_operators_match_oracle(q, small, nrhs):
+    U, Ub = small
+    u = q.Gauge(LS, boundaryT=-1, host=U)
+    p = q.params(LS, KS, MS, 1, -1)
+    src = _srcs(q, LS, nrhs)
+    print(f"batch ops L={LS} nrhs={nrhs}", flush=True)
+    b = q.Spinor(LS, host=src, nrhs=nrhs)
+    out = q.Spinor(LS, nrhs=nrhs)
+    cases = [
+        ("tm", lambda: out.tm(b, u, p), lambda s: orc.tm(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.TM)),
+        ("tm_chiral", lambda: out.tm(b, u, p, q.CHIRAL), lambda s: orc.tm(s, Ub, LS, KS, MS, 1, orc.CHIRAL, orc.TM)),
+        ("tmDag", lambda: out.tmDag(b, u, p), lambda s: orc.tm(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.TM_DAG)),
+        ("tmDagtm", lambda: out.tmDagtm(b, u, p), lambda s: orc.tm(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.TM_DAG_TM)),
+        ("Wilson", lambda: out.Wilson(b, u, p), lambda s: orc.tm(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.WILSON)),
+        ("dslash", lambda: out.applyDslash(b, u), lambda s: orc.hop(s, Ub, LS, orc.UKQCD, 0, orc.ALL)),
+        ("tm_xx_EE", lambda: out.tm_xx(b, u, p, q.EE), lambda s: orc.tm_xx(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.EE, 0)),
+        ("tm_xxDagTm_xx_OO", lambda: out.tm_xxDagTm_xx(b, u, p, q.OO),
+         lambda s: orc.tm_xx(s, Ub, LS, KS, MS, 1, orc.UKQCD, orc.OO, 1)),
+        ("twistInv", lambda: out.twist(b, p, q.TWIST_INV), lambda s: orc.twist(s, LS, KS, MS, 1, orc.UKQCD,
+                                                                              orc.TWIST_INV)),
+    ]
+    for name, run, ref in cases:
+        run()
+        got = out.download()
+        for r in range(nrhs):
+            e = rel(got[r], ref(src[r]))
+            assert e <= 1e-13, f"{name} rhs {r}: rel err {e:.2e}"
+
+
+@pytest.mark.parametrize("prec", ["SINGLE", "HALF", "BF16"])
+def test_batch_normal_op_equals_single_fields(q, small, prec):
+    """A batch field and single fields give bit-identical results (each
+    right-hand side sees the same arithmetic)."""
+    U, _ = small
+    P = getattr(q, prec)
+    u = q.Gauge(LS, boundaryT=-1, host=U)
+    ul = q.Gauge(LS, P, q.RECON_12, -1)
+    ul.convert_from(u)
+    p = q.params(LS, KS, MS, 1, -1)
+    nrhs = 6
+    src = _srcs(q, LS, nrhs, 300)
+    b = q.Spinor(LS, P, q.EVEN, host=src, nrhs=nrhs)
+    out = q.Spinor(LS, P, q.EVEN, nrhs=nrhs)
+    out.tm_xxDagTm_xx(b, ul, p, q.EE)
+    got = out.download()
+    for r in range(nrhs):
+        bs = q.Spinor(LS, P, q.EVEN, host=src[r])
+        os_ = q.Spinor(LS, P, q.EVEN)
+        os_.tm_xxDagTm_xx(bs, ul, p, q.EE)
+        assert np.array_equal(got[r], os_.download()), f"{prec} rhs {r}"
+
+
+def test_batch_rejects_mismatched_fields(q, small):
+    U, _ = small
+    u = q.Gauge(LS, boundaryT=-1, host=U)
+    p = q.params(LS, KS, MS, 1, -1)
+    a = q.Spinor(LS, nrhs=3)
+    c = q.Spinor(LS, nrhs=2)
+    with pytest.raises(q.QkxError):
+        c.tm(a, u, p)
+    with pytest.raises(q.QkxError):
+        q.Spinor(LS, nrhs=q.MAX_RHS + 1)
+
+
+# config-1 lattice: 8^3x16, eps 0.4, kappa 0.156, antiperiodic t
+L1 = (8, 8, 8, 16)
+K1, M1 = 0.156, 0.01
+
+
+@pytest.fixture(scope="module")
+def cfg1(q):
+    U = q.gen_gauge(L1, 0.4, 1)
+    return U, orc.apply_bc(U, L1, -1)
+
+
+@pytest.mark.parametrize("eo,prec", [(True, "DOUBLE"), (False, "DOUBLE"), (True, "SINGLE"), (True, "HALF"),
+                                     (True, "BF16")])
+def test_solve_batch_per_rhs_mu(q, cfg1, eo, prec):
+    """Four right-hand sides, four twisted masses on one gauge field: each
+    solution meets 1e-10 against the oracle operator with its own mu; the
+    first (config-1 source and mu) also matches the reference CG solution."""
+    U, Ub = cfg1
+    g = np.load(os.path.join(GOLD, "cg_cfg1.npz"), allow_pickle=False)
+    mus = np.array([M1, 0.02, 0.05, 0.1])
+    src = np.stack([q.gen_z2(L1, 2)] + [q.gen_z2(L1, 40 + r) for r in range(3)])
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1)
+    x, b = q.Spinor(L1, nrhs=4), q.Spinor(L1, host=src, nrhs=4)
+    delta = {"DOUBLE": 0.1, "SINGLE": 0.01, "HALF": 0.05, "BF16": 0.1}[prec]
+    print(f"solve_batch eo={eo} prec={prec}", flush=True)
+    st = q.solve_batch(x, b, u, p, mu=mus, eo=eo, inner_prec=getattr(q, prec), delta=delta, tol=1e-10,
+                       maxIter=20000)
+    xs = x.download()
+    for r in range(4):
+        res = rel(orc.tm(xs[r], Ub, L1, K1, mus[r], 1), src[r])
+        print(f"  rhs {r} mu={mus[r]} res={res:.2e} st={st[r].true_residual:.2e} it={st[r].iterations}"
+              f" rel={st[r].reliable_updates}", flush=True)
+        assert st[r].converged and res <= 1e-10
+        assert abs(st[r].true_residual - res) <= 1e-3 * res + 1e-14
+    R, sites = probes(L1)
+    s = summarize(xs[0], R, sites)
+    n = np.sqrt(float(g["norm2_cg_x"]))
+    assert abs(np.sqrt(s["norm2"]) - n) <= 1e-9 * n
+    assert rel(s["sites"], g["site_cg_x"]) <= 1e-8
+
+
+def test_solve_batch_deterministic_and_zero_source(q, cfg1):
+    """A zero right-hand side gives x = 0 without stalling the others, and two
+    runs give bit-identical solutions (fixed-order reductions)."""
+    U, Ub = cfg1
+    src = np.stack([q.gen_z2(L1, 7), np.zeros((int(np.prod(L1)), 4, 3), complex), q.gen_z2(L1, 8)])
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1)
+    b = q.Spinor(L1, host=src, nrhs=3)
+    outs = []
+    for _ in range(2):
+        x = q.Spinor(L1, nrhs=3)
+        st = q.solve_batch(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.05, tol=1e-10)
+        outs.append(x.download())
+        assert all(s.converged for s in st)
+    assert np.array_equal(outs[0], outs[1])
+    assert not np.any(outs[0][1])
+    for r in (0, 2):
+        assert rel(orc.tm(outs[0][r], Ub, L1, K1, M1, 1), src[r]) <= 1e-10
+
+
+def test_solve_batch_propagator_point_sources(q, cfg1):
+    """The 12 spin-colour point sources of a propagator in one batch (the
+    reference's test_invert_prop loop) at FP16 inner precision."""
+    U, Ub = cfg1
+    V = int(np.prod(L1))
+    src = np.zeros((12, V, 4, 3), complex)
+    site = 3 + 8 * (5 + 8 * (2 + 8 * 9))
+    for sc in range(12):
+        src[sc, site, sc // 3, sc % 3] = 1.0
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1)
+    x, b = q.Spinor(L1, nrhs=12), q.Spinor(L1, host=src, nrhs=12)
+    st = q.solve_batch(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.05, tol=1e-10)
+    xs = x.download()
+    for sc in range(12):
+        res = rel(orc.tm(xs[sc], Ub, L1, K1, M1, 1), src[sc])
+        assert st[sc].converged and res <= 1e-10, f"spin-colour {sc}: {res:.2e}"
