@@ -7,8 +7,14 @@ mu = 0.004, antiperiodic time boundary, Z2 source.  A "step" is one complete
 solve D x = b from a device-resident source to FP64-verified true relative
 residual <= 1e-10.  The timed steps use the fastest solver (mixed precision);
 the plain FP64 CG (full lattice, D^dag D, the reference algorithm) and the
-other precisions are timed once each for comparison, and every solution's
+other precisions are timed too (median of 3 each), and every solution's
 distance to an FP64 solution (solved to 1e-13) is reported.
+
+The roofline object describes the dominant kernel of the timed solves, the
+FP16 even-odd hop, timed with HIP events over 100 back-to-back calls of the
+inner operator (4 hops each) after 10 warm-ups, median of 5 trials, against
+its compulsory traffic; `traffic` is the measured HBM traffic per launch from
+the hardware counters (FETCH_SIZE, WRITE_SIZE, calibrated on a copy kernel).
 
 Prints ONE JSON line.
 """
@@ -39,6 +45,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the measured kernels a few times each (for hardware-counter runs): "
+                         "FP64 tm, FP16 even-odd normal operator, FP64 field copy (counter calibration)")
     return ap.parse_args()
 
 
@@ -85,9 +94,38 @@ def cpu_baseline(U, src, fp64_iters):
                       f" CG iterations (+10% for the BLAS)"}
 
 
+def loop_time(q, fn, launches_per_call, calls=100, trials=5, warm=10):
+    """Average launch time (us) of back-to-back launches: HIP events on the
+    library stream around `calls` calls, after `warm` warm-up calls; median of
+    `trials` trials."""
+    for _ in range(warm):
+        fn()
+    q.sync()
+    per = []
+    for _ in range(trials):
+        t = q.Timer()
+        t.start()
+        for _ in range(calls):
+            fn()
+        per.append(t.stop() * 1e3 / (calls * launches_per_call))
+    return float(np.median(per))
+
+
+def bytes_per_call(q, fn):
+    """Algorithmic bytes of one call as the library accounts them (compulsory
+    traffic: every operand read once, every result written once)."""
+    q.prof_reset()
+    q.prof_enable(True)
+    fn()
+    n, _, by = q.prof_read()
+    q.prof_enable(False)
+    q.prof_reset()
+    return n, by
+
+
 # tuned reliable-update trigger per inner precision (24^3x48 sweep, see profiles/)
 DELTA = {"fp32": 0.003, "fp16": 0.01, "bf16": 0.03}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hop_fp16_eo_24x24x24x48_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hop_fp16_eo_24x24x24x48_traffic.txt")
 
 
 def main():
@@ -106,6 +144,27 @@ def main():
     x = q.Spinor(L)
     V = int(np.prod(L))
 
+    # the measured kernels on their own fields: FP64 tm (the drop-in Dslash)
+    # and the FP16 even-odd normal operator (4 hops; the inner operator of the
+    # fastest solver)
+    y = q.Spinor(L)
+    uh = q.Gauge(L, q.HALF, q.RECON_12, -1)
+    uh.convert_from(u)
+    eh = q.Spinor(L, q.HALF, q.EVEN, host=src)
+    oh = q.Spinor(L, q.HALF, q.EVEN)
+    tm64 = lambda: y.tm(b, u, p)                                  # noqa: E731
+    normal16 = lambda: oh.tm_xxDagTm_xx(eh, uh, p, q.EE)          # noqa: E731
+    if args.kernels_only:
+        for _ in range(5):
+            tm64()
+        for _ in range(5):
+            normal16()
+        for _ in range(5):
+            y.copy(b)    # 2 x 192 B/site: calibrates FETCH_SIZE / WRITE_SIZE
+        q.sync()
+        print("kernels done", flush=True)
+        return
+
     # FP64 reference solution, solved tighter than the target
     q.solve(x, b, u, p, eo=True, inner_prec=q.DOUBLE, tol=1e-13, maxIter=20000)
     xref = x.download()
@@ -122,19 +181,18 @@ def main():
     for name, kw in variants.items():
         x.zero()
         q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)      # warm: inner gauge, temporaries
-        best_t = None
+        times = []
         for _ in range(3):
             x.zero()
             q.sync()
             t = time.perf_counter()
             st = q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
             q.sync()
-            dt = time.perf_counter() - t
-            best_t = dt if best_t is None else min(best_t, dt)
+            times.append(time.perf_counter() - t)
         xs = x.download()
-        results[name] = {"time_s": best_t, "iterations": st.iterations, "reliable_updates": st.reliable_updates,
-                         "true_residual": st.true_residual, "delta": kw.get("delta"),
-                         "err_vs_fp64": float(np.linalg.norm(xs - xref) / nref)}
+        results[name] = {"time_s": float(np.median(times)), "iterations": st.iterations,
+                         "reliable_updates": st.reliable_updates, "true_residual": st.true_residual,
+                         "delta": kw.get("delta"), "err_vs_fp64": float(np.linalg.norm(xs - xref) / nref)}
 
     best = min((k for k in results if k.startswith(("fp32", "fp16", "bf16"))), key=lambda k: results[k]["time_s"])
     kw = variants[best]
@@ -157,37 +215,37 @@ def main():
         raise SystemExit(f"timed solve missed the residual target: {st.true_residual}")
     med = float(np.median(times))
 
-    # ---- the same solves instrumented: HIP events around every hop launch ----
-    # (one iteration per host check, so no launch past the stopping iteration
-    # is counted; the dominant kernel is the inner-precision hop)
-    inner = kw["inner_prec"]
+    # ---- dominant kernel: the FP16 even-odd hop --------------------------------
+    # 100 back-to-back normal-operator calls (400 hops) after 10 warm-ups,
+    # median of 5 trials; bytes = compulsory traffic of the four hops
+    n16, by16 = bytes_per_call(q, normal16)
+    us16 = loop_time(q, normal16, n16)
+    achieved = by16 / n16 / (us16 * 1e-6) / 1e9
+    # the same hops inside the timed solves (HIP events around each launch,
+    # one iteration per host check so no launch past convergence is counted;
+    # includes the per-launch event overhead)
     q.set_cg_batch(1)
+    q.prof_reset()
     q.prof_enable(True)
     for _ in range(args.steps):
         x.zero()
         q.solve(x, b, u, p, tol=TOL, maxIter=20000, **kw)
-    n_in, ms_in, by_in = q.prof_read(inner)
-    n_all, ms_all, by_all = q.prof_read()
+    n_in, ms_in, by_in = q.prof_read(kw["inner_prec"])
     q.prof_enable(False)
+    q.prof_reset()
     q.set_cg_batch(16)
-    achieved = by_in / (ms_in * 1e-3) / 1e9
 
-    # ---- FP64 tm() kernel bandwidth (the drop-in Dslash) ----------------------
-    y = q.Spinor(L)
-    for _ in range(10):
-        y.tm(b, u, p)
-    q.prof_enable(True)
-    for _ in range(100):
-        y.tm(b, u, p)
-    n64, ms64, by64 = q.prof_read()
-    q.prof_enable(False)
-    tm_us = ms64 / n64 * 1e3
-    tm_gbs = by64 / (ms64 * 1e-3) / 1e9
+    # ---- FP64 tm(): the drop-in Dslash -------------------------------------------
+    n64, by64 = bytes_per_call(q, tm64)
+    us64 = loop_time(q, tm64, n64)
+    gbs64 = by64 / n64 / (us64 * 1e-6) / 1e9
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = float(line.split(":", 1)[1])
 
     cpu = None if args.no_cpu else cpu_baseline(U, src, results["fp64_cg_full"]["iterations"])
 
@@ -211,16 +269,18 @@ def main():
                    "lattice": list(L), "solver": best, "reliable_delta": kw.get("delta"), "sites": V},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hop_kernel, inner-precision even-odd hops of the timed solves",
-                     "launches": n_in, "avg_launch_us": ms_in / max(n_in, 1) * 1e3,
-                     "bytes_per_launch": by_in / max(n_in, 1),
-                     "all_hops": {"launches": n_all, "avg_launch_us": ms_all / max(n_all, 1) * 1e3,
-                                  "GB/s": by_all / (ms_all * 1e-3) / 1e9}},
+                     "kernel": "hop_kernel, FP16 even-odd hop (16-bit 12-real links) of the inner operator",
+                     "avg_launch_us": us16, "bytes_per_launch": by16 / n16,
+                     "timing": "HIP events, 100 back-to-back normal-operator calls (4 hops each) after 10 "
+                               "warm-ups, median of 5 trials",
+                     "in_solve": {"launches": n_in, "avg_launch_us": ms_in / max(n_in, 1) * 1e3,
+                                  "GB/s": by_in / max(ms_in, 1e-30) / 1e6,
+                                  "note": "per-launch events inside the timed solves (adds event overhead)"}},
         "cpu_baseline": cpu,
         "time_to_solution": results,
         "speedup_vs_fp64_cg": results["fp64_cg_full"]["time_s"] / med,
-        "dslash_fp64_tm": {"avg_launch_us": tm_us, "GB/s": tm_gbs, "frac": tm_gbs / HBM_PEAK_GBS,
-                           "bytes_per_site": 960},
+        "dslash_fp64_tm": {"avg_launch_us": us64, "GB/s": gbs64, "frac": gbs64 / HBM_PEAK_GBS,
+                           "bytes_per_site": by64 / n64 / V},
         "timed_total_s": total,
     }
     print(json.dumps(out))
