@@ -45,6 +45,11 @@ def lib():
             "orc_norm2": ([P, ctypes.c_long], D),
             "orc_cg": ([P, P, P, IP, D, D, I, D, I, ctypes.POINTER(ctypes.c_double)], I),
             "orc_apply_bc": ([P, IP, I], None),
+            "orc_ptc": ([P, P, P, IP, I, I, P], None),
+            "orc_ptc_adj": ([P, P, P, IP, I, I, P], None),
+            "orc_ptc_wgrad": ([P, P, P, P, IP, I, I], None),
+            "orc_net_apply": ([P, P, P, IP, I, P, P], None),
+            "orc_net_loss_grad": ([P, P, P, IP, I, P, P, P, P], D),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -151,3 +156,51 @@ def cg(b, U, L, kappa, mu, s, tol, maxIter, x0=None):
     r2 = ctypes.c_double()
     k = lib().orc_cg(_p(x), _p(b), _p(U), _L(L), kappa, mu, s, tol, maxIter, ctypes.byref(r2))
     return x, k, r2.value
+
+
+# ---- learned preconditioner: parallel-transport convolutions (qkx_oracle_ptc.c)
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def ptc(f, U, L, W):
+    """One PTC layer; f [V, cin, 3], W [9, cout, cin] -> [V, cout, 3]."""
+    f, U, W = _c(f), _c(U), _c(W)
+    cout, cin = W.shape[1], W.shape[2]
+    out = np.zeros((_vol(L), cout, 3), dtype=np.complex128)
+    lib().orc_ptc(_p(out), _p(f), _p(U), _L(L), cin, cout, _p(W))
+    return out
+
+
+def ptc_adj(g, U, L, W):
+    """Adjoint of ptc with weights W [9, cout, cin]; g [V, cout, 3] -> [V, cin, 3]."""
+    g, U, W = _c(g), _c(U), _c(W)
+    cout, cin = W.shape[1], W.shape[2]
+    out = np.zeros((_vol(L), cin, 3), dtype=np.complex128)
+    lib().orc_ptc_adj(_p(out), _p(g), _p(U), _L(L), cin, cout, _p(W))
+    return out
+
+
+def ptc_wgrad(g, f, U, L):
+    """sum_x g_a(x) . conj(T_p f_b (x)) -> [9, cout, cin]."""
+    g, f, U = _c(g), _c(f), _c(U)
+    cout, cin = g.shape[1], f.shape[1]
+    G = np.zeros((9, cout, cin), dtype=np.complex128)
+    lib().orc_ptc_wgrad(_p(G), _p(g), _p(f), _p(U), _L(L), cin, cout)
+    return G
+
+
+def net_apply(psi, U, L, W1, W2):
+    """M psi = L2(L1(psi)); psi [V, 4, 3]; W1 [9, ch, 4], W2 [9, 4, ch]."""
+    psi, U, W1, W2 = _c(psi), _c(U), _c(W1), _c(W2)
+    out = _new(L)
+    lib().orc_net_apply(_p(out), _p(psi), _p(U), _L(L), W1.shape[1], _p(W1), _p(W2))
+    return out
+
+
+def net_loss_grad(r, e, U, L, W1, W2):
+    """(|M r - e|^2, dL/dW1, dL/dW2) in the dL/dRe + i dL/dIm convention."""
+    r, e, U, W1, W2 = _c(r), _c(e), _c(U), _c(W1), _c(W2)
+    G1, G2 = np.zeros_like(W1), np.zeros_like(W2)
+    loss = lib().orc_net_loss_grad(_p(r), _p(e), _p(U), _L(L), W1.shape[1], _p(W1), _p(W2), _p(G1), _p(G2))
+    return loss, G1, G2

@@ -68,6 +68,21 @@ int orc_cg(double *x, const double *b, const double *U, const int L[4],
 /* Antiperiodic (or periodic) time boundary as a link sign on the last slice */
 void orc_apply_bc(double *U, const int L[4], int boundaryT);
 
+/* ---- learned preconditioner (qkx_oracle_ptc.c): gauge-equivariant
+ * parallel-transport convolutions.  Fields f[V][C][3] complex; W[9][cout][cin]
+ * complex, path 0 = identity, 1+2mu = U_mu(x) f(x+mu), 2+2mu = U_mu(x-mu)^dag
+ * f(x-mu).  Links carry the boundary sign. */
+void orc_ptc(double *out, const double *in, const double *U, const int L[4], int cin, int cout,
+             const double *W);
+void orc_ptc_adj(double *out, const double *in, const double *U, const int L[4], int cin, int cout,
+                 const double *W);
+void orc_ptc_wgrad(double *G, const double *g, const double *in, const double *U, const int L[4], int cin,
+                   int cout);
+void orc_net_apply(double *out, const double *in, const double *U, const int L[4], int ch, const double *W1,
+                   const double *W2);
+double orc_net_loss_grad(const double *r, const double *e, const double *U, const int L[4], int ch,
+                         const double *W1, const double *W2, double *G1, double *G2);
+
 #ifdef __cplusplus
 }
 #endif
