@@ -45,6 +45,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="single", choices=["single", "learned", "propagator", "production"],
+                    help="single: the 24^3x48 single-RHS solve (default, the headline line); "
+                         "learned: the 16^3x32 learned-preconditioner study (trains on the GPU, then compares "
+                         "solvers on unseen gauge fields); propagator: 12 point sources at 32^3x64 in one "
+                         "batch plus multi-mu batches at 16^3x32 and 24^3x48; production: the 12-source "
+                         "propagator at 48^3x96")
+    ap.add_argument("--chunk", type=int, default=12, help="right-hand sides per batch (propagator workloads)")
+    ap.add_argument("--inner", default="fp16", choices=["fp16", "bf16", "fp32"],
+                    help="inner precision of the batched propagator solves")
+    ap.add_argument("--epochs", type=int, default=6, help="training epochs of the learned workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the measured kernels a few times each (for hardware-counter runs): "
                          "FP64 tm, FP16 even-odd normal operator, FP64 field copy (counter calibration)")
@@ -128,6 +138,187 @@ DELTA = {"fp32": 0.003, "fp16": 0.01, "bf16": 0.03}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hop_fp16_eo_24x24x24x48_traffic.txt")
 
 
+def run_learned(q, args):
+    """16^3x32, kappa 0.1625, mu 0.004, eps 0.3: train the 4 -> 16 -> 4 PTC
+    network on 4096 (residual, error) pairs from plain-CG iterates on 8 seeded
+    gauge fields, then solve on 3 unseen gauge fields with flexible GCR + the
+    network (BF16 weights, FP32 accumulation) and compare with the plain FP64
+    CG (full lattice, the reference algorithm) and with GCR without it."""
+    from qkxtm_mpi_amd import learned
+    Ll, kap, mu, eps = (16, 16, 16, 32), 0.1625, 0.004, 0.3
+    t0 = time.perf_counter()
+    pairs = learned.make_pairs(Ll, kap, mu, eps, gauge_seeds=list(range(101, 109)), sources_per_gauge=16, every=2,
+                               max_pairs=4096)
+    q.sync()
+    t_pairs = time.perf_counter() - t0
+    net = q.Net(16, seed=1)
+    loss0 = learned.pair_loss(net, pairs, kap, mu)
+    t0 = time.perf_counter()
+    hist = learned.train(net, pairs, kap, mu, epochs=args.epochs, batch=16, lr=2e-3, seed=2,
+                         log=lambda m: print(m, file=sys.stderr, flush=True))
+    q.sync()
+    t_train = time.perf_counter() - t0
+    del pairs
+    rows = []
+    for gs in (201, 202, 203):
+        U = q.gen_gauge(Ll, eps, gs)
+        u64 = q.Gauge(Ll, boundaryT=-1, host=U)
+        u32 = q.Gauge(Ll, q.SINGLE, q.RECON_18, -1)
+        u32.convert_from(u64)
+        p = q.params(Ll, kap, mu, 1, -1)
+        b = q.Spinor(Ll, host=q.gen_z2(Ll, 7 + gs))
+        x = q.Spinor(Ll)
+        runs = {
+            "fp64_cg_full": lambda: q.solve(x, b, u64, p, eo=False, inner_prec=q.DOUBLE, tol=TOL, maxIter=20000),
+            "gcr": lambda: q.solve_gcr(x, b, u64, p, restart=8, tol=TOL, maxIter=20000),
+            "gcr_learned": lambda: q.solve_gcr(x, b, u64, p, net=net, u_net=u32, restart=8, tol=TOL,
+                                               maxIter=20000, stall=0.5),
+            "fp16_eo_rel": lambda: q.solve(x, b, u64, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=TOL,
+                                           maxIter=20000),
+        }
+        row = {"gauge_seed": gs}
+        for name, fn in runs.items():
+            x.zero()
+            fn()
+            times = []
+            for _ in range(3):
+                x.zero()
+                q.sync()
+                t = time.perf_counter()
+                st = fn()
+                q.sync()
+                times.append(time.perf_counter() - t)
+            row[name] = {"time_s": float(np.median(times)), "iterations": st.iterations,
+                         "true_residual": st.true_residual, "converged": bool(st.converged),
+                         "fallback_iterations": st.fallback_iterations if name == "gcr_learned" else 0}
+        rows.append(row)
+    med = lambda k, f: float(np.median([r[k][f] for r in rows]))   # noqa: E731
+    out = {
+        "metric": "time-to-1e-10 resid, learned-preconditioned flexible GCR vs plain FP64 CG, 16^3x32",
+        "value": med("gcr_learned", "time_s"), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "fp64 vectors, network bf16 weights / fp32 accumulate",
+        "data": "synthetic (seeded SU(3) gauge eps=0.3, Z2 sources)",
+        "config": {"workload": "16^3x32 learned preconditioner, kappa=0.1625 mu=0.004, 4096 pairs from plain-CG "
+                               "iterates on 8 gauge fields, tested on 3 unseen gauge fields",
+                   "network": "PTC 4->16->4 colour-vector channels (12->48->12 complex), paths 0 and +-mu"},
+        "pairs_seconds": t_pairs, "train_seconds": t_train, "epochs": args.epochs, "loss_untrained": loss0,
+        "loss_history": hist,
+        "speedup_vs_fp64_cg": med("fp64_cg_full", "time_s") / med("gcr_learned", "time_s"),
+        "iterations": {k: med(k, "iterations") for k in ("fp64_cg_full", "gcr", "gcr_learned", "fp16_eo_rel")},
+        "time_s": {k: med(k, "time_s") for k in ("fp64_cg_full", "gcr", "gcr_learned", "fp16_eo_rel")},
+        "per_gauge": rows,
+    }
+    print(json.dumps(out))
+
+
+def point_batch(q, L, x0, first, n):
+    """Spin-colour point sources first..first+n-1 at site x0, as one batch."""
+    V = int(np.prod(L))
+    iv = ((x0[3] * L[2] + x0[2]) * L[1] + x0[1]) * L[0] + x0[0]
+    b = q.Spinor(L, nrhs=n)
+    psi = np.zeros((V, 4, 3), np.complex128)
+    for r in range(n):
+        sc = first + r
+        psi[iv, sc // 3, sc % 3] = 1.0
+        b.upload_rhs(r, psi)
+        psi[iv, sc // 3, sc % 3] = 0.0
+    return b
+
+
+def timed(q, fn, reps):
+    fn()
+    ts = []
+    st = None
+    for _ in range(reps):
+        q.sync()
+        t = time.perf_counter()
+        st = fn()
+        q.sync()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), st
+
+
+def run_propagator(q, args, L, kap, mu, eps, mu_batches):
+    """Twelve spin-colour point sources solved as batches of `chunk`
+    right-hand sides (even-odd, low-precision inner, FP64 reliable updates),
+    against one FP64 CG solve and one single-RHS mixed solve scaled by 12."""
+    P = {"fp16": q.HALF, "bf16": q.BF16, "fp32": q.SINGLE}[args.inner]
+    delta = DELTA[args.inner]
+    reps = 1 if np.prod(L) > 4e6 else 3
+    U = q.gen_gauge(L, eps, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    del U
+    p = q.params(L, kap, mu, 1, -1)
+    x0 = q.gen_site(L, 5)
+    chunk = max(1, min(args.chunk, 12))
+    t_batch, iters, worst = 0.0, [], 0.0
+    for first in range(0, 12, chunk):
+        n = min(chunk, 12 - first)
+        b = point_batch(q, L, x0, first, n)
+        x = q.Spinor(L, nrhs=n)
+        t, sts = timed(q, lambda: q.solve_batch(x, b, u, p, eo=True, inner_prec=P, delta=delta, tol=TOL,
+                                                 maxIter=50000), reps)
+        t_batch += t
+        iters.append(sts[0].iterations)
+        worst = max(worst, max(s.true_residual for s in sts))
+        del b, x
+    b1 = point_batch(q, L, x0, 0, 1)
+    x1 = q.Spinor(L)
+    t_single, st1 = timed(q, lambda: q.solve(x1, b1, u, p, eo=True, inner_prec=P, delta=delta, tol=TOL,
+                                              maxIter=50000), reps)
+    t_fp64, st64 = timed(q, lambda: q.solve(x1, b1, u, p, eo=False, inner_prec=q.DOUBLE, tol=TOL,
+                                             maxIter=50000), 1)
+    # Dslash rates at this volume: FP64 tm and the inner-precision EO normal operator
+    y = q.Spinor(L)
+    n64, by64 = bytes_per_call(q, lambda: y.tm(b1, u, p))
+    us64 = loop_time(q, lambda: y.tm(b1, u, p), n64, calls=20, trials=3, warm=3)
+    ul = q.Gauge(L, P, q.RECON_12, -1)
+    ul.convert_from(u)
+    e = q.Spinor(L, P, q.EVEN)
+    e.copy(b1)
+    o = q.Spinor(L, P, q.EVEN)
+    nlo, bylo = bytes_per_call(q, lambda: o.tm_xxDagTm_xx(e, ul, p, q.EE))
+    uslo = loop_time(q, lambda: o.tm_xxDagTm_xx(e, ul, p, q.EE), nlo, calls=20, trials=3, warm=3)
+    mu_rows = []
+    for Lm in mu_batches:
+        Um = q.gen_gauge(Lm, 0.3, 3)
+        um = q.Gauge(Lm, boundaryT=-1, host=Um)
+        mus = np.array([0.004, 0.01, 0.02])
+        src = q.gen_z2(Lm, 9)
+        bm = q.Spinor(Lm, host=np.stack([src] * 3), nrhs=3)
+        xm = q.Spinor(Lm, nrhs=3)
+        pm = q.params(Lm, kap, mus[0], 1, -1)
+        tm_, stm = timed(q, lambda: q.solve_batch(xm, bm, um, pm, mu=mus, eo=True, inner_prec=P, delta=delta,
+                                                   tol=TOL, maxIter=50000), 3)
+        bs, xs_ = q.Spinor(Lm, host=src), q.Spinor(Lm)
+        tseq = 0.0
+        for m in mus:
+            pm1 = q.params(Lm, kap, m, 1, -1)
+            t1, _ = timed(q, lambda: q.solve(xs_, bs, um, pm1, eo=True, inner_prec=P, delta=delta, tol=TOL,
+                                               maxIter=50000), 3)
+            tseq += t1
+        mu_rows.append({"lattice": list(Lm), "mu": mus.tolist(), "batch_s": tm_, "sequential_s": tseq,
+                        "iterations": stm[0].iterations, "max_true_residual": max(s.true_residual for s in stm)})
+    out = {
+        "metric": f"time-to-1e-10 resid, 12-source propagator at {'x'.join(map(str, L))}, batched mixed-precision "
+                  f"CG vs plain FP64 CG",
+        "value": t_batch, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": f"{args.inner}-stored inner, fp64 outer",
+        "data": f"synthetic (seeded SU(3) gauge eps={eps}, point sources at seeded site {x0})",
+        "config": {"workload": f"{'x'.join(map(str, L))} 12 spin-colour point sources, kappa={kap} mu={mu}, "
+                               f"batches of {chunk}", "lattice": list(L), "chunk": chunk},
+        "batched_iterations": iters, "max_true_residual": worst,
+        "single_rhs_mixed_s": t_single, "single_rhs_mixed_x12_s": 12 * t_single,
+        "single_rhs_iterations": st1.iterations,
+        "fp64_cg_one_rhs_s": t_fp64, "fp64_cg_x12_s": 12 * t_fp64, "fp64_cg_iterations": st64.iterations,
+        "speedup_vs_fp64_cg": 12 * t_fp64 / t_batch, "speedup_vs_sequential_mixed": 12 * t_single / t_batch,
+        "dslash": {"fp64_tm_us": us64, "fp64_tm_GBs": by64 / n64 / (us64 * 1e-6) / 1e9,
+                   f"{args.inner}_eo_hop_us": uslo, f"{args.inner}_eo_hop_GBs": bylo / nlo / (uslo * 1e-6) / 1e9},
+        "multi_mu": mu_rows,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -135,6 +326,15 @@ def main():
         sys.exit(2)
     import qkxtm_mpi_amd as q
     q.init(0)
+    if args.workload == "learned":
+        run_learned(q, args)
+        return
+    if args.workload == "propagator":
+        run_propagator(q, args, (32, 32, 32, 64), 0.1632, 0.004, 0.3, [(16, 16, 16, 32), (24, 24, 24, 48)])
+        return
+    if args.workload == "production":
+        run_propagator(q, args, (48, 48, 48, 96), 0.1632, 0.004, 0.25, [])
+        return
 
     U = q.gen_gauge(L, EPS, 1)
     src = q.gen_z2(L, 2)

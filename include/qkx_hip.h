@@ -206,6 +206,7 @@ typedef struct {
     double iterated_r2;       /* final iterated |r|^2 (normal equations)       */
     double true_residual;     /* |b - D x| / |b| recomputed in FP64 (-1: n/a)  */
     double seconds;           /* device wall time of the solve                 */
+    int    fallback_iterations; /* qkx_solve_gcr: CG iterations after a stall  */
 } qkx_solver_stats;
 
 /* CG::CG + CG::operator() (include/solvers.h:12,21; lib/solvers.cpp:8-81):
@@ -215,6 +216,14 @@ typedef struct {
  * -1 (the reference computes none). */
 int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
            const qkx_params *p, qkx_solver_stats *st);
+
+/* qkx_cg that also copies the iterate x into store[i] after every `every`
+ * iterations (i < nstore; *nstored receives the count).  No reference
+ * counterpart: it produces the (residual, error) training pairs of the
+ * learned preconditioner from plain-CG iterates. */
+int qkx_cg_record(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+                  const qkx_params *p, int every, qkx_spinor **store,
+                  int nstore, int *nstored, qkx_solver_stats *st);
 
 /* Solve D x = b (UKQCD basis) to true relative residual |b - D x|/|b| <= tol.
  * No reference counterpart: the reference has no even-odd, mixed-precision
@@ -248,6 +257,55 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
                     const qkx_gauge *U_inner, const qkx_params *p,
                     const double *mu, const qkx_solve_opts *o,
                     qkx_solver_stats *st);
+
+/* ---- learned preconditioner ----------------------------------------------- */
+/* No reference counterpart.  M = L2 L1, two gauge-equivariant
+ * parallel-transport convolutions (Lehner and Wettig, Phys. Rev. D 108,
+ * 034503 (2023)) with paths of length 0 and 1:
+ *   out_a(x) = sum_p sum_b W[p][a][b] (T_p in_b)(x),  T_0 = 1,
+ *   T_{1+2mu} f(x) = U_mu(x) f(x+mu),  T_{2+2mu} f(x) = U_mu(x-mu)^dag f(x-mu),
+ * on 4 -> ch -> 4 colour-vector channels (the 4 spin components of a spinor).
+ * Weights: W1[9][ch][4] then W2[9][4][ch], complex, as interleaved FP32 on
+ * the host (2 * qkx_net_nweights floats).  The network's gauge field is FP32
+ * RECON_18 (its boundaryT as for the operators). */
+typedef struct qkx_net_s qkx_net;
+int qkx_net_create(qkx_net **n, int ch);                  /* ch = 8 or 16 */
+int qkx_net_destroy(qkx_net *n);
+int qkx_net_nweights(const qkx_net *n, int *ncomplex);
+int qkx_net_set_weights(qkx_net *n, const float *w);      /* resets the optimiser */
+int qkx_net_get_weights(const qkx_net *n, float *w);
+/* forward passes read BF16 weights (1, default; FP32 accumulation) or FP32 (0) */
+int qkx_net_set_bf16(qkx_net *n, int on);
+/* out = M in; single full-lattice FP64 or FP32 spinors */
+int qkx_net_apply(qkx_net *n, qkx_spinor *out, const qkx_spinor *in,
+                  const qkx_gauge *U);
+/* Adds the gradient of |M r - e|^2 with respect to the weights
+ * (dL/dRe W + i dL/dIm W) and the loss to the network's accumulators. */
+int qkx_net_accumulate_grad(qkx_net *n, const qkx_spinor *r,
+                            const qkx_spinor *e, const qkx_gauge *U);
+/* accumulated gradient (2 * nweights floats, may be NULL) and loss (may be NULL) */
+int qkx_net_read_grad(qkx_net *n, float *g, double *loss);
+int qkx_net_zero_grad(qkx_net *n);
+/* Adam on the FP32 master weights with the accumulated gradient / count;
+ * clears the gradient (the loss keeps accumulating until qkx_net_zero_grad)
+ * and refreshes the BF16 copy. */
+int qkx_net_adam(qkx_net *n, double lr, double beta1, double beta2, double eps,
+                 int count);
+
+typedef struct {
+    int    restart;   /* GCR cycle length (1..16)                               */
+    int    maxIter;   /* iteration cap (GCR + fallback CG)                       */
+    double tol;       /* target true relative residual                          */
+    double stall;     /* a cycle reducing |r| by less than this factor falls back
+                         to the even-odd mixed-precision CG (0: never)          */
+} qkx_gcr_opts;
+
+/* Flexible GCR on D x = b (full lattice, UKQCD, FP64 vectors) with the
+ * network as right preconditioner (net may be NULL: plain GCR).  U_net is
+ * the network's FP32 gauge field.  No reference counterpart. */
+int qkx_solve_gcr(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+                  const qkx_gauge *U_net, qkx_net *net, const qkx_params *p,
+                  const qkx_gcr_opts *o, qkx_solver_stats *st);
 
 /* Relative true residual |b - D x| / |b| in FP64, full lattice, UKQCD. */
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,

@@ -48,10 +48,16 @@ class Params(ctypes.Structure):
 
 class SolverStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("reliable_updates", ctypes.c_int), ("converged", ctypes.c_int),
-                ("iterated_r2", ctypes.c_double), ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("iterated_r2", ctypes.c_double), ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("fallback_iterations", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GcrOpts(ctypes.Structure):
+    _fields_ = [("restart", ctypes.c_int), ("maxIter", ctypes.c_int), ("tol", ctypes.c_double),
+                ("stall", ctypes.c_double)]
 
 
 class SolveOpts(ctypes.Structure):
@@ -88,6 +94,15 @@ SIGNATURES = {
     "qkx_solve_batch": [_P, _P, _P, _P, ctypes.POINTER(Params), _DP, ctypes.POINTER(SolveOpts),
                         ctypes.POINTER(SolverStats)],
     "qkx_true_residual": [_DP, _P, _P, _P, ctypes.POINTER(Params)],
+    "qkx_cg_record": [_P, _P, _P, ctypes.POINTER(Params), _I, ctypes.POINTER(_P), _I, _IP,
+                      ctypes.POINTER(SolverStats)],
+    "qkx_net_create": [ctypes.POINTER(_P), _I], "qkx_net_destroy": [_P], "qkx_net_nweights": [_P, _IP],
+    "qkx_net_set_weights": [_P, _P], "qkx_net_get_weights": [_P, _P], "qkx_net_set_bf16": [_P, _I],
+    "qkx_net_apply": [_P, _P, _P, _P], "qkx_net_accumulate_grad": [_P, _P, _P, _P],
+    "qkx_net_read_grad": [_P, _P, _DP], "qkx_net_zero_grad": [_P],
+    "qkx_net_adam": [_P, _D, _D, _D, _D, _I],
+    "qkx_solve_gcr": [_P, _P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(GcrOpts),
+                      ctypes.POINTER(SolverStats)],
     "qkx_gen_gauge": [_P, _IP, _D, ctypes.c_ulonglong], "qkx_gen_z2": [_P, _IP, ctypes.c_ulonglong],
     "qkx_gen_gaussian": [_P, _IP, ctypes.c_ulonglong], "qkx_gen_point": [_P, _IP, _IP, _I, _I],
     "qkx_gen_site": [_IP, _IP, ctypes.c_ulonglong],
@@ -379,6 +394,103 @@ def solve_batch(x: Spinor, b: Spinor, u: Gauge, p: Params, mu=None, eo=True, inn
     if not (rc == NOT_CONVERGED and allow_not_converged):
         _check(rc, "qkx_solve_batch")
     return list(st)
+
+
+def cg_record(x: Spinor, b: Spinor, u: Gauge, p: Params, every: int, store):
+    """Reference CG (as CG.__call__) that copies the iterate into store[i]
+    after every `every` iterations; returns (SolverStats, number stored)."""
+    st = SolverStats()
+    arr = (ctypes.c_void_p * max(1, len(store)))(*[f.h.value for f in store])
+    n = ctypes.c_int()
+    rc = lib().qkx_cg_record(x.h, b.h, u.h, ctypes.byref(p), int(every), arr, len(store), ctypes.byref(n),
+                             ctypes.byref(st))
+    if rc != NOT_CONVERGED:
+        _check(rc, "qkx_cg_record")
+    return st, n.value
+
+
+class Net:
+    """Learned preconditioner M = L2 L1: two gauge-equivariant
+    parallel-transport convolutions, 4 -> ch -> 4 colour-vector channels.
+    Weights are complex arrays W1 [9, ch, 4] and W2 [9, 4, ch]; path 0 is the
+    identity, 1+2mu the forward and 2+2mu the backward transport along mu."""
+
+    def __init__(self, ch=16, W1=None, W2=None, seed=0, noise=1e-2):
+        self.ch = ch
+        h = ctypes.c_void_p()
+        _check(lib().qkx_net_create(ctypes.byref(h), ch), "qkx_net_create")
+        self.h = h
+        if W1 is None:
+            W1, W2 = Net.identity_weights(ch, seed, noise)
+        self.set_weights(W1, W2)
+
+    @staticmethod
+    def identity_weights(ch, seed=0, noise=1e-2):
+        """M = 1 plus small complex noise (hidden channels 0..3 carry the spins)."""
+        r = np.random.default_rng(seed)
+        W1 = noise * (r.standard_normal((9, ch, 4)) + 1j * r.standard_normal((9, ch, 4)))
+        W2 = noise * (r.standard_normal((9, 4, ch)) + 1j * r.standard_normal((9, 4, ch)))
+        W1[0, :4, :4] += np.eye(4)
+        W2[0, :4, :4] += np.eye(4)
+        return W1, W2
+
+    def set_weights(self, W1, W2):
+        w = np.concatenate([np.asarray(W1, np.complex64).ravel(), np.asarray(W2, np.complex64).ravel()])
+        assert w.size == 2 * 9 * 4 * self.ch
+        w = np.ascontiguousarray(w.view(np.float32))
+        _check(lib().qkx_net_set_weights(self.h, _ptr(w)), "qkx_net_set_weights")
+
+    def get_weights(self):
+        n = 9 * 4 * self.ch
+        w = np.zeros(4 * n, np.float32)
+        _check(lib().qkx_net_get_weights(self.h, _ptr(w)), "qkx_net_get_weights")
+        c = w.view(np.complex64).astype(np.complex128)
+        return c[:n].reshape(9, self.ch, 4), c[n:].reshape(9, 4, self.ch)
+
+    def set_bf16(self, on=True):
+        _check(lib().qkx_net_set_bf16(self.h, int(on)), "qkx_net_set_bf16")
+
+    def apply(self, out: Spinor, inp: Spinor, u_net: Gauge):
+        _check(lib().qkx_net_apply(self.h, out.h, inp.h, u_net.h), "qkx_net_apply")
+
+    def accumulate_grad(self, r: Spinor, e: Spinor, u_net: Gauge):
+        _check(lib().qkx_net_accumulate_grad(self.h, r.h, e.h, u_net.h), "qkx_net_accumulate_grad")
+
+    def read_grad(self):
+        n = 9 * 4 * self.ch
+        g = np.zeros(4 * n, np.float32)
+        loss = ctypes.c_double()
+        _check(lib().qkx_net_read_grad(self.h, _ptr(g), ctypes.byref(loss)), "qkx_net_read_grad")
+        c = g.view(np.complex64).astype(np.complex128)
+        return c[:n].reshape(9, self.ch, 4), c[n:].reshape(9, 4, self.ch), loss.value
+
+    def read_loss(self):
+        loss = ctypes.c_double()
+        _check(lib().qkx_net_read_grad(self.h, None, ctypes.byref(loss)), "qkx_net_read_grad")
+        return loss.value
+
+    def zero_grad(self):
+        _check(lib().qkx_net_zero_grad(self.h), "qkx_net_zero_grad")
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, count=1):
+        _check(lib().qkx_net_adam(self.h, lr, beta1, beta2, eps, int(count)), "qkx_net_adam")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qkx_net_destroy(self.h)
+            self.h = None
+
+
+def solve_gcr(x: Spinor, b: Spinor, u: Gauge, p: Params, net: "Net | None" = None, u_net: Gauge | None = None,
+              restart=8, tol=1e-10, maxIter=10000, stall=0.0, allow_not_converged=False):
+    """Flexible GCR on D x = b with the network as right preconditioner."""
+    o = GcrOpts(int(restart), int(maxIter), tol, stall)
+    st = SolverStats()
+    rc = lib().qkx_solve_gcr(x.h, b.h, u.h, u_net.h if u_net else None, net.h if net else None, ctypes.byref(p),
+                             ctypes.byref(o), ctypes.byref(st))
+    if not (rc == NOT_CONVERGED and allow_not_converged):
+        _check(rc, "qkx_solve_gcr")
+    return st
 
 
 def true_residual(x: Spinor, b: Spinor, u: Gauge, p: Params):

@@ -264,8 +264,19 @@ __device__ __forceinline__ int logical_block(int nblk8)
     return b;
 }
 
+// Minimum waves per SIMD requested from the register allocator for the 16-bit
+// hops (1: no request).  At 24^3x48 a half-lattice launch is 1296 workgroups;
+// residency of 6 waves per SIMD holds all of them at once.
+#ifndef QKX_HOP16_WAVES
+#define QKX_HOP16_WAVES 1
+#endif
+template <class S> struct HopWaves { static constexpr int v = 1; };
+template <> struct HopWaves<SpinorH> { static constexpr int v = QKX_HOP16_WAVES; };
+template <> struct HopWaves<SpinorB> { static constexpr int v = QKX_HOP16_WAVES; };
+
 template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH>
-__global__ __launch_bounds__(256) void hop_kernel(HopArgs<SIn, SOut, SAux, G> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SIn>::v, 8))) void hop_kernel(
+    HopArgs<SIn, SOut, SAux, G> a)
 {
     using T = typename SOut::Real;
     if (a.skip && *a.skip) return;   // uniform over the grid

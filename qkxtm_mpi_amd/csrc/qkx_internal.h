@@ -100,6 +100,28 @@ template <class E> __device__ __forceinline__ E ld(const E *base, uint32_t idx)
 {
     return *reinterpret_cast<const E *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(E));
 }
+// The same load marked non-temporal (streamed past the caches): for operands
+// read exactly once per launch, such as the links of a half-lattice hop, so
+// that they do not evict the neighbour spinors that are read up to 8 times.
+#ifndef QKX_NT_LINKS
+#define QKX_NT_LINKS 0
+#endif
+template <int N> struct NtVec;
+template <> struct NtVec<8> { typedef __attribute__((ext_vector_type(2))) unsigned int type; };
+template <> struct NtVec<16> { typedef __attribute__((ext_vector_type(4))) unsigned int type; };
+template <class E> __device__ __forceinline__ E ld_link(const E *base, uint32_t idx)
+{
+    if constexpr (QKX_NT_LINKS && (sizeof(E) == 8 || sizeof(E) == 16)) {
+        using V = typename NtVec<sizeof(E)>::type;
+        const V v = __builtin_nontemporal_load(
+            reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(E)));
+        E e;
+        __builtin_memcpy(&e, &v, sizeof(E));
+        return e;
+    } else {
+        return ld(base, idx);
+    }
+}
 template <class E> __device__ __forceinline__ void st(E *base, uint32_t idx, E v)
 {
     *reinterpret_cast<E *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(E)) = v;
@@ -328,7 +350,7 @@ template <class E, int NC, class R> struct GaugeVec {
     {
         Raw r;
 #pragma unroll
-        for (int k = 0; k < NC; k++) r.c[k] = ld(p + (size_t)(mu * NC + k) * V, (uint32_t)s);
+        for (int k = 0; k < NC; k++) r.c[k] = ld_link(p + (size_t)(mu * NC + k) * V, (uint32_t)s);
         return r;
     }
 };
@@ -388,8 +410,8 @@ template <bool BF> struct Gauge16 {
     __device__ __forceinline__ Raw fetch(int mu, int s) const
     {
         Raw r;
-        r.a = ld(p4 + (size_t)mu * V, (uint32_t)s);
-        r.b = ld(p2 + (size_t)mu * V, (uint32_t)s);
+        r.a = ld_link(p4 + (size_t)mu * V, (uint32_t)s);
+        r.b = ld_link(p2 + (size_t)mu * V, (uint32_t)s);
         return r;
     }
     __device__ __forceinline__ void unpack(const Raw &r, Cx<float> U[9]) const

@@ -370,8 +370,10 @@ int cg_batch_size = 16;
 extern "C" {
 
 // Reference CG (lib/solvers.cpp:16-81), FP64, full lattice.  p.Ap is
-// evaluated as |D p|^2, which equals <p, D^dag D p>.
-int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p, qkx_solver_stats *st)
+// evaluated as |D p|^2, which equals <p, D^dag D p>.  With nstore > 0 the
+// iterate x is copied into store[i] after every `every` iterations.
+static int cg_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p,
+                   qkx_solver_stats *st, int every, qkx_spinor **store, int nstore, int *nstored)
 {
     QKX_CHECK_ARG(x && b && U && p, "qkx_cg: null argument");
     QKX_CHECK_ARG(x->prec == QKX_PREC_DOUBLE && b->prec == QKX_PREC_DOUBLE && U->prec == QKX_PREC_DOUBLE,
@@ -398,15 +400,21 @@ int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_par
     int status = QKX_SUCCESS;
     const int maxIter = p->maxIter;
     if (h.r2 > E.stop2) {
+        int saved = 0;
         while (true) {
-            int batch = std::min(qkx::cg_batch_size, maxIter - h.k);
+            int batch = std::min(nstore > 0 ? every : qkx::cg_batch_size, maxIter - h.k);
             if (batch <= 0) break;
             for (int i = 0; i < batch; i++)
                 if ((rc = enqueue_iteration(E, A, x, r.f, pp.f, Ap.f))) return rc;
             if ((rc = read_slot(E, h))) return rc;
             if (h.flag == 3) { status = QKX_ERR_BREAKDOWN; break; }
             if (h.flag == 2) break;
+            if (saved < nstore) {
+                if ((rc = qkx_spinor_copy(store[saved], x))) return rc;
+                saved++;
+            }
         }
+        if (nstored) *nstored = saved;
         if (h.flag != 2 && status == QKX_SUCCESS) status = QKX_NOT_CONVERGED;
     }
     if (st) {
@@ -420,6 +428,24 @@ int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_par
     if (status == QKX_NOT_CONVERGED) qkx::set_error("Warning solver reached number of max iterations");
     if (status == QKX_ERR_BREAKDOWN) qkx::set_error("qkx_cg: breakdown (p.Ap <= 0 or non-finite residual)");
     return status;
+}
+
+int qkx_cg(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p, qkx_solver_stats *st)
+{
+    return cg_impl(x, b, U, p, st, 0, nullptr, 0, nullptr);
+}
+
+int qkx_cg_record(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p, int every,
+                  qkx_spinor **store, int nstore, int *nstored, qkx_solver_stats *st)
+{
+    QKX_CHECK_ARG(every > 0 && nstore >= 0 && (nstore == 0 || store), "qkx_cg_record: bad arguments");
+    for (int i = 0; i < nstore; i++)
+        QKX_CHECK_ARG(store[i] && store[i]->subset == QKX_SUBSET_FULL && store[i]->nrhs == 1 &&
+                          store[i]->L[0] == b->L[0] && store[i]->L[1] == b->L[1] && store[i]->L[2] == b->L[2] &&
+                          store[i]->L[3] == b->L[3],
+                      "qkx_cg_record: store fields must be single full fields on the lattice of b");
+    if (nstored) *nstored = 0;
+    return cg_impl(x, b, U, p, st, every, store, nstore, nstored);
 }
 
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p)

@@ -468,17 +468,18 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         Ah.a[r] = Al.a[r] = 2.0 * p->kappa * m * p->twistSign;
         Ah.nrm[r] = Al.nrm[r] = 1.0 + 4.0 * p->kappa * p->kappa * m * m;
     }
-    auto T = [&](int prec, int subset) { return qkx::pool_get_batch(L, prec, subset, nr); };
-    qkx_spinor_s *hf = T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL), *ho = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
-                 *he = T(QKX_PREC_DOUBLE, QKX_SUBSET_EVEN), *lf = T(lp, QKX_SUBSET_FULL),
-                 *lo = T(lp, QKX_SUBSET_ODD), *le = T(lp, QKX_SUBSET_EVEN);
-    const int xp = lp == QKX_PREC_DOUBLE ? QKX_PREC_DOUBLE : QKX_PREC_SINGLE;
-    const int rp = lp == QKX_PREC_BF16 ? QKX_PREC_SINGLE : lp;
-    qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub), *xs = T(QKX_PREC_DOUBLE, sub),
-                 *s = T(QKX_PREC_DOUBLE, sub), *rh = T(QKX_PREC_DOUBLE, sub), *bo = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
-                 *to = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD), *r = T(rp, sub), *pp = T(lp, sub), *Ap = T(lp, sub),
-                 *xl = T(xp, sub), *tf = T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL);
-    std::vector<qkx_spinor_s *> temps = {hf, ho, he, lf, lo, le, bh, bn, xs, s, rh, bo, to, r, pp, Ap, xl, tf};
+    // temporaries; those a variant does not use are not allocated (a 12-rhs
+    // FP64 full field at 48^3x96 is 24.5 GB)
+    const bool eo = o->eo != 0;
+    std::vector<qkx_spinor_s *> temps;
+    bool nomem = false;
+    auto T = [&](int prec, int subset, bool need = true) -> qkx_spinor_s * {
+        if (!need) return nullptr;
+        qkx_spinor_s *f = qkx::pool_get_batch(L, prec, subset, nr);
+        if (!f) nomem = true;
+        temps.push_back(f);
+        return f;
+    };
     struct Release {
         std::vector<qkx_spinor_s *> &t;
         ~Release()
@@ -487,8 +488,23 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
                 if (f) qkx::pool_put(f);
         }
     } release{temps};
-    for (auto f : temps)
-        if (!f) return QKX_ERR_NOMEM;
+    const int xp = lp == QKX_PREC_DOUBLE ? QKX_PREC_DOUBLE : QKX_PREC_SINGLE;
+    const int rp = lp == QKX_PREC_BF16 ? QKX_PREC_SINGLE : lp;
+    qkx_spinor_s *hf = T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL, !eo), *ho = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo),
+                 *he = T(QKX_PREC_DOUBLE, QKX_SUBSET_EVEN, eo), *lf = T(lp, QKX_SUBSET_FULL, !eo && lp != QKX_PREC_DOUBLE),
+                 *lo = T(lp, QKX_SUBSET_ODD, eo && lp != QKX_PREC_DOUBLE),
+                 *le = T(lp, QKX_SUBSET_EVEN, eo && lp != QKX_PREC_DOUBLE);
+    if (lp == QKX_PREC_DOUBLE) {
+        lf = hf;
+        lo = ho;
+        le = he;
+    }
+    qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub), *xs = T(QKX_PREC_DOUBLE, sub),
+                 *s = T(QKX_PREC_DOUBLE, sub), *rh = T(QKX_PREC_DOUBLE, sub, mixed),
+                 *bo = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo), *to = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo),
+                 *r = T(rp, sub), *pp = T(lp, sub), *Ap = T(lp, sub), *xl = T(xp, sub, mixed),
+                 *tf = eo ? T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL) : hf;
+    if (nomem) return QKX_ERR_NOMEM;
     Ah.tmp_full = hf; Ah.tmp_odd = ho; Ah.tmp_even = he;
     Al.tmp_full = lf; Al.tmp_odd = lo; Al.tmp_even = le;
 
@@ -512,7 +528,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         if ((rc = qkx_spinor_copy(bh, b))) return rc;
     }
     if ((rc = Ah.apply_dag(bn, bh))) return rc;
-    if ((rc = qkx_spinor_zero(xs)) || (rc = qkx_spinor_zero(xl))) return rc;
+    if ((rc = qkx_spinor_zero(xs)) || (xl && (rc = qkx_spinor_zero(xl)))) return rc;
     if ((rc = qkx_spinor_copy(r, bn))) return rc;
     if ((rc = norms(bn, bn2))) return rc;
 
