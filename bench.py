@@ -170,9 +170,9 @@ def run_learned(q, args):
         x = q.Spinor(Ll)
         runs = {
             "fp64_cg_full": lambda: q.solve(x, b, u64, p, eo=False, inner_prec=q.DOUBLE, tol=TOL, maxIter=20000),
-            "gcr": lambda: q.solve_gcr(x, b, u64, p, restart=8, tol=TOL, maxIter=20000),
+            "gcr": lambda: q.solve_gcr(x, b, u64, p, restart=8, tol=TOL, maxIter=20000, stall=0.9),
             "gcr_learned": lambda: q.solve_gcr(x, b, u64, p, net=net, u_net=u32, restart=8, tol=TOL,
-                                               maxIter=20000, stall=0.5),
+                                               maxIter=20000, stall=0.9),
             "fp16_eo_rel": lambda: q.solve(x, b, u64, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=TOL,
                                            maxIter=20000),
         }
@@ -190,7 +190,7 @@ def run_learned(q, args):
                 times.append(time.perf_counter() - t)
             row[name] = {"time_s": float(np.median(times)), "iterations": st.iterations,
                          "true_residual": st.true_residual, "converged": bool(st.converged),
-                         "fallback_iterations": st.fallback_iterations if name == "gcr_learned" else 0}
+                         "fallback_iterations": st.fallback_iterations if name.startswith("gcr") else 0}
         rows.append(row)
     med = lambda k, f: float(np.median([r[k][f] for r in rows]))   # noqa: E731
     out = {

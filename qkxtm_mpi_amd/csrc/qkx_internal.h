@@ -265,7 +265,11 @@ struct SpinorH {
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        // a site whose largest component is below the smallest normal float
+        // (far from a point source) is stored as zero: 1/m would overflow
+        const bool keep = m >= 1.17549435e-38f;
+        if (!keep) m = 0.f;
+        const float inv = keep ? 1.0f / m : 0.f;
         st(nrm, (uint32_t)s, m);
 #pragma unroll
         for (int j = 0; j < 3; j++) {

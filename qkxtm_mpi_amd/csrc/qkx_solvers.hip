@@ -45,12 +45,17 @@ CGSlot *g_slots = nullptr;       // device, two slots
 CGSlot *g_slots_host = nullptr;  // pinned mirror
 double *g_part = nullptr;        // partials: [0, cap) p.Ap, [cap, 2 cap) |r|^2
 int g_part_cap = 0;
+double *g_red = nullptr;         // [0] p.Ap, [1] |r|^2 pre-summed when the partials are many
+// Above this many partials a single-workgroup kernel sums them first, so that
+// each consuming workgroup reads one value instead of all of them.
+constexpr int kDirectPartials = 2048;
 
 int ensure_state(int nblocks)
 {
     if (!g_slots) {
         QKX_HIP(hipMalloc(&g_slots, 2 * sizeof(CGSlot)));
         QKX_HIP(hipHostMalloc((void **)&g_slots_host, 2 * sizeof(CGSlot), hipHostMallocDefault));
+        QKX_HIP(hipMalloc(&g_red, 2 * sizeof(double)));
     }
     if (nblocks > g_part_cap) {
         if (g_part) (void)hipFree(g_part);
@@ -76,6 +81,31 @@ __device__ double block_sum_array(const double *v, int n)
     double r = lds[0];
     __syncthreads();
     return r;
+}
+
+// out[0] = sum(part[0..n)) in a fixed order, by one workgroup
+__global__ __launch_bounds__(1024) void sum_partials_kernel(const double *part, int n, double *out)
+{
+    __shared__ double lds[1024];
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = lds[0];
+}
+
+// partials (part, n) as the consumers should read them: pre-summed into
+// dst when there are many
+void reduce_if_many(const double *&part, int &n, double *dst)
+{
+    if (n <= kDirectPartials) return;
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, n, dst);
+    part = dst;
+    n = 1;
 }
 
 __device__ void block_partial(double v, double *dst)
@@ -264,11 +294,16 @@ int enqueue_blas(Engine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, q
     SX X = make_spinor<SX>(x);
     SR R = make_spinor<SR>(r);
     S P = make_spinor<S>(p), AP = make_spinor<S>(Ap);
-    double *pap_part = g_part, *r2_part = g_part + g_part_cap;
+    const double *pap_part = g_part;
+    double *r2_part = g_part + g_part_cap;
+    reduce_if_many(pap_part, npap, g_red);
     hipLaunchKernelGGL((cg_update_kernel<SX, SR, S>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, X, R, P, AP, n,
                        E.cur(), pap_part, npap, r2_part);
+    const double *r2_src = r2_part;
+    int nr2 = nblk(n);
+    reduce_if_many(r2_src, nr2, g_red + 1);
     hipLaunchKernelGGL((cg_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P, R, n, E.cur(),
-                       E.next(), r2_part, nblk(n), pap_part, npap, E.stop2, E.delta2, 0);
+                       E.next(), r2_src, nr2, pap_part, npap, E.stop2, E.delta2, 0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -314,11 +349,14 @@ int restart(Engine &E, qkx_spinor_s *p, qkx_spinor_s *r, int mode)
     if (mode == 0) {
         if ((rc = qkx_spinor_copy(p, r))) return rc;
     }
+    const double *r2_src = g_part + g_part_cap;
+    int nr2 = nblk(n);
+    reduce_if_many(r2_src, nr2, g_red + 1);
     rc = with_spinor(p, [&](auto P) {
         return with_spinor(r, [&](auto R) {
             hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0,
-                               qkx::g_stream, P, R, mode == 0 ? 0 : n, E.cur(), E.next(), g_part + g_part_cap,
-                               nblk(n), g_part, 0, 0.0, 0.0, 1);
+                               qkx::g_stream, P, R, mode == 0 ? 0 : n, E.cur(), E.next(), r2_src, nr2, g_part, 0,
+                               0.0, 0.0, 1);
             QKX_HIP(hipGetLastError());
             return QKX_SUCCESS;
         });

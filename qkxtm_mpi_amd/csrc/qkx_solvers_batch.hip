@@ -38,6 +38,8 @@ BSlot *g_bslots = nullptr, *g_bslots_host = nullptr;
 double *g_bpart = nullptr;
 int g_bpart_cap = 0;
 double *g_bres = nullptr, *g_bres_host = nullptr;
+double *g_bred = nullptr;   // [0, NR) p.Ap, [NR, 2 NR) |r|^2, pre-summed when the partials are many
+constexpr int kDirectPartials = 2048;   // workgroups' partials read directly by each consumer
 
 int ensure_bstate(int nblocks)
 {
@@ -46,6 +48,7 @@ int ensure_bstate(int nblocks)
         QKX_HIP(hipHostMalloc((void **)&g_bslots_host, sizeof(BSlot), hipHostMallocDefault));
         QKX_HIP(hipMalloc(&g_bres, NR * sizeof(double)));
         QKX_HIP(hipHostMalloc((void **)&g_bres_host, NR * sizeof(double), hipHostMallocDefault));
+        QKX_HIP(hipMalloc(&g_bred, 2 * NR * sizeof(double)));
     }
     size_t need = (size_t)nblocks * NR;
     if ((int)need > g_bpart_cap) {
@@ -218,6 +221,24 @@ __global__ __launch_bounds__(kT) void normb_finish_kernel(const double *part, in
     if ((int)threadIdx.x < nr) res[threadIdx.x] = s[threadIdx.x];
 }
 
+// out[r] = sum_b part[b*nr + r], fixed order, one workgroup of 1024
+__global__ __launch_bounds__(1024) void sum_rhs_kernel(const double *part, int nblocks, int nr, double *out)
+{
+    __shared__ double lds[1024];
+    const int groups = blockDim.x / nr;
+    const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+    double v = 0;
+    if (g < groups)
+        for (int b = g; b < nblocks; b += groups) v += part[(size_t)b * nr + r];
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+        double t = 0;
+        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
 template <class SL>
 __global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
 {
@@ -233,6 +254,16 @@ __global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
 }
 
 int nblk(int n) { return (n + kT - 1) / kT; }
+
+// per-rhs partials (part, nblocks) as consumers should read them: pre-summed
+// into dst when there are many
+void reduce_if_many(const double *&part, int &nblocks, int nr, double *dst)
+{
+    if (nblocks <= kDirectPartials) return;
+    hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, nblocks, nr, dst);
+    part = dst;
+    nblocks = 1;
+}
 
 template <class F> int with_spinor(const qkx_spinor_s *f, F fn)
 {
@@ -352,14 +383,19 @@ template <class SX, class SR, class S>
 int enqueue_bblas(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap, int npap)
 {
     const int n = entries(p), nr = p->nrhs;
-    double *pap_part = g_bpart, *r2_part = g_bpart + g_bpart_cap;
+    const double *pap_part = g_bpart;
+    double *r2_part = g_bpart + g_bpart_cap;
     SX X = make_spinor<SX>(x);
     SR R = make_spinor<SR>(r);
     S P = make_spinor<S>(p), AP = make_spinor<S>(Ap);
+    reduce_if_many(pap_part, npap, nr, g_bred);
     hipLaunchKernelGGL((cgb_update_kernel<SX, SR, S>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, X, R, P, AP, n, nr,
                        E.cur(), pap_part, npap, r2_part);
+    const double *r2_src = r2_part;
+    int nr2 = nblk(n);
+    reduce_if_many(r2_src, nr2, nr, g_bred + NR);
     hipLaunchKernelGGL((cgb_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, P, R, n, nr, E.cur(),
-                       E.next(), r2_part, nblk(n), pap_part, npap, E.delta2, 0);
+                       E.next(), r2_src, nr2, pap_part, npap, E.delta2, 0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -400,11 +436,13 @@ int brestart(BEngine &E, qkx_spinor_s *p, qkx_spinor_s *r, int initial)
     });
     if (rc) return rc;
     if (initial && (rc = qkx_spinor_copy(p, r))) return rc;
+    const double *r2_src = g_bpart + g_bpart_cap;
+    int nr2 = nblk(n);
+    reduce_if_many(r2_src, nr2, nr, g_bred + NR);
     rc = with_spinor(p, [&](auto P) {
         return with_spinor(r, [&](auto R) {
             hipLaunchKernelGGL((cgb_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream,
-                               P, R, initial ? 0 : n, nr, E.cur(), E.next(), g_bpart + g_bpart_cap, nblk(n), g_bpart,
-                               0, 0.0, 1);
+                               P, R, initial ? 0 : n, nr, E.cur(), E.next(), r2_src, nr2, g_bpart, 0, 0.0, 1);
             QKX_HIP(hipGetLastError());
             return QKX_SUCCESS;
         });

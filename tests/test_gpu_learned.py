@@ -119,20 +119,23 @@ def test_gcr_without_net_matches_oracle(q, lat):
     assert st.converged and res <= 1e-10 and abs(st.true_residual - res) <= 1e-3 * res + 1e-14
 
 
+# kappa below the critical value of these eps = 0.3 fields, where restarted
+# GCR on D converges without help, so the network's gain is measurable
 L8 = (8, 8, 8, 8)
-K8, M8, E8 = 0.1625, 0.004, 0.3
+K8, M8, E8 = 0.13, 0.01, 0.3
 
 
 def test_training_and_gain(q):
     """Train on plain-CG pairs from two gauge fields, then solve on an unseen
-    gauge field: the trained network lowers the pair loss and the GCR
-    iteration count, and the solution meets 1e-10 against the oracle."""
+    gauge field: the trained network lowers the pair loss and cuts the GCR
+    iteration count by at least 20%, and the solution meets 1e-10 against the
+    oracle."""
     from qkxtm_mpi_amd import learned
     print("making pairs", flush=True)
     pairs = learned.make_pairs(L8, K8, M8, E8, gauge_seeds=[11, 12], sources_per_gauge=2, every=2, max_pairs=256)
     net = q.Net(16, seed=1)
     loss0 = learned.pair_loss(net, pairs, K8, M8)
-    hist = learned.train(net, pairs, K8, M8, epochs=8, batch=8, lr=3e-3, seed=2, log=print)
+    hist = learned.train(net, pairs, K8, M8, epochs=10, batch=8, lr=3e-3, seed=2, log=print)
     print(f"pairs {len(pairs)} loss {loss0:.4f} -> {hist[-1]:.4f}", flush=True)
     assert hist[-1] < 0.7 * loss0
 
@@ -149,5 +152,5 @@ def test_training_and_gain(q):
     res1 = rel(orc.tm(x1.download(), orc.apply_bc(U, L8, -1), L8, K8, M8, 1), src)
     print(f"GCR iterations without net {st0.iterations}, with net {st1.iterations} (fallback "
           f"{st1.fallback_iterations}); residual {res1:.2e}", flush=True)
-    assert st1.converged and res1 <= 1e-10
-    assert st1.iterations < st0.iterations
+    assert st0.converged and st1.converged and res1 <= 1e-10
+    assert st1.fallback_iterations == 0 and st1.iterations <= 0.8 * st0.iterations

@@ -88,3 +88,33 @@ def test_solve_zero_source(q):
     x, b = q.Spinor(L, host=q.gen_z2(L, 1)), q.Spinor(L)
     st = q.solve(x, b, u, q.params(L, 0.15, 0.01), tol=1e-10)
     assert st.converged and not np.any(x.download())
+
+
+def test_half_spinor_tiny_sites_store_as_zero(q):
+    """A site whose largest component is an FP32 denormal is stored as zero in
+    FP16 (its per-site scale would overflow on inversion), not as NaN."""
+    L = (4, 4, 4, 4)
+    psi = np.zeros((256, 4, 3), complex)
+    psi[0] = 1.0
+    psi[5] = 1e-39
+    psi[9, 2, 1] = 3e-38
+    h = q.Spinor(L, q.HALF, host=psi)
+    out = h.download()
+    assert np.all(np.isfinite(out))
+    assert not np.any(out[5])
+    assert np.allclose(out[0], 1.0, rtol=1e-3) and abs(out[9, 2, 1] - 3e-38) <= 1e-3 * 3e-38
+
+
+def test_half_inner_point_source(q):
+    """FP16 inner solve from a point source: the residual far from the source
+    falls below the smallest normal float without breaking the recurrence."""
+    L = (16, 16, 16, 32)
+    U = q.gen_gauge(L, 0.3, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    p = q.params(L, 0.1632, 0.004, 1, -1)
+    src = q.gen_point(L, q.gen_site(L, 5), 0, 0)
+    x, b = q.Spinor(L), q.Spinor(L, host=src)
+    st = q.solve(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=1e-10, maxIter=20000)
+    res = rel(orc.tm(x.download(), orc.apply_bc(U, L, -1), L, 0.1632, 0.004, 1), src)
+    print(f"fp16 point source: iterations {st.iterations}, residual {res:.2e}", flush=True)
+    assert st.converged and res <= 1e-10
