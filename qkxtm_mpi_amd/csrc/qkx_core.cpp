@@ -252,6 +252,13 @@ int qkx_prof_reset(void)
     return QKX_SUCCESS;
 }
 
+int qkx_set_rhs_tile(int rt)
+{
+    QKX_CHECK_ARG(rt >= 0 && rt <= 4, "qkx_set_rhs_tile: tile must be 0 (automatic) or 1..4");
+    qkx::set_rhs_tile(rt);
+    return QKX_SUCCESS;
+}
+
 int qkx_set_cg_batch(int iterations)
 {
     QKX_CHECK_ARG(iterations >= 1 && iterations <= 1024, "qkx_set_cg_batch: 1..1024");

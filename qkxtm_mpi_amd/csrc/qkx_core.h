@@ -24,6 +24,7 @@ int site_batch(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, 
                const double coef[][4], int g5out);
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
+void set_rhs_tile(int rt);
 // even-odd Schur operator on parity pe (qkx_ops.cpp):
 //   out_pe = [A in - kappa^2 H A^-1 H in]_pe (then g5 if g5out); the optional
 //   norm_partial collects |out|^2 per workgroup of the second hop

@@ -264,6 +264,62 @@ __device__ __forceinline__ int logical_block(int nblk8)
     return b;
 }
 
+// Parity and unit index of this thread; a unit is one (site, group of
+// right-hand sides) of the output parity, `upsite` units per site.
+__device__ __forceinline__ void hop_unit(const Geom &g, int par_out, int nblk8, int upsite, int &p, int &u)
+{
+    if (par_out >= 0) {
+        p = par_out;
+        u = logical_block(nblk8) * blockDim.x + threadIdx.x;
+    } else if (nblk8 > 0) {
+        // full lattice: XCD k owns one slab of checkerboard indices and
+        // alternates between the two parities, so the links and spinors of a
+        // slab (both parities) are fetched into one L2 only
+        const int b = blockIdx.x, i = b >> 3;
+        p = i & 1;
+        u = ((b & 7) * nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
+    } else {
+        const int o = blockIdx.x * blockDim.x + threadIdx.x;
+        const int vhn = g.Vh * upsite;
+        p = (o >= vhn) ? 1 : 0;
+        u = o - p * vhn;
+    }
+}
+
+// Checkerboard indices of the 8 neighbours of (p, cb) in the other parity,
+// in the order +x -x +y -y +z -z +t -t, and whether the t hops wrap.
+struct HopSite {
+    int nb[8];
+    bool tw_p, tw_m;
+};
+__device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
+{
+    HopSite h;
+    const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
+    const int xh = cb % g.Lh0;
+    const int rest = cb / g.Lh0;
+    const int y = rest % L1;
+    const int zt = rest / L1;
+    const int z = zt % L2;
+    const int t = zt / L2;
+    const int x = 2 * xh + ((p + y + z + t) & 1);
+    const int xp = (x + 1 == L0) ? 0 : x + 1;
+    const int xm = (x == 0) ? L0 - 1 : x - 1;
+    h.nb[0] = (xp >> 1) + g.Lh0 * rest;
+    h.nb[1] = (xm >> 1) + g.Lh0 * rest;
+    h.nb[2] = cb + g.Lh0 * ((y + 1 == L1) ? -(L1 - 1) : 1);
+    h.nb[3] = cb + g.Lh0 * ((y == 0) ? (L1 - 1) : -1);
+    const int LY = L1 * g.Lh0;
+    h.nb[4] = cb + ((z + 1 == L2) ? -(L2 - 1) * LY : LY);
+    h.nb[5] = cb + ((z == 0) ? (L2 - 1) * LY : -LY);
+    const int LZ = L2 * LY;
+    h.tw_p = (t + 1 == L3);
+    h.tw_m = (t == 0);
+    h.nb[6] = cb + (h.tw_p ? -(L3 - 1) * LZ : LZ);
+    h.nb[7] = cb + (h.tw_m ? (L3 - 1) * LZ : -LZ);
+    return h;
+}
+
 // Minimum waves per SIMD requested from the register allocator for the 16-bit
 // hops (1: no request).  At 24^3x48 a half-lattice launch is 1296 workgroups;
 // residency of 6 waves per SIMD holds all of them at once.
@@ -283,54 +339,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     const Geom &g = a.g;
     const int nr = a.nrhs;
     int p, u;   // parity and unit (site x rhs) index within it
-    if (a.par_out >= 0) {
-        p = a.par_out;
-        u = logical_block(a.nblk8) * blockDim.x + threadIdx.x;
-    } else if (a.nblk8 > 0) {
-        // full lattice: XCD k owns one slab of checkerboard indices and
-        // alternates between the two parities, so the links and spinors of a
-        // slab (both parities) are fetched into one L2 only
-        const int b = blockIdx.x, i = b >> 3;
-        p = i & 1;
-        u = ((b & 7) * a.nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
-    } else {
-        const int o = blockIdx.x * blockDim.x + threadIdx.x;
-        const int vhn = g.Vh * nr;
-        p = (o >= vhn) ? 1 : 0;
-        u = o - p * vhn;
-    }
+    hop_unit(g, a.par_out, a.nblk8, nr, p, u);
     const int cb = nr == 1 ? u : u / nr;
     const int rr = u - cb * nr;
     double nrm2 = 0.0;
     if (cb < g.Vh) {
-    // coordinates: cb = xh + Lh0 * rest,  rest = y + Ly*(z + Lz*t)
-    const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
-    const int xh = cb % g.Lh0;
-    const int rest = cb / g.Lh0;
-    const int y = rest % L1;
-    const int zt = rest / L1;
-    const int z = zt % L2;
-    const int t = zt / L2;
-    const int x = 2 * xh + ((p + y + z + t) & 1);
+    const HopSite hs = hop_site(g, p, cb);
     const int q = p ^ 1;
     const int qoff = q * g.Vh, poff = p * g.Vh;
     const int in_q = a.in_off[q];
-
-    const int xp = (x + 1 == L0) ? 0 : x + 1;
-    const int xm = (x == 0) ? L0 - 1 : x - 1;
-    const int cb_xp = (xp >> 1) + g.Lh0 * rest;
-    const int cb_xm = (xm >> 1) + g.Lh0 * rest;
-    const int ry = (y + 1 == L1) ? -(L1 - 1) : 1;
-    const int rym = (y == 0) ? (L1 - 1) : -1;
-    const int cb_yp = cb + g.Lh0 * ry;
-    const int cb_ym = cb + g.Lh0 * rym;
-    const int LY = L1 * g.Lh0;
-    const int cb_zp = cb + ((z + 1 == L2) ? -(L2 - 1) * LY : LY);
-    const int cb_zm = cb + ((z == 0) ? (L2 - 1) * LY : -LY);
-    const int LZ = L2 * LY;
-    const bool tw_p = (t + 1 == L3), tw_m = (t == 0);
-    const int cb_tp = cb + (tw_p ? -(L3 - 1) * LZ : LZ);
-    const int cb_tm = cb + (tw_m ? (L3 - 1) * LZ : -LZ);
+    const int cb_xp = hs.nb[0], cb_xm = hs.nb[1], cb_yp = hs.nb[2], cb_ym = hs.nb[3];
+    const int cb_zp = hs.nb[4], cb_zm = hs.nb[5], cb_tp = hs.nb[6], cb_tm = hs.nb[7];
+    const bool tw_p = hs.tw_p, tw_m = hs.tw_m;
     const T bc = T(g.bc);
 
     Cx<T> R[12];
@@ -420,6 +440,123 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     }
 }
 
+// One hop for RT right-hand sides of one site: the link is fetched and
+// unpacked once, then applied to the RT neighbour spinors (consecutive in
+// the batch layout).
+template <int B, int MU, int KIND, bool DAGL, int RT, class SIn, class G, class T>
+__device__ __forceinline__ void tile_dir(Cx<T> (&R)[RT][12], const SIn &in, const G &U, int s_nb, int s_link,
+                                         T sign)
+{
+    const typename G::Raw lr = U.fetch(MU, s_link);
+    typename SIn::Raw pr[RT];
+#pragma unroll
+    for (int j = 0; j < RT; j++) pr[j] = in.template fetch<spin_mask<B, MU, KIND>()>(s_nb + j);
+    Cx<T> Ul[9];
+    U.unpack(lr, Ul);
+#pragma unroll
+    for (int j = 0; j < RT; j++) {
+        Cx<T> psi[12];
+        in.template unpack<spin_mask<B, MU, KIND>()>(pr[j], psi);
+        Cx<T> h[6];
+        project_k<B, MU, KIND, 0>(psi, h);
+        project_k<B, MU, KIND, 1>(psi, h);
+        if (sign != T(1)) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) h[k] = sign * h[k];
+        }
+        Cx<T> xi[6];
+        link_mul<DAGL>(Ul, h, xi);
+        collect_s<B, MU, KIND, 0>(R[j], xi);
+        collect_s<B, MU, KIND, 1>(R[j], xi);
+        collect_s<B, MU, KIND, 2>(R[j], xi);
+        collect_s<B, MU, KIND, 3>(R[j], xi);
+    }
+}
+
+template <int RT, class T> __device__ __forceinline__ void pin_tile(Cx<T> (&R)[RT][12])
+{
+#pragma unroll
+    for (int j = 0; j < RT; j++) pin_acc(R[j]);
+}
+
+// Hop kernel for batches: one thread per (site, RT consecutive right-hand
+// sides), so each link is fetched once per RT spinors instead of once per
+// spinor.  Same epilogue, boundary and mapping rules as hop_kernel.
+template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int RT>
+__global__ __launch_bounds__(256) void hop_tile_kernel(HopArgs<SIn, SOut, SAux, G> a)
+{
+    using T = typename SOut::Real;
+    if (a.skip && *a.skip) return;   // uniform over the grid
+    const Geom &g = a.g;
+    const int nr = a.nrhs, ng = nr / RT;
+    int p, u;
+    hop_unit(g, a.par_out, a.nblk8, ng, p, u);
+    const int cb = u / ng;
+    const int r0 = (u - cb * ng) * RT;
+    double nrm2[RT];
+#pragma unroll
+    for (int j = 0; j < RT; j++) nrm2[j] = 0.0;
+    if (cb < g.Vh) {
+        const HopSite hs = hop_site(g, p, cb);
+        const int poff = p * g.Vh, qoff = (p ^ 1) * g.Vh;
+        const int in_q = a.in_off[p ^ 1];
+        const T bc = T(g.bc);
+        Cx<T> R[RT][12];
+#pragma unroll
+        for (int j = 0; j < RT; j++)
+#pragma unroll
+            for (int k = 0; k < 12; k++) R[j][k] = {T(0), T(0)};
+        constexpr int FW = DAG ? 1 : 0;
+        constexpr int BW = DAG ? 0 : 1;
+        const SIn &in = a.in;
+        const G &U = a.U;
+        tile_dir<B, 0, FW, false, RT>(R, in, U, in_q + hs.nb[0] * nr + r0, poff + cb, T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 0, BW, true, RT>(R, in, U, in_q + hs.nb[1] * nr + r0, qoff + hs.nb[1], T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 1, FW, false, RT>(R, in, U, in_q + hs.nb[2] * nr + r0, poff + cb, T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 1, BW, true, RT>(R, in, U, in_q + hs.nb[3] * nr + r0, qoff + hs.nb[3], T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 2, FW, false, RT>(R, in, U, in_q + hs.nb[4] * nr + r0, poff + cb, T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 2, BW, true, RT>(R, in, U, in_q + hs.nb[5] * nr + r0, qoff + hs.nb[5], T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 3, FW, false, RT>(R, in, U, in_q + hs.nb[6] * nr + r0, poff + cb, hs.tw_p ? bc : T(1));
+        pin_tile<RT>(R);
+        tile_dir<B, 3, BW, true, RT>(R, in, U, in_q + hs.nb[7] * nr + r0, qoff + hs.nb[7], hs.tw_m ? bc : T(1));
+#pragma unroll
+        for (int j = 0; j < RT; j++) {
+            const Epi ep = a.epi[r0 + j];
+            Cx<T> res[12];
+            if constexpr (AUX) {
+                Cx<T> xv[12];
+                a.aux.load(a.aux_off[p] + cb * nr + r0 + j, xv);
+                epilogue<B>(ep, R[j], xv, res);
+            } else {
+                epilogue<B, T>(ep, R[j], nullptr, res);
+            }
+            a.out.store(a.out_off[p] + cb * nr + r0 + j, res);
+            if (a.norm_partial) {
+#pragma unroll
+                for (int k = 0; k < 12; k++)
+                    nrm2[j] += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+            }
+        }
+    }
+    if (a.norm_partial) {
+        __shared__ double acc[QKX_MAX_RHS];
+        if (threadIdx.x < QKX_MAX_RHS) acc[threadIdx.x] = 0.0;
+        __syncthreads();
+        if (cb < g.Vh) {
+#pragma unroll
+            for (int j = 0; j < RT; j++) atomicAdd(&acc[r0 + j], nrm2[j]);
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < nr) a.norm_partial[blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+    }
+}
+
 // Site-local combination: out = cR y + gR i g5 y + cX x + gX i g5 x (g5out),
 // optionally restricted to one parity with zeros written on the other.
 struct EpiSet {
@@ -459,6 +596,7 @@ int g_xcd_remap = 1;
 int g_block = 256;
 int g_depth64 = 2;   // pipeline depth of the FP64 hop (2 measured faster than 1 at 24^3x48)
 int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
+int g_rhs_tile = 0;  // right-hand sides per thread in batched hops (0 or 1: one)
 const int *g_skip = nullptr;
 
 template <class S, class G, int B>
@@ -482,34 +620,53 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     for (int r = 0; r < a.nrhs; r++) a.epi[r] = e[r];
     a.skip = g_skip;
     a.norm_partial = norm_partial;
-    int nb = (a.nout + g_block - 1) / g_block;
+    // right-hand sides per thread for batches (links fetched once per tile)
+    // (automatic: 1 -- at 24^3x48 with 12 right-hand sides, tiles of 2, 3, 4
+    // lose more to lower occupancy than they save in link loads)
+    int rt = 1;
+    if (a.nrhs > 1 && g_rhs_tile > 1 && g_rhs_tile <= 4 && a.nrhs % g_rhs_tile == 0) rt = g_rhs_tile;
+    const int upsite = a.nrhs / rt;   // threads per output site
+    const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
+    int nb = (sites_out * upsite + g_block - 1) / g_block;
     a.nblk8 = 0;
     if (g_xcd_remap && nb >= 64) {
         if (par_out >= 0) {
             nb = (nb + 7) / 8 * 8;
             a.nblk8 = nb / 8;
         } else {
-            int nbh = ((a.g.Vh * a.nrhs + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
+            int nbh = ((a.g.Vh * upsite + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
             a.nblk8 = nbh / 8;
             nb = 2 * nbh;
         }
     }
     if (nblocks_out) *nblocks_out = nb;
-    // bytes moved per launch (compulsory traffic) for the instrumentation
+    // bytes moved per launch (compulsory traffic) for the instrumentation:
+    // links once per site, spinors once per (site, rhs)
     const int nout = a.nout;
     double ws = (double)qkx::spinor_elem_bytes(in->prec) * qkx::spinor_nchunks(in->prec) +
                 (in->prec == QKX_PREC_HALF ? 4.0 : 0.0);
     double wg = (U->prec == QKX_PREC_DOUBLE ? 16.0 : U->prec == QKX_PREC_SINGLE ? 8.0 : 4.0) *
                 (U->recon == QKX_RECON_18 ? 9 : 6);
-    double bytes = par_out < 0 ? (double)nout * (4 * wg + 2 * ws)
-                               : (double)nout * (8 * wg + 2 * ws);
+    double bytes = (double)sites_out * (par_out < 0 ? 4 : 8) * wg + (double)nout * 2 * ws;
     if (aux && aux != in) bytes += (double)nout * ws;
     qkx::prof_begin();
     const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
     const dim3 grid(nb), blk(g_block);
     hipStream_t st = qkx::g_stream;
 #define QKX_LAUNCH_HOP(DG, AX, DP) hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, DG, AX, DP>), grid, blk, 0, st, a)
-    if (depth == 2) {
+#define QKX_LAUNCH_TILE(DG, AX, RT) hipLaunchKernelGGL((hop_tile_kernel<S, S, S, G, B, DG, AX, RT>), grid, blk, 0, st, a)
+#define QKX_LAUNCH_TILES(RT)                                                              \
+    do {                                                                                  \
+        if (dagger) { if (aux) QKX_LAUNCH_TILE(true, true, RT); else QKX_LAUNCH_TILE(true, false, RT); } \
+        else        { if (aux) QKX_LAUNCH_TILE(false, true, RT); else QKX_LAUNCH_TILE(false, false, RT); } \
+    } while (0)
+    if (rt == 4) {
+        QKX_LAUNCH_TILES(4);
+    } else if (rt == 3) {
+        QKX_LAUNCH_TILES(3);
+    } else if (rt == 2) {
+        QKX_LAUNCH_TILES(2);
+    } else if (depth == 2) {
         if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else QKX_LAUNCH_HOP(true, false, 2); }
         else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else QKX_LAUNCH_HOP(false, false, 2); }
     } else {
@@ -517,6 +674,8 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
         else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else QKX_LAUNCH_HOP(false, false, 1); }
     }
 #undef QKX_LAUNCH_HOP
+#undef QKX_LAUNCH_TILE
+#undef QKX_LAUNCH_TILES
     qkx::prof_end(bytes, in->prec);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
@@ -542,6 +701,8 @@ int launch_site_t(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *
 namespace qkx {
 
 void set_hop_skip(const int *flag) { g_skip = flag; }
+
+void set_rhs_tile(int rt) { g_rhs_tile = rt; }
 
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
