@@ -402,6 +402,7 @@ int qkx_gauge_create(qkx_gauge **out, const int L[4], int prec, int recon, int b
     g->recon = recon;
     g->bc = boundaryT;
     g->inner = nullptr;
+    g->mid = nullptr;
     size_t per = (prec == QKX_PREC_DOUBLE ? 16 : prec == QKX_PREC_SINGLE ? 8 : 4) * (recon == 18 ? 9 : 6);
     g->bytes = per * 4 * (size_t)g->V;
     if (hipMalloc(&g->data, g->bytes) != hipSuccess) {
@@ -418,6 +419,7 @@ int qkx_gauge_destroy(qkx_gauge *g)
     if (!g) return QKX_SUCCESS;
     (void)hipStreamSynchronize(qkx::g_stream);
     if (g->inner) qkx_gauge_destroy(g->inner);
+    if (g->mid) qkx_gauge_destroy(g->mid);
     if (g->data) (void)hipFree(g->data);
     delete g;
     return QKX_SUCCESS;

@@ -463,6 +463,8 @@ struct qkx_gauge_s {
     size_t bytes;
     // cached low-precision copy for the mixed solvers (owned)
     qkx_gauge_s *inner;
+    // cached FP32 12-real copy for the reliable updates of the mixed solvers (owned)
+    qkx_gauge_s *mid;
 };
 
 // library state

@@ -25,6 +25,7 @@
 #include "qkx_internal.h"
 #include "qkx_core.h"
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <vector>
 
@@ -548,8 +549,40 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     qkx::Temp lf(L, lp, QKX_SUBSET_FULL), lo_(L, lp, QKX_SUBSET_ODD), le(L, lp, QKX_SUBSET_EVEN);
     NormalOp Al{Ui, kappa, a, nrm, o->eo != 0, lf.f, lo_.f, le.f};
 
+    // FP32 operator for the parts whose result is stored at FP16/FP32 anyway:
+    // the normal-equation right-hand side and the residual of each reliable
+    // update (r = Op^dag s); s itself, the true residual, stays FP64
+    const qkx_gauge_s *Um = U;
+    if (mixed) {
+        qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
+        if (!Uc->mid) {
+            if ((rc = qkx_gauge_create(&Uc->mid, L, QKX_PREC_SINGLE, QKX_RECON_12, U->bc))) return rc;
+            if ((rc = qkx_gauge_convert(Uc->mid, U))) return rc;
+        }
+        Um = Uc->mid;
+    }
+    std::unique_ptr<qkx::Temp> mo, me, mf, sm, rm;   // FP32 temporaries of a mixed solve
+    NormalOp Am{Um, kappa, a, nrm, o->eo != 0, nullptr, nullptr, nullptr};
+    if (mixed) {
+        const int S = QKX_PREC_SINGLE;
+        if (o->eo) {
+            mo.reset(new qkx::Temp(L, S, QKX_SUBSET_ODD));
+            me.reset(new qkx::Temp(L, S, QKX_SUBSET_EVEN));
+            if (!mo->f || !me->f) return QKX_ERR_NOMEM;
+            Am.tmp_odd = mo->f;
+            Am.tmp_even = me->f;
+        } else {
+            mf.reset(new qkx::Temp(L, S, QKX_SUBSET_FULL));
+            if (!mf->f) return QKX_ERR_NOMEM;
+            Am.tmp_full = mf->f;
+        }
+        sm.reset(new qkx::Temp(L, S, sub));
+        rm.reset(new qkx::Temp(L, S, sub));
+        if (!sm->f || !rm->f) return QKX_ERR_NOMEM;
+    }
+
     qkx::Temp bh(L, QKX_PREC_DOUBLE, sub), bn(L, QKX_PREC_DOUBLE, sub), xs(L, QKX_PREC_DOUBLE, sub),
-        s(L, QKX_PREC_DOUBLE, sub), rh(L, QKX_PREC_DOUBLE, sub), bo(L, QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
+        s(L, QKX_PREC_DOUBLE, sub), bo(L, QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
         to(L, QKX_PREC_DOUBLE, QKX_SUBSET_ODD);
     // p and Ap in the inner storage precision; the solution increment in FP32
     // (FP64 for a pure FP64 solve); the residual in FP32 for BF16 storage
@@ -557,7 +590,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     const int xp = lp == QKX_PREC_DOUBLE ? QKX_PREC_DOUBLE : QKX_PREC_SINGLE;
     const int rp = lp == QKX_PREC_BF16 ? QKX_PREC_SINGLE : lp;
     qkx::Temp r(L, rp, sub), pp(L, lp, sub), Ap(L, lp, sub), xl(L, xp, sub);
-    if (!hf.f || !ho.f || !he.f || !lf.f || !lo_.f || !le.f || !bh.f || !bn.f || !xs.f || !s.f || !rh.f || !bo.f ||
+    if (!hf.f || !ho.f || !he.f || !lf.f || !lo_.f || !le.f || !bh.f || !bn.f || !xs.f || !s.f || !bo.f ||
         !to.f || !r.f || !pp.f || !Ap.f || !xl.f)
         return QKX_ERR_NOMEM;
 
@@ -577,8 +610,15 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     } else {
         if ((rc = qkx_spinor_copy(bh.f, b))) return rc;
     }
-    // normal-equation rhs bn = Op^dag bh; x = 0; r = bn; p = r
-    if ((rc = Ah.apply_dag(bn.f, bh.f))) return rc;
+    // normal-equation rhs bn = Op^dag bh (FP32 for a mixed solve: it only
+    // seeds r and the first stopping guess); x = 0; r = bn; p = r
+    if (mixed) {
+        if ((rc = qkx_spinor_copy(sm->f, bh.f)) || (rc = Am.apply_dag(rm->f, sm->f)) ||
+            (rc = qkx_spinor_copy(bn.f, rm->f)))
+            return rc;
+    } else if ((rc = Ah.apply_dag(bn.f, bh.f))) {
+        return rc;
+    }
     if ((rc = qkx_spinor_zero(xs.f)) || (rc = qkx_spinor_zero(xl.f))) return rc;
     if ((rc = qkx_spinor_copy(r.f, bn.f))) return rc;
 
@@ -595,8 +635,23 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     int status = QKX_NOT_CONVERGED;
     int nrel = 0;
     qkx_spinor_s *xiter = mixed ? xl.f : xs.f;
+    int k_prev = -1;
+    double r2_prev = 0;
     while (h.k < o->maxIter) {
+        // Enqueue about as many iterations as the residual needs to reach the
+        // next event (stop or reliable update) at its recent rate of decrease:
+        // launches past an event exit at once but still cost a dispatch each.
         int batch = std::min(qkx::cg_batch_size, o->maxIter - h.k);
+        if (k_prev >= 0 && h.k > k_prev && h.r2 > 0 && r2_prev > h.r2 && qkx::cg_batch_size > 2) {
+            const double rate = std::log(h.r2 / r2_prev) / (h.k - k_prev);   // < 0
+            const double thr = std::max(E.stop2, E.delta2 * h.maxr2);
+            if (thr < h.r2) {
+                const double need = std::log(thr / h.r2) / rate;
+                batch = std::min(batch, std::max(2, (int)std::ceil(need) + 1));
+            }
+        }
+        k_prev = h.k;
+        r2_prev = h.r2;
         for (int i = 0; i < batch; i++)
             if ((rc = enqueue_iteration(E, Al, xiter, r.f, pp.f, Ap.f))) return rc;
         if ((rc = read_slot(E, h))) return rc;
@@ -624,8 +679,12 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
             // reliable update: r = Op^dag s in FP64, into the low-precision r;
             // p = r + beta p and the slot restart on the device
             nrel++;
-            if ((rc = Ah.apply_dag(rh.f, s.f))) return rc;
-            if ((rc = qkx_spinor_copy(r.f, rh.f))) return rc;
+            if ((rc = qkx_spinor_copy(sm->f, s.f))) return rc;
+            if (r.f->prec == QKX_PREC_SINGLE) {
+                if ((rc = Am.apply_dag(r.f, sm->f))) return rc;
+            } else if ((rc = Am.apply_dag(rm->f, sm->f)) || (rc = qkx_spinor_copy(r.f, rm->f))) {
+                return rc;
+            }
             if ((rc = restart(E, pp.f, r.f, 1))) return rc;
         } else {
             // continue the same recurrence: p = r + beta p with the iterated r2
