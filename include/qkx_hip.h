@@ -123,6 +123,10 @@ int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
 /* CG iterations enqueued between host reads of the solver state (default 16;
  * 1 gives no launches past the stopping iteration). */
 int  qkx_set_cg_batch(int iterations);
+/* 1 (default): the CG update x += alpha p, r -= alpha Ap is fused into the
+ * last hop of the normal operator (no Ap field, one launch less per
+ * iteration); 0: separate update kernel. */
+int  qkx_set_cg_fuse(int on);
 /* Right-hand sides per thread in batched hops (1..4, each link fetched once
  * per tile; a tile that does not divide nrhs falls back to 1; 0 = automatic,
  * which is 1: wider tiles were measured slower on MI355X). */

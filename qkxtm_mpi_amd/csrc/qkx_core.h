@@ -22,6 +22,19 @@ int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int ba
          double cR, double gR, double cX, double gX, int g5out);
 int site_batch(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
                const double coef[][4], int g5out);
+// The last hop of a CG normal operator with the CG update fused in:
+// out = epilogue(hop) is not stored; instead, with alpha = *r2cur / sum(pap),
+// x += alpha p and r -= alpha out at every output site, and per-workgroup
+// |r|^2 partials go to r2_part (count in *nblocks_out).
+struct CgFuse {
+    qkx_spinor_s *x, *r, *p;
+    const double *r2cur;
+    const double *pap_part;
+    int npap;
+    double *r2_part;
+};
+int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
+           double cR, double gR, double cX, double gX, int g5out, const CgFuse &f, int *nblocks_out);
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);

@@ -15,6 +15,7 @@
 // include/hopping_term_chiral.h (chiral).  The table entries are the
 // reference's conventions and are necessarily identical to it.
 #include "qkx_internal.h"
+#include <type_traits>
 #include "qkx_core.h"
 
 namespace {
@@ -320,6 +321,41 @@ __device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
     return h;
 }
 
+// What a hop thread does with its result.  StoreFin: write it (and, on
+// request, per-workgroup |out|^2 partials).  CgFin: the CG update fused into
+// the last hop of the normal operator -- with alpha = r2 / p.Ap (p.Ap summed
+// from the partials of an earlier hop, in a fixed order), x += alpha p and
+// r -= alpha Ap at this site, per-workgroup |r|^2 partials; Ap itself is never
+// stored.  x, r, p share the output field's layout.
+struct StoreFin {
+    template <class A> __device__ __forceinline__ void begin(const A &) {}
+};
+
+template <class SX, class SR, class SP> struct CgFin {
+    SX x;
+    SR r;
+    SP p;
+    const double *r2cur;      // |r|^2 of the current iteration (device)
+    const double *pap_part;   // p.Ap partials
+    int npap;
+    double *r2_part;          // out: per-workgroup |r|^2 after the update
+    double alpha;
+    template <class A> __device__ __forceinline__ void begin(const A &)
+    {
+        __shared__ double lds[256];
+        double v = 0;
+        for (int i = threadIdx.x; i < npap; i += blockDim.x) v += pap_part[i];
+        lds[threadIdx.x] = v;
+        __syncthreads();
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
+            __syncthreads();
+        }
+        alpha = *r2cur / lds[0];
+        __syncthreads();
+    }
+};
+
 // Minimum waves per SIMD requested from the register allocator for the 16-bit
 // hops (1: no request).  At 24^3x48 a half-lattice launch is 1296 workgroups;
 // residency of 6 waves per SIMD holds all of them at once.
@@ -330,12 +366,14 @@ template <class S> struct HopWaves { static constexpr int v = 1; };
 template <> struct HopWaves<SpinorH> { static constexpr int v = QKX_HOP16_WAVES; };
 template <> struct HopWaves<SpinorB> { static constexpr int v = QKX_HOP16_WAVES; };
 
-template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH>
+template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH, class Fin = StoreFin>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SIn>::v, 8))) void hop_kernel(
-    HopArgs<SIn, SOut, SAux, G> a)
+    HopArgs<SIn, SOut, SAux, G> a, Fin fin)
 {
     using T = typename SOut::Real;
+    constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
     if (a.skip && *a.skip) return;   // uniform over the grid
+    fin.begin(a);
     const Geom &g = a.g;
     const int nr = a.nrhs;
     int p, u;   // parity and unit (site x rhs) index within it
@@ -419,13 +457,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     } else {
         epilogue<B, T>(ep, R, nullptr, res);
     }
-    a.out.store(a.out_off[p] + cb * nr + rr, res);
-    if (a.norm_partial) {
+    if constexpr (CG) {
+        // x += alpha p, r -= alpha Ap (single right-hand side)
+        const int so = a.out_off[p] + cb;
+        using TX = typename decltype(fin.x)::Real;
+        using TR = typename decltype(fin.r)::Real;
+        using TP = typename decltype(fin.p)::Real;
+        Cx<TX> xv[12];
+        Cx<TP> pv[12];
+        fin.x.load(so, xv);
+        fin.p.load(so, pv);
+        const TX ax = TX(fin.alpha);
 #pragma unroll
-        for (int k = 0; k < 12; k++) nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+        fin.x.store(so, xv);
+        Cx<TR> rv[12];
+        fin.r.load(so, rv);
+        const TR ar = TR(fin.alpha);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            rv[k] = {rv[k].re - ar * TR(res[k].re), rv[k].im - ar * TR(res[k].im)};
+            nrm2 += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+        }
+        fin.r.store(so, rv);
+    } else {
+        a.out.store(a.out_off[p] + cb * nr + rr, res);
+        if (a.norm_partial) {
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+        }
     }
     }
-    if (a.norm_partial) {
+    if constexpr (CG) {
+        block_partial(nrm2, fin.r2_part + blockIdx.x);
+    } else if (a.norm_partial) {
         if (nr == 1) {
             block_partial(nrm2, a.norm_partial + blockIdx.x);
         } else {
@@ -599,10 +665,10 @@ int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 int g_rhs_tile = 0;  // right-hand sides per thread in batched hops (0 or 1: one)
 const int *g_skip = nullptr;
 
-template <class S, class G, int B>
+template <class S, class G, int B, class Fin = StoreFin>
 int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
                  const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, double *norm_partial,
-                 int *nblocks_out)
+                 int *nblocks_out, const Fin &fin = Fin{})
 {
     HopArgs<S, S, S, G> a;
     a.g = qkx::make_geom(U->L, U->bc);
@@ -653,19 +719,21 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
     const dim3 grid(nb), blk(g_block);
     hipStream_t st = qkx::g_stream;
-#define QKX_LAUNCH_HOP(DG, AX, DP) hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, DG, AX, DP>), grid, blk, 0, st, a)
+#define QKX_LAUNCH_HOP(DG, AX, DP) \
+    hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, DG, AX, DP, Fin>), grid, blk, 0, st, a, fin)
 #define QKX_LAUNCH_TILE(DG, AX, RT) hipLaunchKernelGGL((hop_tile_kernel<S, S, S, G, B, DG, AX, RT>), grid, blk, 0, st, a)
 #define QKX_LAUNCH_TILES(RT)                                                              \
     do {                                                                                  \
         if (dagger) { if (aux) QKX_LAUNCH_TILE(true, true, RT); else QKX_LAUNCH_TILE(true, false, RT); } \
         else        { if (aux) QKX_LAUNCH_TILE(false, true, RT); else QKX_LAUNCH_TILE(false, false, RT); } \
     } while (0)
-    if (rt == 4) {
-        QKX_LAUNCH_TILES(4);
-    } else if (rt == 3) {
-        QKX_LAUNCH_TILES(3);
-    } else if (rt == 2) {
-        QKX_LAUNCH_TILES(2);
+    constexpr bool store = std::is_same_v<Fin, StoreFin>;
+    if (store && rt == 4) {
+        if constexpr (store) QKX_LAUNCH_TILES(4);
+    } else if (store && rt == 3) {
+        if constexpr (store) QKX_LAUNCH_TILES(3);
+    } else if (store && rt == 2) {
+        if constexpr (store) QKX_LAUNCH_TILES(2);
     } else if (depth == 2) {
         if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else QKX_LAUNCH_HOP(true, false, 2); }
         else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else QKX_LAUNCH_HOP(false, false, 2); }
@@ -773,6 +841,56 @@ int hop_batch(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux
             return launch_hop_t<SpinorB, GaugeB12, QKX_BASIS_UKQCD>(out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out);
     }
     set_error("hop: unsupported spinor/gauge precision combination");
+    return QKX_ERR_ARG;
+}
+
+// The last hop of a CG normal operator with the CG update fused in (UKQCD
+// basis, single right-hand side).  in/aux/p share one precision; x and r are
+// the solver's accumulator and residual.
+template <class S, class G>
+int launch_hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
+                  const Epi &e, const CgFuse &f, int *nblocks_out)
+{
+    auto go = [&](auto X, auto R) {
+        CgFin<decltype(X), decltype(R), S> fin{X, R, make_spinor<S>(f.p), f.r2cur, f.pap_part, f.npap, f.r2_part,
+                                               0.0};
+        return launch_hop_t<S, G, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, &e, nullptr, nblocks_out, fin);
+    };
+    const int xp = f.x->prec, rp = f.r->prec;
+    if constexpr (std::is_same_v<S, SpinorD>) {
+        if (xp == QKX_PREC_DOUBLE && rp == QKX_PREC_DOUBLE) return go(make_spinor<SpinorD>(f.x), make_spinor<SpinorD>(f.r));
+    } else {
+        if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_SINGLE) return go(make_spinor<SpinorF>(f.x), make_spinor<SpinorF>(f.r));
+        if constexpr (std::is_same_v<S, SpinorH>) {
+            if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_HALF) return go(make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r));
+        }
+    }
+    set_error("hop_cg: unsupported x/r precision combination");
+    return QKX_ERR_ARG;
+}
+
+int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
+           double cR, double gR, double cX, double gX, int g5out, const CgFuse &f, int *nblocks_out)
+{
+    if (in->nrhs != 1 || f.p->prec != in->prec || (aux && aux->prec != in->prec)) {
+        set_error("hop_cg: single right-hand side, one precision for in, aux and p");
+        return QKX_ERR_ARG;
+    }
+    const Epi e{cR, gR, cX, gX, g5out};
+    const int sp = in->prec;
+    if (sp == QKX_PREC_DOUBLE && U->prec == QKX_PREC_DOUBLE)
+        return U->recon == QKX_RECON_18 ? launch_hop_cg<SpinorD, GaugeD18>(in, aux, U, dagger, par_out, e, f, nblocks_out)
+                                         : launch_hop_cg<SpinorD, GaugeD12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
+    if (sp == QKX_PREC_SINGLE && U->prec == QKX_PREC_SINGLE)
+        return U->recon == QKX_RECON_18 ? launch_hop_cg<SpinorF, GaugeF18>(in, aux, U, dagger, par_out, e, f, nblocks_out)
+                                         : launch_hop_cg<SpinorF, GaugeF12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
+    if (sp == QKX_PREC_HALF && U->prec == QKX_PREC_HALF && U->recon == QKX_RECON_12)
+        return launch_hop_cg<SpinorH, GaugeH12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
+    if (sp == QKX_PREC_HALF && U->prec == QKX_PREC_SINGLE && U->recon == QKX_RECON_12)
+        return launch_hop_cg<SpinorH, GaugeF12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
+    if (sp == QKX_PREC_BF16 && U->prec == QKX_PREC_BF16 && U->recon == QKX_RECON_12)
+        return launch_hop_cg<SpinorB, GaugeB12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
+    set_error("hop_cg: unsupported spinor/gauge precision combination");
     return QKX_ERR_ARG;
 }
 

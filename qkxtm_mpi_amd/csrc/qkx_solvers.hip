@@ -42,6 +42,7 @@ struct CGSlot {
     int k;          // iterations done
 };
 
+int g_cg_fuse = 1;               // fuse the CG update into the operator's last hop
 CGSlot *g_slots = nullptr;       // device, two slots
 CGSlot *g_slots_host = nullptr;  // pinned mirror
 double *g_part = nullptr;        // partials: [0, cap) p.Ap, [cap, 2 cap) |r|^2
@@ -260,6 +261,29 @@ struct NormalOp {
         if (rc) return rc;
         return qkx::schur(out, tmp_even, U, kappa, -a, nrm, QKX_BASIS_UKQCD, 0, 1, tmp_odd);
     }
+    // The same with the CG update fused into the last hop (no Ap field): the
+    // first half leaves the |Op p|^2 partials, which f then points at
+    // (pre-summed when many); the last hop updates x and r and leaves |r|^2
+    // partials in f.r2_part, *nr2 of them.
+    int apply_cg(const qkx_spinor_s *in, double *part, qkx::CgFuse &f, int *nr2) const
+    {
+        int np = 0, rc;
+        if (!eo) {
+            if ((rc = qkx::hop(tmp_full, in, in, U, QKX_BASIS_UKQCD, 0, -1, -kappa, 0, 1.0, a, 0, part, &np)))
+                return rc;
+        } else if ((rc = qkx::schur(tmp_even, in, U, kappa, a, nrm, QKX_BASIS_UKQCD, 0, 1, tmp_odd, part, &np))) {
+            return rc;
+        }
+        const double *pp = part;
+        reduce_if_many(pp, np, g_red);
+        f.pap_part = pp;
+        f.npap = np;
+        if (!eo) return qkx::hop_cg(tmp_full, tmp_full, U, 1, -1, -kappa, 0, 1.0, -a, 0, f, nr2);
+        // second Schur half with the twist sign flipped: hop to the odd sites, then the fused hop back
+        if ((rc = qkx::hop(tmp_odd, tmp_even, nullptr, U, QKX_BASIS_UKQCD, 0, 1, 1.0 / nrm, a / nrm, 0.0, 0.0, 0)))
+            return rc;
+        return qkx::hop_cg(tmp_odd, tmp_even, U, 0, 0, -kappa * kappa, 0.0, 1.0, -a, 1, f, nr2);
+    }
     int apply_op(qkx_spinor_s *out, const qkx_spinor_s *in) const
     {
         if (!eo) return qkx::hop(out, in, in, U, QKX_BASIS_UKQCD, 0, -1, -kappa, 0, 1.0, a, 0);
@@ -309,15 +333,43 @@ int enqueue_blas(Engine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, q
     return QKX_SUCCESS;
 }
 
+// direction step alone (the update was fused into the operator's last hop)
+template <class S, class SR>
+int enqueue_dir(Engine &E, qkx_spinor_s *r, qkx_spinor_s *p, const qkx::CgFuse &f, int nr2)
+{
+    const int n = entries(p);
+    const double *r2_src = f.r2_part;
+    reduce_if_many(r2_src, nr2, g_red + 1);
+    hipLaunchKernelGGL((cg_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, make_spinor<S>(p),
+                       make_spinor<SR>(r), n, E.cur(), E.next(), r2_src, nr2, f.pap_part, f.npap, E.stop2, E.delta2,
+                       0);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
 int enqueue_iteration(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
                       qkx_spinor_s *Ap)
 {
+    const int px = x->prec, pr = r->prec, pp = p->prec;
+    if (g_cg_fuse) {
+        qkx::set_hop_skip(&E.cur()->flag);
+        qkx::CgFuse f{x, r, p, &E.cur()->r2, nullptr, 0, g_part + g_part_cap};
+        int nr2 = 0;
+        int rc = A.apply_cg(p, g_part, f, &nr2);
+        qkx::set_hop_skip(nullptr);
+        if (rc) return rc;
+        if (pp == QKX_PREC_DOUBLE) rc = enqueue_dir<SpinorD, SpinorD>(E, r, p, f, nr2);
+        else if (pp == QKX_PREC_SINGLE) rc = enqueue_dir<SpinorF, SpinorF>(E, r, p, f, nr2);
+        else if (pp == QKX_PREC_HALF) rc = enqueue_dir<SpinorH, SpinorH>(E, r, p, f, nr2);
+        else rc = enqueue_dir<SpinorB, SpinorF>(E, r, p, f, nr2);
+        E.j++;
+        return rc;
+    }
     qkx::set_hop_skip(&E.cur()->flag);
     int npap = 0;
     int rc = A.apply(Ap, p, g_part, &npap);
     qkx::set_hop_skip(nullptr);
     if (rc) return rc;
-    const int px = x->prec, pr = r->prec, pp = p->prec;
     if (px == QKX_PREC_DOUBLE && pr == QKX_PREC_DOUBLE && pp == QKX_PREC_DOUBLE)
         rc = enqueue_blas<SpinorD, SpinorD, SpinorD>(E, x, r, p, Ap, npap);
     else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_SINGLE)
@@ -407,6 +459,13 @@ int cg_batch_size = 16;
 }
 
 extern "C" {
+
+int qkx_set_cg_fuse(int on)
+{
+    g_cg_fuse = on ? 1 : 0;
+    return QKX_SUCCESS;
+}
+
 
 // Reference CG (lib/solvers.cpp:16-81), FP64, full lattice.  p.Ap is
 // evaluated as |D p|^2, which equals <p, D^dag D p>.  With nstore > 0 the
