@@ -727,19 +727,23 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
         if (dagger) { if (aux) QKX_LAUNCH_TILE(true, true, RT); else QKX_LAUNCH_TILE(true, false, RT); } \
         else        { if (aux) QKX_LAUNCH_TILE(false, true, RT); else QKX_LAUNCH_TILE(false, false, RT); } \
     } while (0)
+    // the fused CG hop always has an aux field; RHS tiles exist for stored
+    // UKQCD-basis batches only (chiral batches use one RHS per thread)
     constexpr bool store = std::is_same_v<Fin, StoreFin>;
-    if (store && rt == 4) {
-        if constexpr (store) QKX_LAUNCH_TILES(4);
-    } else if (store && rt == 3) {
-        if constexpr (store) QKX_LAUNCH_TILES(3);
-    } else if (store && rt == 2) {
-        if constexpr (store) QKX_LAUNCH_TILES(2);
+    constexpr bool tiles = store && B == QKX_BASIS_UKQCD;
+    if (!tiles) rt = 1;
+    if (rt == 4) {
+        if constexpr (tiles) QKX_LAUNCH_TILES(4);
+    } else if (rt == 3) {
+        if constexpr (tiles) QKX_LAUNCH_TILES(3);
+    } else if (rt == 2) {
+        if constexpr (tiles) QKX_LAUNCH_TILES(2);
     } else if (depth == 2) {
-        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else QKX_LAUNCH_HOP(true, false, 2); }
-        else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else QKX_LAUNCH_HOP(false, false, 2); }
+        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 2); }
+        else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 2); }
     } else {
-        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 1); else QKX_LAUNCH_HOP(true, false, 1); }
-        else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else QKX_LAUNCH_HOP(false, false, 1); }
+        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 1); }
+        else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 1); }
     }
 #undef QKX_LAUNCH_HOP
 #undef QKX_LAUNCH_TILE

@@ -118,3 +118,34 @@ def test_half_inner_point_source(q):
     res = rel(orc.tm(x.download(), orc.apply_bc(U, L, -1), L, 0.1632, 0.004, 1), src)
     print(f"fp16 point source: iterations {st.iterations}, residual {res:.2e}", flush=True)
     assert st.converged and res <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["DOUBLE", "HALF"])
+def test_solve_with_separate_update_kernel(q, cfg1, prec):
+    """The unfused CG (separate update kernel and Ap field) reaches the same
+    solution as the default fused one."""
+    U, Ub, src = cfg1
+    u = q.Gauge(L1, boundaryT=-1, host=U)
+    p = q.params(L1, K1, M1, 1, -1)
+    b = q.Spinor(L1, host=src)
+    out = []
+    try:
+        for fuse in (True, False):
+            q.set_cg_fuse(fuse)
+            x = q.Spinor(L1)
+            st = q.solve(x, b, u, p, eo=True, inner_prec=getattr(q, prec), delta=0.05, tol=1e-10)
+            assert st.converged
+            out.append(x.download())
+            res = rel(orc.tm(out[-1], Ub, L1, K1, M1, 1), src)
+            assert res <= 1e-10
+    finally:
+        q.set_cg_fuse(True)
+    assert rel(out[0], out[1]) <= 1e-9
+    x = q.Spinor(L1)
+    try:
+        q.set_cg_fuse(False)
+        bn = q.Spinor(L1, host=orc.tm(src, Ub, L1, K1, M1, 1, orc.UKQCD, orc.TM_DAG))
+        st = q.CG(q.params(L1, K1, M1, 1, -1, tol=1e-10, maxIter=5000), u)(x, bn)
+        assert st.converged
+    finally:
+        q.set_cg_fuse(True)
