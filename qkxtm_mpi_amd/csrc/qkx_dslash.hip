@@ -109,7 +109,10 @@ __device__ __forceinline__ void collect_s(Cx<T> R[12], const Cx<T> xi[6])
 }
 
 // xi = U h (DAGL=false) or U^dag h (DAGL=true), h holds two colour vectors.
-template <bool DAGL, class T>
+// FMA: explicit FMA chains -- fewer instructions but less instruction-level
+// parallelism; chosen for the 16-bit hops, which the ALU bounds (the FP32 and
+// FP64 hops are faster with the independent products).
+template <bool DAGL, bool FMA = false, class T>
 __device__ __forceinline__ void link_mul(const Cx<T> U[9], const Cx<T> h[6], Cx<T> xi[6])
 {
 #pragma unroll
@@ -117,7 +120,15 @@ __device__ __forceinline__ void link_mul(const Cx<T> U[9], const Cx<T> h[6], Cx<
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             Cx<T> acc;
-            if constexpr (!DAGL) {
+            if constexpr (FMA && !DAGL) {
+                acc = cmul_f(U[a * 3 + 0], h[k * 3 + 0]);
+                acc = cmac(acc, U[a * 3 + 1], h[k * 3 + 1]);
+                acc = cmac(acc, U[a * 3 + 2], h[k * 3 + 2]);
+            } else if constexpr (FMA) {
+                acc = cjmul_f(U[0 * 3 + a], h[k * 3 + 0]);
+                acc = cjmac(acc, U[1 * 3 + a], h[k * 3 + 1]);
+                acc = cjmac(acc, U[2 * 3 + a], h[k * 3 + 2]);
+            } else if constexpr (!DAGL) {
                 acc = cmul(U[a * 3 + 0], h[k * 3 + 0]);
                 acc = acc + cmul(U[a * 3 + 1], h[k * 3 + 1]);
                 acc = acc + cmul(U[a * 3 + 2], h[k * 3 + 2]);
@@ -129,6 +140,9 @@ __device__ __forceinline__ void link_mul(const Cx<T> U[9], const Cx<T> h[6], Cx<
             xi[k * 3 + a] = acc;
         }
 }
+
+// 16-bit spinors: the hop is bound by ALU instructions, not memory
+template <class S> constexpr bool kFmaChain = std::is_same_v<S, SpinorH> || std::is_same_v<S, SpinorB>;
 
 // Raw operands of one hop (the neighbour spin pairs the projector reads + link).
 template <class SIn, class G> struct HopOps {
@@ -167,7 +181,7 @@ __device__ __forceinline__ void apply_dir(Cx<T> R[12], const SIn &in, const G &U
         for (int k = 0; k < 6; k++) h[k] = sign * h[k];
     }
     Cx<T> xi[6];
-    link_mul<DAGL>(Ul, h, xi);
+    link_mul<DAGL, kFmaChain<SIn>>(Ul, h, xi);
     collect_s<B, MU, KIND, 0>(R, xi);
     collect_s<B, MU, KIND, 1>(R, xi);
     collect_s<B, MU, KIND, 2>(R, xi);
@@ -531,7 +545,7 @@ __device__ __forceinline__ void tile_dir(Cx<T> (&R)[RT][12], const SIn &in, cons
             for (int k = 0; k < 6; k++) h[k] = sign * h[k];
         }
         Cx<T> xi[6];
-        link_mul<DAGL>(Ul, h, xi);
+        link_mul<DAGL, kFmaChain<SIn>>(Ul, h, xi);
         collect_s<B, MU, KIND, 0>(R[j], xi);
         collect_s<B, MU, KIND, 1>(R[j], xi);
         collect_s<B, MU, KIND, 2>(R[j], xi);

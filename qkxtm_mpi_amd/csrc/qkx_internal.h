@@ -44,6 +44,25 @@ template <class T> __device__ __forceinline__ Cx<T> cjmul(Cx<T> a, Cx<T> b)
 }
 // i * a
 template <class T> __device__ __forceinline__ Cx<T> timesi(Cx<T> a) { return {-a.im, a.re}; }
+// Complex products written as explicit FMA chains (1 mul + 1 fma for the
+// first product of a sum, 2 fma for each further one per real component):
+// a*b, acc + a*b, conj(a)*b, acc + conj(a)*b.
+template <class T> __device__ __forceinline__ Cx<T> cmul_f(Cx<T> a, Cx<T> b)
+{
+    return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+template <class T> __device__ __forceinline__ Cx<T> cmac(Cx<T> acc, Cx<T> a, Cx<T> b)
+{
+    return {fma(-a.im, b.im, fma(a.re, b.re, acc.re)), fma(a.im, b.re, fma(a.re, b.im, acc.im))};
+}
+template <class T> __device__ __forceinline__ Cx<T> cjmul_f(Cx<T> a, Cx<T> b)
+{
+    return {fma(a.re, b.re, a.im * b.im), fma(a.re, b.im, -a.im * b.re)};
+}
+template <class T> __device__ __forceinline__ Cx<T> cjmac(Cx<T> acc, Cx<T> a, Cx<T> b)
+{
+    return {fma(a.im, b.im, fma(a.re, b.re, acc.re)), fma(-a.im, b.re, fma(a.re, b.im, acc.im))};
+}
 
 // ---------------------------------------------------------------------------
 // geometry passed to kernels
@@ -332,12 +351,21 @@ struct SpinorB {
 
 // ---------------------------------------------------------------------------
 // gauge storage accessors: fetch(mu, s) raw words, unpack -> U[row*3+col].
-template <class T> __device__ __forceinline__ void recon_row2(Cx<T> U[9])
+// row2 = conj(row0 x row1)  (SU(3)).  FMA: the products as explicit FMA
+// chains (fewer instructions; used where the ALU, not memory, bounds the hop).
+template <bool FMA = false, class T> __device__ __forceinline__ void recon_row2(Cx<T> U[9])
 {
-    // row2 = conj(row0 x row1)  (SU(3))
-    Cx<T> a = cmul(U[1], U[5]) - cmul(U[2], U[4]);
-    Cx<T> b = cmul(U[2], U[3]) - cmul(U[0], U[5]);
-    Cx<T> c = cmul(U[0], U[4]) - cmul(U[1], U[3]);
+    Cx<T> a, b, c;
+    if constexpr (FMA) {
+        const Cx<T> n2 = {-U[2].re, -U[2].im}, n0 = {-U[0].re, -U[0].im}, n1 = {-U[1].re, -U[1].im};
+        a = cmac(cmul_f(U[1], U[5]), n2, U[4]);
+        b = cmac(cmul_f(U[2], U[3]), n0, U[5]);
+        c = cmac(cmul_f(U[0], U[4]), n1, U[3]);
+    } else {
+        a = cmul(U[1], U[5]) - cmul(U[2], U[4]);
+        b = cmul(U[2], U[3]) - cmul(U[0], U[5]);
+        c = cmul(U[0], U[4]) - cmul(U[1], U[3]);
+    }
     U[6] = {a.re, -a.im};
     U[7] = {b.re, -b.im};
     U[8] = {c.re, -c.im};
@@ -432,7 +460,7 @@ template <bool BF> struct Gauge16 {
         for (int k = 0; k < 4; k++) U[k] = {f[2 * k], f[2 * k + 1]};
         U[4] = {g[0], g[1]};
         U[5] = {g[2], g[3]};
-        recon_row2(U);
+        recon_row2<true>(U);
     }
     __device__ __forceinline__ void load(int mu, int s, Cx<float> U[9]) const { unpack(fetch(mu, s), U); }
 };
