@@ -311,12 +311,12 @@ __device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
 {
     HopSite h;
     const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
-    const int xh = cb % g.Lh0;
-    const int rest = cb / g.Lh0;
-    const int y = rest % L1;
-    const int zt = rest / L1;
-    const int z = zt % L2;
-    const int t = zt / L2;
+    const int rest = fdiv(cb, g.dLh0);
+    const int xh = cb - rest * g.Lh0;
+    const int zt = fdiv(rest, g.dL1);
+    const int y = rest - zt * L1;
+    const int t = fdiv(zt, g.dL2);
+    const int z = zt - t * L2;
     const int x = 2 * xh + ((p + y + z + t) & 1);
     const int xp = (x + 1 == L0) ? 0 : x + 1;
     const int xm = (x == 0) ? L0 - 1 : x - 1;

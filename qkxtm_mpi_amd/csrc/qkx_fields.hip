@@ -221,6 +221,9 @@ Geom make_geom(const int L[4], int bc)
     g.V = L[0] * L[1] * L[2] * L[3];
     g.Vh = g.V / 2;
     g.bc = bc;
+    g.dLh0 = make_fastdiv(g.Lh0);
+    g.dL1 = make_fastdiv(L[1]);
+    g.dL2 = make_fastdiv(L[2]);
     return g;
 }
 

@@ -66,11 +66,35 @@ template <class T> __device__ __forceinline__ Cx<T> cjmac(Cx<T> acc, Cx<T> a, Cx
 
 // ---------------------------------------------------------------------------
 // geometry passed to kernels
+// Division of an index n < 2^30 by a fixed d as a multiply and a shift:
+// n / d = (n * m) >> s with s = 30 + ceil(log2 d), m = floor(2^s / d) + 1
+// (Granlund and Montgomery, PLDI 1994: exact because m d - 2^s <= 2^(s-30)).
+struct FastDiv {
+    uint32_t m;
+    uint32_t s;
+    int d;
+};
+__host__ __device__ __forceinline__ int fdiv(int n, const FastDiv &f)
+{
+    return (int)(((uint64_t)(uint32_t)n * f.m) >> f.s);
+}
+inline FastDiv make_fastdiv(int d)
+{
+    int l = 0;
+    while ((1 << l) < d) l++;
+    FastDiv f;
+    f.s = 30u + (uint32_t)l;
+    f.m = (uint32_t)((1ull << f.s) / (uint64_t)d + 1ull);
+    f.d = d;
+    return f;
+}
+
 struct Geom {
     int L[4];
     int Lh0;     // L[0]/2
     int V, Vh;
     int bc;      // sign applied to t-links of the last time slice (+1/-1)
+    FastDiv dLh0, dL1, dL2;   // divisions by Lh0, L[1], L[2]
 };
 
 // ---------------------------------------------------------------------------
