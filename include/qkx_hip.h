@@ -144,6 +144,9 @@ int qkx_gauge_destroy(qkx_gauge *g);
 int qkx_gauge_upload(qkx_gauge *g, const double *U_aos);
 /* Device -> device precision/compression change (for mixed-precision solvers). */
 int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src);
+/* Device -> host AoS (reference layout), FP64 recon-18 fields; the links as
+ * stored (the boundaryT = -1 sign is applied by the kernels, not stored). */
+int qkx_gauge_download(const qkx_gauge *g, double *U_aos);
 
 /* Vector::Vector lib/lattice_util.cpp:382-413 */
 int qkx_spinor_create(qkx_spinor **s, const int L[4], int prec, int subset);
@@ -318,6 +321,32 @@ int qkx_solve_gcr(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
 /* Relative true residual |b - D x| / |b| in FP64, full lattice, UKQCD. */
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,
                       const qkx_gauge *U, const qkx_params *p);
+
+/* ---- source preparation (include/lattice_util.h:209-210, 131, 325) ---------
+ * FP64 like the reference; gauge fields FP64 recon 18.  A write to a gauge
+ * field here (or by upload/convert) refreshes the low-precision copies the
+ * mixed solvers cache from it. */
+/* Gauge::APE_smearing lib/lattice_util.cpp:950-1078: the spatial links of out
+ * become nsmear APE steps of in's, U_i -> Proj_SU3[U_i + alpha * (sum of the
+ * four spatial staples)]; the temporal links of out become in's.  nsmear = 0
+ * leaves out untouched, as the reference does.  out != in, same lattice and
+ * boundaryT; in is not modified (the reference uses its U_tmp as scratch). */
+int qkx_ape_smear(qkx_gauge *out, const qkx_gauge *in, int nsmear, double alpha);
+/* qcd_projectSU33d lib/blas.cpp:6-422: the spatial links of g, in place. */
+int qkx_gauge_project_su3(qkx_gauge *g);
+/* Gauge::calculatePlaquette lib/lattice_util.cpp:923-946: mean of
+ * Re tr(plaquette)/3 over sites and the six planes. */
+int qkx_plaquette(double *plaq, const qkx_gauge *g);
+/* Vector::Gaussian_smearing lib/lattice_util.cpp:1080-1146:
+ *   out <- nsmear steps of psi -> (psi + alpha H3 psi) / (1 + 6 alpha),
+ *   H3 psi(x) = sum_{i<3} [U_i(x) psi(x+i) + U_i(x-i)^dag psi(x-i)].
+ * in/out: full FP64 fields (batches smear every member); out != in.
+ * t = -1 smears the whole lattice (the reference); t >= 0 smears time slice t
+ * only and zeroes out elsewhere (the same result when in vanishes off slice
+ * t, as a point source does, at 1/L_t of the cost).  nsmear = 0 leaves out
+ * untouched, as the reference does. */
+int qkx_gauss_smear(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, int nsmear, double alpha,
+                    int t);
 
 /* ---- synthetic inputs (stand-in for the LIME readers, lib/read_conf_lime.cpp) */
 /* Counter-hash RNG keyed by (seed, site, direction): thread-order independent.

@@ -50,6 +50,11 @@ def lib():
             "orc_ptc_wgrad": ([P, P, P, P, IP, I, I], None),
             "orc_net_apply": ([P, P, P, IP, I, P, P], None),
             "orc_net_loss_grad": ([P, P, P, IP, I, P, P, P, P], D),
+            "orc_su3_project": ([P], None),
+            "orc_project_su3_field": ([P, IP], None),
+            "orc_ape_smear": ([P, P, IP, I, D], None),
+            "orc_gauss_smear": ([P, P, P, IP, I, D], None),
+            "orc_plaquette": ([P, IP], D),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -204,3 +209,38 @@ def net_loss_grad(r, e, U, L, W1, W2):
     G1, G2 = np.zeros_like(W1), np.zeros_like(W2)
     loss = lib().orc_net_loss_grad(_p(r), _p(e), _p(U), _L(L), W1.shape[1], _p(W1), _p(W2), _p(G1), _p(G2))
     return loss, G1, G2
+
+
+# ---- source preparation (qkx_oracle_smear.c)
+def su3_project(M):
+    """SU(3) projection of one 3x3 complex matrix (the reference's algorithm)."""
+    m = np.array(M, dtype=np.complex128, copy=True).reshape(3, 3)
+    lib().orc_su3_project(_p(m))
+    return m
+
+
+def project_su3_field(U, L):
+    """Projection of the spatial links of a gauge field [V, 4, 3, 3]."""
+    U = np.array(U, dtype=np.complex128, copy=True)
+    lib().orc_project_su3_field(_p(U), _L(L))
+    return U
+
+
+def ape_smear(U, L, nsmear, alpha):
+    """nsmear APE steps on the spatial links; temporal links unchanged."""
+    U = _c(U)
+    out = U.copy()
+    lib().orc_ape_smear(_p(out), _p(U), _L(L), nsmear, alpha)
+    return out
+
+
+def gauss_smear(psi, U, L, nsmear, alpha):
+    """nsmear Gaussian smearing steps of a spinor [V, 4, 3] with links U."""
+    psi, U = _c(psi), _c(U)
+    out = psi.copy()
+    lib().orc_gauss_smear(_p(out), _p(psi), _p(U), _L(L), nsmear, alpha)
+    return out
+
+
+def plaquette(U, L):
+    return lib().orc_plaquette(_p(_c(U)), _L(L))

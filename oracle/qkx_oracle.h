@@ -83,6 +83,17 @@ void orc_net_apply(double *out, const double *in, const double *U, const int L[4
 double orc_net_loss_grad(const double *r, const double *e, const double *U, const int L[4], int ch,
                          const double *W1, const double *W2, double *G1, double *G2);
 
+/* ---- source preparation (qkx_oracle_smear.c) ----
+ * SU(3) projection of one 3x3 matrix (18 reals, row-major) in place, and of the
+ * spatial links of a field; APE smearing of the spatial links (temporal links
+ * copied from in); Gaussian smearing of a spinor; the plaquette.  nsmear = 0
+ * leaves out untouched, as the reference does. */
+void orc_su3_project(double *m18);
+void orc_project_su3_field(double *U, const int L[4]);
+void orc_ape_smear(double *out, const double *in, const int L[4], int nsmear, double alpha);
+void orc_gauss_smear(double *out, const double *in, const double *U, const int L[4], int nsmear, double alpha);
+double orc_plaquette(const double *U, const int L[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,10 @@
 //        ref_harness cg  <in.bin> <out.bin>
 //        ref_harness time_tm <in.bin> <reps>       (prints seconds per tm())
 //        ref_harness time_cg <in.bin> <iters>      (prints seconds per CG iteration)
+//        ref_harness proj <in.bin> <out.bin>       (qcd_projectSU33d of the gauge)
+//        ref_harness smear <in.bin> <out.bin> <nAPE> <alphaAPE> <nGauss> <alphaGauss>
+//        ref_harness time_ape <in.bin> <iters>     (prints seconds per APE step)
+//        ref_harness time_gauss <in.bin> <iters>   (prints seconds per Gaussian step)
 //
 // in.bin (little endian):
 //   int32 L[4]; double kappa, mu; int32 twistSign, boundaryT; double tol;
@@ -36,6 +40,7 @@ static void put(const char *name, const double *d, long n)
 }
 static void putv(const char *name, Vector &v) { put(name, &(v.M[0][0][0].real()), v.lV4d * 24); }
 static void puts1(const char *name, double x) { put(name, &x, 1); }
+static void putg(const char *name, Gauge &g) { put(name, &(g.M[0][0][0][0].real()), g.lV4d * 72); }
 
 int main(int argc, char **argv)
 {
@@ -78,6 +83,32 @@ int main(int argc, char **argv)
     // (bin/test_invert_prop.cpp:72, lib/lattice_util.cpp:708-724).
     u->applyBoundaryCondition(&info);
 
+    if (!strcmp(mode, "time_ape") || !strcmp(mode, "time_gauss")) {
+        // Smearing cost per step on this lattice (Gauge::APE_smearing,
+        // Vector::Gaussian_smearing), alpha as in bin/test_gauss_smearing.cpp:32-34.
+        int reps = atoi(argv[3]);
+        Gauge *tmpg = new Gauge(&info);
+        tmpg->copy(*u);
+        double t0 = MPI_Wtime();
+        if (!strcmp(mode, "time_ape")) {
+            info.NsmearAPE = reps;
+            info.alphaAPE = 0.5;
+            Gauge *ape = new Gauge(&info);
+            ape->APE_smearing(*tmpg, &info);
+            delete ape;
+        } else {
+            info.NsmearGauss = reps;
+            info.alphaGauss = 4.0;
+            tmp->copy(*psi);
+            out->Gaussian_smearing(*tmp, *u, &info);
+        }
+        double t1 = MPI_Wtime();
+        fprintf(stderr, "%.9e\n", (t1 - t0) / reps);
+        delete tmpg;
+        MPI_Finalize();
+        return 0;
+    }
+
     if (!strcmp(mode, "time_tm") || !strcmp(mode, "time_cg")) {
         int reps = atoi(argv[3]);
         if (!strcmp(mode, "time_tm")) {
@@ -116,6 +147,46 @@ int main(int argc, char **argv)
         tmp->tm(*out, *u, &info);
         putv("cg_Dx", *tmp);
         fclose(g_out);
+        MPI_Finalize();
+        return 0;
+    }
+
+    if (!strcmp(mode, "proj")) {
+        // qcd_projectSU33d (lib/blas.cpp:6) on the spatial links of the input
+        puts1("plaq_in", u->calculatePlaquette());
+        qcd_projectSU33d(*u);
+        putg("proj", *u);
+        fclose(g_out);
+        MPI_Finalize();
+        return 0;
+    }
+
+    if (!strcmp(mode, "smear")) {
+        // bin/test_gauss_smearing.cpp:64-85: APE-smear a copy of the gauge, then
+        // Gaussian-smear psi with the APE gauge; also psi smeared with the input
+        // gauge itself.
+        if (argc < 8) { fprintf(stderr, "smear needs nAPE alphaAPE nGauss alphaGauss\n"); return 2; }
+        info.NsmearAPE = atoi(argv[4]);
+        info.alphaAPE = atof(argv[5]);
+        info.NsmearGauss = atoi(argv[6]);
+        info.alphaGauss = atof(argv[7]);
+        puts1("plaq_in", u->calculatePlaquette());
+        Gauge *gtmp = new Gauge(&info);
+        Gauge *ape = new Gauge(&info);
+        gtmp->copy(*u);
+        ape->copy(*u);
+        ape->APE_smearing(*gtmp, &info);
+        delete gtmp;
+        putg("ape", *ape);
+        puts1("plaq_ape", ape->calculatePlaquette());
+        tmp->copy(*psi);
+        out->Gaussian_smearing(*tmp, *ape, &info);
+        putv("gauss_ape", *out);
+        tmp->copy(*psi);
+        out->Gaussian_smearing(*tmp, *u, &info);
+        putv("gauss", *out);
+        fclose(g_out);
+        delete ape;
         MPI_Finalize();
         return 0;
     }

@@ -78,7 +78,9 @@ SIGNATURES = {
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
     "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_cg_fuse": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
-    "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P],
+    "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
+    "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
+    "qkx_gauss_smear": [_P, _P, _P, _I, _D, _I],
     "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
     "qkx_spinor_create_batch": [ctypes.POINTER(_P), _IP, _I, _I, _I],
     "qkx_spinor_upload_rhs": [_P, _I, _P], "qkx_spinor_download_rhs": [_P, _I, _P],
@@ -227,6 +229,27 @@ class Gauge:
     def convert_from(self, src: "Gauge"):
         _check(lib().qkx_gauge_convert(self.h, src.h), "qkx_gauge_convert")
 
+    def download(self):
+        """FP64 recon-18 links as stored, complex128 [V, 4, 3, 3]."""
+        U = np.empty((volume(self.L), 4, 3, 3), dtype=np.complex128)
+        _check(lib().qkx_gauge_download(self.h, _ptr(U)), "qkx_gauge_download")
+        return U
+
+    # ---- source preparation, named after the reference's Gauge methods ------
+    def APE_smearing(self, U_tmp: "Gauge", nsmear, alpha):
+        """Gauge::APE_smearing (lib/lattice_util.cpp:950): this field's spatial
+        links become nsmear APE steps of U_tmp's, its temporal links U_tmp's."""
+        _check(lib().qkx_ape_smear(self.h, U_tmp.h, int(nsmear), float(alpha)), "qkx_ape_smear")
+
+    def projectSU3(self):
+        """qcd_projectSU33d (lib/blas.cpp:6) on the spatial links, in place."""
+        _check(lib().qkx_gauge_project_su3(self.h), "qkx_gauge_project_su3")
+
+    def calculatePlaquette(self):
+        v = ctypes.c_double()
+        _check(lib().qkx_plaquette(ctypes.byref(v), self.h), "qkx_plaquette")
+        return v.value
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.qkx_gauge_destroy(self.h)
@@ -287,6 +310,13 @@ class Spinor:
         v = ctypes.c_double()
         _check(lib().qkx_norm2(ctypes.byref(v), self.h), "qkx_norm2")
         return v.value
+
+    def Gaussian_smearing(self, vec_tmp: "Spinor", gauge: Gauge, nsmear, alpha, t=-1):
+        """Vector::Gaussian_smearing (lib/lattice_util.cpp:1080): this field
+        becomes nsmear smearing steps of vec_tmp; t >= 0 smears only that time
+        slice and zeroes the rest."""
+        _check(lib().qkx_gauss_smear(self.h, vec_tmp.h, gauge.h, int(nsmear), float(alpha), int(t)),
+               "qkx_gauss_smear")
 
     # ---- operators, named after the reference's Vector methods -------------
     def applyDslash(self, psi, u, basis=UKQCD):

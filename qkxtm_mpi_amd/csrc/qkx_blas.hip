@@ -150,6 +150,13 @@ int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result)
     });
 }
 
+int finish_sum_dev(const double *partial, int n, double *result)
+{
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, g_stream, partial, n, result);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
 int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode)
 {
     int rc = check_pair(x, y, "axpby");

@@ -47,11 +47,17 @@ int schur(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, doubl
 
 // BLAS (qkx_blas.hip): device-resident results, no host synchronisation
 int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);
+// *result = sum of partial[0..n) in a fixed order (double-double), one launch
+int finish_sum_dev(const double *partial, int n, double *result);
 // x = a x + b y; sc_mode 1: b = sc[0], 2: a = sc[0], 3: b = -sc[0]
 int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode);
 
 // fields (qkx_fields.hip)
 int zero_parity(qkx_spinor_s *f, int parity);
+// the links of g changed: cached low-precision copies made from g are stale
+void gauge_modified(qkx_gauge_s *g);
+// *cache holds a conversion of U at precision prec / recon, current with U's links
+int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon);
 
 // core (qkx_core.cpp)
 double *result_slot(int k);

@@ -168,6 +168,17 @@ __global__ void gauge_from_d18(void *dst, const double2 *src, int V)
     }
 }
 
+// FP64 recon-18 device links -> reference layout U[lex][mu][row][col]
+__global__ void gauge_to_aos_d18(double2 *dst, const double2 *src, Geom g)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.V) return;
+    int lex = blocked_to_lex(s, g);
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+        for (int k = 0; k < 9; k++) dst[((size_t)lex * 4 + mu) * 9 + k] = src[(size_t)(mu * 9 + k) * g.V + s];
+}
+
 // ---- staging buffer ---------------------------------------------------------
 void *g_stage = nullptr;
 size_t g_stage_bytes = 0;
@@ -225,6 +236,23 @@ Geom make_geom(const int L[4], int bc)
     g.dL1 = make_fastdiv(L[1]);
     g.dL2 = make_fastdiv(L[2]);
     return g;
+}
+
+void gauge_modified(qkx_gauge_s *g) { g->gen++; }
+
+int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon)
+{
+    int rc;
+    if (*cache && ((*cache)->prec != prec || (*cache)->recon != recon || (*cache)->bc != U->bc)) {
+        qkx_gauge_destroy(*cache);
+        *cache = nullptr;
+    }
+    if (!*cache) {
+        if ((rc = qkx_gauge_create(cache, U->L, prec, recon, U->bc))) return rc;
+    } else if ((*cache)->src_gen == U->gen) {
+        return QKX_SUCCESS;
+    }
+    return qkx_gauge_convert(*cache, U);   // records U->gen
 }
 
 // zero the sites of one parity of a full field
@@ -406,6 +434,8 @@ int qkx_gauge_create(qkx_gauge **out, const int L[4], int prec, int recon, int b
     g->bc = boundaryT;
     g->inner = nullptr;
     g->mid = nullptr;
+    g->gen = 0;
+    g->src_gen = 0;
     size_t per = (prec == QKX_PREC_DOUBLE ? 16 : prec == QKX_PREC_SINGLE ? 8 : 4) * (recon == 18 ? 9 : 6);
     g->bytes = per * 4 * (size_t)g->V;
     if (hipMalloc(&g->data, g->bytes) != hipSuccess) {
@@ -446,6 +476,7 @@ int qkx_gauge_upload(qkx_gauge *g, const double *U)
     else hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
     QKX_HIP(hipGetLastError());
     QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    qkx::gauge_modified(g);
     return QKX_SUCCESS;
 }
 
@@ -465,6 +496,25 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src)
     else if (dst->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
     else hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
     QKX_HIP(hipGetLastError());
+    qkx::gauge_modified(dst);
+    dst->src_gen = src->gen;
+    return QKX_SUCCESS;
+}
+
+int qkx_gauge_download(const qkx_gauge *g, double *U)
+{
+    QKX_CHECK_ARG(g && U, "qkx_gauge_download: null argument");
+    QKX_CHECK_ARG(g->prec == QKX_PREC_DOUBLE && g->recon == QKX_RECON_18,
+                  "qkx_gauge_download: only FP64 recon-18 fields are downloaded");
+    size_t bytes = sizeof(double) * 72 * (size_t)g->V;
+    void *st = stage(bytes);
+    if (!st) { qkx::set_error("qkx_gauge_download: staging allocation failed"); return QKX_ERR_NOMEM; }
+    Geom geo = qkx::make_geom(g->L, g->bc);
+    hipLaunchKernelGGL(gauge_to_aos_d18, dim3((g->V + 255) / 256), dim3(256), 0, qkx::g_stream, (double2 *)st,
+                       (const double2 *)g->data, geo);
+    QKX_HIP(hipGetLastError());
+    QKX_HIP(hipMemcpyAsync(U, st, bytes, hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
     return QKX_SUCCESS;
 }
 

@@ -517,6 +517,11 @@ struct qkx_gauge_s {
     qkx_gauge_s *inner;
     // cached FP32 12-real copy for the reliable updates of the mixed solvers (owned)
     qkx_gauge_s *mid;
+    // generation of the links: bumped by every write (upload, conversion into
+    // this field, smearing, projection); a cached copy records the generation
+    // of the field it was converted from and is refreshed when they differ
+    unsigned gen;
+    unsigned src_gen;
 };
 
 // library state

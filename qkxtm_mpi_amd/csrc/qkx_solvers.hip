@@ -589,12 +589,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
             qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
             int want_recon = o->inner_recon == QKX_RECON_18 ? QKX_RECON_18 : QKX_RECON_12;
             if (lp == QKX_PREC_HALF || lp == QKX_PREC_BF16) want_recon = QKX_RECON_12;
-            if (!Uc->inner || Uc->inner->prec != lp || Uc->inner->recon != want_recon) {
-                if (Uc->inner) qkx_gauge_destroy(Uc->inner);
-                Uc->inner = nullptr;
-                if ((rc = qkx_gauge_create(&Uc->inner, L, lp, want_recon, U->bc))) return rc;
-                if ((rc = qkx_gauge_convert(Uc->inner, U))) return rc;
-            }
+            if ((rc = qkx::gauge_cached(&Uc->inner, U, lp, want_recon))) return rc;
             Ui = Uc->inner;
         }
     }
@@ -614,10 +609,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     const qkx_gauge_s *Um = U;
     if (mixed) {
         qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
-        if (!Uc->mid) {
-            if ((rc = qkx_gauge_create(&Uc->mid, L, QKX_PREC_SINGLE, QKX_RECON_12, U->bc))) return rc;
-            if ((rc = qkx_gauge_convert(Uc->mid, U))) return rc;
-        }
+        if ((rc = qkx::gauge_cached(&Uc->mid, U, QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
         Um = Uc->mid;
     }
     std::unique_ptr<qkx::Temp> mo, me, mf, sm, rm;   // FP32 temporaries of a mixed solve

@@ -13,7 +13,7 @@ Inputs are not stored: they are regenerated from (L, eps, seeds) by the
 library's counter-hash generator; a hash of them is kept to detect drift.
 
 usage (in the build container, after __graft_entry__.build()):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [ops] [smear]     (default: all)
 """
 import os
 import struct
@@ -27,7 +27,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 import qkxtm_mpi_amd as q  # noqa: E402  (generator only, host code)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from golden_util import input_hash, probes, summarize  # noqa: E402
+from golden_util import (input_hash, probes, summarize, gauge_probes, summarize_gauge,  # noqa: E402
+                         generic_matrices)
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -133,13 +134,81 @@ def run_cg_case(tmpdir):
     print(f"cg_cfg1.npz: reference CG true residual {res:.3e}")
 
 
+# smearing: (tag, L, eps, boundaryT, gauge seed, psi seed, nAPE, alphaAPE, nGauss, alphaGauss)
+# with the reference driver's alphas (bin/test_gauss_smearing.cpp:31-34) and
+# other ones, odd and even step counts
+SMEAR_CASES = [
+    ("s4648", (4, 6, 4, 8), 0.4, -1, 41, 42, 3, 0.5, 5, 4.0),
+    ("s6448", (6, 4, 4, 8), 0.3, +1, 51, 52, 4, 0.7, 2, 2.5),
+]
+# projection of generic (non-unitary) matrices: (tag, L, seed)
+PROJ_CASE = ("p4444", (4, 4, 4, 4), 61)
+
+
+def _gauge_summary(data, key, U, R, sites):
+    s = summarize_gauge(U, R, sites)
+    data["gnorm2_" + key] = np.array(s["norm2"])
+    data["gproj_" + key] = s["proj"]
+    data["gsite_" + key] = s["sites"]
+
+
+def run_smear_case(case, tmpdir):
+    tag, L, eps, bt, sg, sp, na, aa, ng, ag = case
+    U = q.gen_gauge(L, eps, sg)
+    psi = q.gen_gaussian(L, sp)
+    fin = os.path.join(tmpdir, "in.bin")
+    fout = os.path.join(tmpdir, "out.bin")
+    write_input(fin, L, 0.1, 0.0, 1, bt, 1e-10, 1, U, psi, np.zeros_like(psi))
+    subprocess.run([HARNESS, "smear", fin, fout, str(na), repr(aa), str(ng), repr(ag)], check=True,
+                   stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    R, sites = probes(L)
+    GR, gsites = gauge_probes(L)
+    data = {"L": np.array(L), "params": np.array([eps, bt, sg, sp, na, aa, ng, ag]),
+            "input_hash": np.array(input_hash(U, psi)), "sites": sites, "gsites": gsites}
+    data["plaq_in"] = recs["plaq_in"]
+    data["plaq_ape"] = recs["plaq_ape"]
+    _gauge_summary(data, "ape", recs["ape"].view(np.complex128), GR, gsites)
+    for name in ("gauss", "gauss_ape"):
+        s = summarize(recs[name].view(np.complex128), R, sites)
+        data["norm2_" + name] = np.array(s["norm2"])
+        data["proj_" + name] = s["proj"]
+        data["site_" + name] = s["sites"]
+    np.savez_compressed(os.path.join(OUT, f"smear_{tag}.npz"), **data)
+    print(f"smear_{tag}.npz: plaquette {recs['plaq_in'][0]:.12f} -> {recs['plaq_ape'][0]:.12f}")
+
+
+def run_proj_case(tmpdir):
+    tag, L, seed = PROJ_CASE
+    M = generic_matrices(L, seed)
+    psi = np.zeros((M.shape[0], 4, 3), np.complex128)
+    fin = os.path.join(tmpdir, "in.bin")
+    fout = os.path.join(tmpdir, "out.bin")
+    write_input(fin, L, 0.1, 0.0, 1, 1, 1e-10, 1, M, psi, psi)
+    subprocess.run([HARNESS, "proj", fin, fout], check=True, stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    GR, gsites = gauge_probes(L)
+    gsites = np.union1d(gsites, [0, 1])
+    data = {"L": np.array(L), "seed": np.array(seed), "input_hash": np.array(input_hash(M)), "gsites": gsites}
+    _gauge_summary(data, "proj", recs["proj"].view(np.complex128), GR, gsites)
+    np.savez_compressed(os.path.join(OUT, f"proj_{tag}.npz"), **data)
+    print(f"proj_{tag}.npz")
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: make -C oracle/ref (needs the reference sources)")
+    only = set(sys.argv[1:])   # e.g. "smear": only the smearing fixtures
     with tempfile.TemporaryDirectory() as d:
-        for c in CASES:
-            run_ops_case(c, d)
-        run_cg_case(d)
+        if not only or "ops" in only:
+            for c in CASES:
+                run_ops_case(c, d)
+            run_cg_case(d)
+        if only and "smear" not in only:
+            return
+        for c in SMEAR_CASES:
+            run_smear_case(c, d)
+        run_proj_case(d)
 
 
 if __name__ == "__main__":
