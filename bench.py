@@ -45,12 +45,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="single", choices=["single", "learned", "propagator", "production"],
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "learned", "propagator", "production", "smear"],
                     help="single: the 24^3x48 single-RHS solve (default, the headline line); "
                          "learned: the 16^3x32 learned-preconditioner study (trains on the GPU, then compares "
                          "solvers on unseen gauge fields); propagator: 12 point sources at 32^3x64 in one "
                          "batch plus multi-mu batches at 16^3x32 and 24^3x48; production: the 12-source "
-                         "propagator at 48^3x96")
+                         "propagator at 48^3x96; smear: source preparation at 48^3x96 (20 APE steps, 50 "
+                         "Gaussian steps on 12 point sources and on 12 sink fields)")
     ap.add_argument("--chunk", type=int, default=12, help="right-hand sides per batch (propagator workloads)")
     ap.add_argument("--inner", default="fp16", choices=["fp16", "bf16", "fp32"],
                     help="inner precision of the batched propagator solves")
@@ -319,6 +321,150 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
     print(json.dumps(out))
 
 
+SMEAR_L = (48, 48, 48, 96)
+N_APE, A_APE, N_GAUSS, A_GAUSS = 20, 0.5, 50, 4.0   # bin/test_gauss_smearing.cpp:31-34
+
+
+def smear_cpu_baseline(q, per_site_steps):
+    """Reference CPU smearing on this host: seconds per site per step of
+    Gauge::APE_smearing and Vector::Gaussian_smearing (oracle/_ref/ref_harness,
+    the reference's own lib/lattice_util.cpp, one MPI rank) on a bounded
+    sample at 24^3x48, scaled to the workload's site-steps."""
+    import struct
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    Ls = (24, 24, 24, 48)
+    Vs = int(np.prod(Ls))
+    n_ape, n_gauss = 2, 6
+    if not os.path.exists(harness):
+        return None
+    U = q.gen_gauge(Ls, 0.25, 1)
+    psi = q.gen_gaussian(Ls, 2)
+    per = {}
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "in.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", *Ls))
+            f.write(struct.pack("<dd", 0.1, 0.0))
+            f.write(struct.pack("<ii", 1, 1))
+            f.write(struct.pack("<d", 1e-10))
+            f.write(struct.pack("<i", 1))
+            f.write(np.ascontiguousarray(U).tobytes())
+            f.write(np.ascontiguousarray(psi).tobytes())
+            f.write(np.zeros_like(psi).tobytes())
+        for mode, n in (("time_ape", n_ape), ("time_gauss", n_gauss)):
+            r = subprocess.run([harness, mode, fin, str(n)], capture_output=True, text=True, timeout=300,
+                               env=dict(os.environ, OMP_NUM_THREADS="1"))
+            per[mode] = float(r.stderr.strip().splitlines()[-1]) / Vs
+    value = per["time_ape"] * per_site_steps["ape"] + per["time_gauss"] * per_site_steps["gauss"]
+    return {"value": value, "unit": "s", "cores": 1, "kind": "reference",
+            "sample": f"{n_ape} APE steps and {n_gauss} Gaussian steps (one field) of the reference "
+                      f"(lib/lattice_util.cpp:950-1146) at 24^3x48, one MPI rank: "
+                      f"{per['time_ape'] * 1e9:.0f} ns and {per['time_gauss'] * 1e9:.0f} ns per site-step, "
+                      f"scaled to the workload's {per_site_steps['ape']:.3g} APE and {per_site_steps['gauss']:.3g} "
+                      f"Gaussian site-steps (the reference smears every source over the whole lattice)"}
+
+
+def run_smear(q, args):
+    """Source preparation as the reference's drivers do it
+    (bin/test_invert.cpp:77-95, bin/createTwopWtSLProp.cpp): 20 APE steps of
+    the links, 50 Gaussian steps of the 12 spin-colour point sources (on their
+    time slice, where they are non-zero) and 50 Gaussian steps of 12 sink
+    fields (whole lattice; seeded Gaussian fields stand in for the solutions,
+    the cost is the same).  A step is the whole chain."""
+    L = SMEAR_L
+    V = int(np.prod(L))
+    U = q.gen_gauge(L, 0.25, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    del U
+    ape = q.Gauge(L, boundaryT=-1)
+    x0 = q.gen_site(L, 5)
+    src = point_batch(q, L, x0, 0, 12)
+    b = q.Spinor(L, nrhs=12)
+    sol = q.Spinor(L, nrhs=12)
+    for r in range(12):
+        sol.upload_rhs(r, q.gen_gaussian(L, 100 + r))
+    out = q.Spinor(L, nrhs=12)
+    t0 = x0[3]
+    if args.kernels_only:
+        ape.APE_smearing(u, 3, A_APE)
+        b.Gaussian_smearing(src, ape, 3, A_GAUSS, t=t0)
+        out.Gaussian_smearing(sol, ape, 3, A_GAUSS)
+        sol.copy(out)       # calibrates FETCH_SIZE / WRITE_SIZE: 2 x 192 B per site and source
+        q.sync()
+        print("kernels done", flush=True)
+        return
+
+    def chain():
+        ape.APE_smearing(u, N_APE, A_APE)
+        b.Gaussian_smearing(src, ape, N_GAUSS, A_GAUSS, t=t0)
+        out.Gaussian_smearing(sol, ape, N_GAUSS, A_GAUSS)
+
+    for _ in range(args.warmup):
+        chain()
+    q.sync()
+    times = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        chain()
+        q.sync()
+        times.append(time.perf_counter() - ts)
+    med = float(np.median(times))
+
+    # per-launch kernel times: HIP events around n+1 and 1 steps, difference / n
+    def per_launch(fn, n=10, trials=3):
+        res = []
+        for _ in range(trials):
+            t = q.Timer()
+            t.start()
+            fn(1)
+            a = t.stop()
+            t.start()
+            fn(n + 1)
+            bb = t.stop()
+            res.append((bb - a) / n * 1e3)
+        return float(np.median(res))
+
+    us_ape = per_launch(lambda n: ape.APE_smearing(u, n, A_APE))
+    us_gs = per_launch(lambda n: b.Gaussian_smearing(src, ape, n, A_GAUSS, t=t0))
+    us_gf = per_launch(lambda n: out.Gaussian_smearing(sol, ape, n, A_GAUSS))
+    Vt = V // L[3]
+    by_gf = V * (432 + 384 * 12)
+    by_gs = Vt * (432 + 384 * 12)
+    by_ape = V * 864
+    share = {"ape": N_APE * us_ape, "gauss_slice": N_GAUSS * us_gs, "gauss_full": N_GAUSS * us_gf}
+    gbs = by_gf / (us_gf * 1e-6) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "gauss_step_48x48x48x96_nrhs12_traffic.txt")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = float(line.split(":", 1)[1])
+    cpu = None if args.no_cpu else smear_cpu_baseline(
+        q, {"ape": N_APE * V, "gauss": 2 * 12 * N_GAUSS * V})
+    res = {
+        "metric": "source preparation time at 48^3x96: 20 APE steps + 50 Gaussian steps on 12 point sources "
+                  "and on 12 sink fields",
+        "value": med, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": med * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (seeded SU(3) gauge eps=0.25, point sources at seeded site {x0}, seeded Gaussian "
+                f"sink fields)",
+        "config": {"workload": "48^3x96 APE 20 x 0.5, Gaussian 50 x 4.0 on 12 sources (time slice) and 12 sinks "
+                               "(whole lattice)", "lattice": list(L)},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "gauss_step_kernel, whole lattice, 12 right-hand sides",
+                     "avg_launch_us": us_gf, "bytes_per_launch": by_gf,
+                     "timing": "HIP events around 11 and 1 smearing steps, difference / 10, median of 3"},
+        "kernels": {"ape_step_us": us_ape, "ape_step_GBs": by_ape / (us_ape * 1e-6) / 1e9,
+                    "gauss_slice_step_us": us_gs, "gauss_slice_GBs": by_gs / (us_gs * 1e-6) / 1e9,
+                    "gauss_full_step_us": us_gf, "time_share_ms": {k: v / 1e3 for k, v in share.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -334,6 +480,9 @@ def main():
         return
     if args.workload == "production":
         run_propagator(q, args, (48, 48, 48, 96), 0.1632, 0.004, 0.25, [])
+        return
+    if args.workload == "smear":
+        run_smear(q, args)
         return
 
     U = q.gen_gauge(L, EPS, 1)

@@ -322,7 +322,7 @@ int qkx_solve_gcr(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,
                       const qkx_gauge *U, const qkx_params *p);
 
-/* ---- source preparation (include/lattice_util.h:209-210, 131, 325) ---------
+/* ---- source preparation (include/lattice_util.h:101, 209-210, 325) ---------
  * FP64 like the reference; gauge fields FP64 recon 18.  A write to a gauge
  * field here (or by upload/convert) refreshes the low-precision copies the
  * mixed solvers cache from it. */

@@ -90,6 +90,10 @@ class Vector : public LatticeGeometry {
     void tm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
     void tm_xxDagTm_xx_chiral(Vector &psi, Gauge &u, LatticeInfo *latInfo, PARITY parity);
 
+    /* source preparation: include/lattice_util.h:101 (NsmearGauss, alphaGauss
+     * from latInfo; this != &vec_tmp) */
+    void Gaussian_smearing(Vector &vec_tmp, Gauge &gauge, LatticeInfo *latInfo);
+
     /* device mirror (library internal) */
     qkx_spinor_s *dev();
     void upload();
@@ -115,6 +119,10 @@ class Gauge : public LatticeGeometry {
     void copy(Gauge &src);
     double norm2();
     void applyBoundaryCondition(LatticeInfo *latInfo);
+    /* include/lattice_util.h:209-210 (NsmearAPE, alphaAPE from latInfo; tmp is
+     * not modified) */
+    double calculatePlaquette();
+    void APE_smearing(Gauge &tmp, LatticeInfo *latInfo);
 
     /* device mirror, uploaded as needed (see constGauge above) */
     qkx_gauge_s *dev();
@@ -127,5 +135,8 @@ class Gauge : public LatticeGeometry {
 };
 
 } // namespace qkxTM
+
+/* include/lattice_util.h:325: SU(3) projection of the spatial links, in place */
+void qcd_projectSU33d(qkxTM::Gauge &gf);
 
 #endif

@@ -155,6 +155,7 @@ int qkx_finalize(void)
     if (!g_init) return QKX_SUCCESS;
     (void)hipStreamSynchronize(g_stream);
     pool_clear();
+    smear_release();
     for (auto &r : g_prof_recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);

@@ -52,6 +52,9 @@ int finish_sum_dev(const double *partial, int n, double *result);
 // x = a x + b y; sc_mode 1: b = sc[0], 2: a = sc[0], 3: b = -sc[0]
 int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode);
 
+// smearing (qkx_smear.hip): frees the cached APE link workspace
+void smear_release();
+
 // fields (qkx_fields.hip)
 int zero_parity(qkx_spinor_s *f, int parity);
 // the links of g changed: cached low-precision copies made from g are stale

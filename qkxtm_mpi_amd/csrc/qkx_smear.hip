@@ -257,11 +257,87 @@ __device__ __forceinline__ int blocked(const Geom &g, const Site &x)
     const int lex = ((x.c[3] * g.L[2] + x.c[2]) * g.L[1] + x.c[1]) * g.L[0] + x.c[0];
     return ((x.c[0] + x.c[1] + x.c[2] + x.c[3]) & 1) * g.Vh + (lex >> 1);
 }
+// x + d mu (d = +-1) with periodic wrap; written over constant indices so a
+// run-time mu selects instead of indexing a private array in memory
 __device__ __forceinline__ Site step(const Geom &g, Site x, int mu, int d)
 {
-    int v = x.c[mu] + d;
-    x.c[mu] = v < 0 ? v + g.L[mu] : (v >= g.L[mu] ? v - g.L[mu] : v);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+        if (m == mu) {
+            const int v = x.c[m] + d;
+            x.c[m] = v < 0 ? v + g.L[m] : (v >= g.L[m] ? v - g.L[m] : v);
+        }
     return x;
+}
+
+// ---- traversal ---------------------------------------------------------------
+// A launch covers the sites of both parities (time slices t_lo .. t_lo+nt-1)
+// with `ups` threads per site (right-hand sides, or link directions; adjacent
+// lanes).  Sites of a parity go x fastest, then yb rows of y, then z, then
+// the next block of rows, then t: consecutive z of a block of rows are close
+// in this order, so the z-neighbours of the sites in flight are still in L2
+// when they are used (whole z-planes of a 12-source FP64 field are not).
+// Workgroups are remapped so that XCD k (hardware workgroup b runs on XCD
+// b % 8) takes the k-th contiguous slab of both parities and sees the
+// neighbour reuse of that slab in its own L2.
+struct Trav {
+    FastDiv dups, dyb, dnyb;
+    int ups, yb, nyb, t_lo;
+    int nunits;     // units of one parity
+    int nblk8;      // workgroups per XCD and parity when remapped, else 0
+};
+
+Trav make_trav(const int L[4], int ups, int t, dim3 &grid)
+{
+    Trav tr;
+    tr.ups = ups;
+    tr.dups = make_fastdiv(ups);
+    tr.yb = 1;   // rows per block: the largest divisor of L[1] up to 8
+    for (int d = 2; d <= 8; d++)
+        if (L[1] % d == 0) tr.yb = d;
+    tr.nyb = L[1] / tr.yb;
+    tr.dyb = make_fastdiv(tr.yb);
+    tr.dnyb = make_fastdiv(tr.nyb);
+    tr.t_lo = t < 0 ? 0 : t;
+    tr.nunits = L[0] * L[1] * L[2] / 2 * (t < 0 ? L[3] : 1) * ups;
+    int nbh = (tr.nunits + kThreads - 1) / kThreads;
+    tr.nblk8 = 0;
+    grid = dim3(nbh, 2);
+    if (nbh >= 32) {
+        nbh = (nbh + 7) / 8 * 8;
+        tr.nblk8 = nbh / 8;
+        grid = dim3(2 * nbh, 1);
+    }
+    return tr;
+}
+
+// This thread's unit: parity p, site x, its blocked index s and sub-index
+// (0 .. ups-1).  False past the end.
+__device__ __forceinline__ bool trav_unit(const Geom &g, const Trav &tr, int &p, Site &x, int &s, int &sub)
+{
+    int lb;
+    if (tr.nblk8 > 0) {
+        const int b = blockIdx.x, i = b >> 3;
+        p = i & 1;
+        lb = (b & 7) * tr.nblk8 + (i >> 1);
+    } else {
+        p = blockIdx.y;
+        lb = blockIdx.x;
+    }
+    const int u = lb * blockDim.x + threadIdx.x;
+    if (u >= tr.nunits) return false;
+    const int v = fdiv(u, tr.dups);
+    sub = u - v * tr.ups;
+    const int q1 = fdiv(v, g.dLh0), xh = v - q1 * g.Lh0;
+    const int q2 = fdiv(q1, tr.dyb), yi = q1 - q2 * tr.yb;
+    const int q3 = fdiv(q2, g.dL2), z = q2 - q3 * g.L[2];
+    const int tt = fdiv(q3, tr.dnyb), ybk = q3 - tt * tr.nyb;
+    x.c[1] = ybk * tr.yb + yi;
+    x.c[2] = z;
+    x.c[3] = tr.t_lo + tt;
+    x.c[0] = 2 * xh + ((p + x.c[1] + z + x.c[3]) & 1);
+    s = p * g.Vh + xh + g.Lh0 * (x.c[1] + g.L[1] * (z + g.L[2] * x.c[3]));
+    return true;
 }
 
 // ---- APE ---------------------------------------------------------------------
@@ -269,20 +345,20 @@ struct ApeArgs {
     const double2 *in;
     double2 *out;
     Geom g;
+    Trav tr;        // 3 threads per site: mu = 0, 1, 2
     double alpha;
     ProjConst pc;
 };
 
 // out_mu(x) = P[ U_mu(x) + alpha (forward staples over nu != mu in ascending
-// order, then the backward ones) ], mu = blockIdx.y < 3
+// order, then the backward ones) ]; the three links of a site are adjacent
+// lanes, which share most of their staples' links
 __global__ __launch_bounds__(kThreads) void ape_step_kernel(ApeArgs a)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const Geom &g = a.g;
-    if (s >= g.V) return;
-    const int mu = blockIdx.y;
-    const int p = s >= g.Vh ? 1 : 0;
-    const Site x = coords(g, p, s - p * g.Vh);
+    int p, s, mu;
+    Site x;
+    if (!trav_unit(g, a.tr, p, x, s, mu)) return;
     const int V = g.V;
     const int xpm = blocked(g, step(g, x, mu, +1));
     const int nus[2] = {mu == 0 ? 1 : 0, mu == 2 ? 1 : 2};
@@ -366,53 +442,68 @@ struct GaussArgs {
     SpinorD in, out;
     const double2 *U;
     Geom g;
-    FastDiv dnr;    // division by the number of right-hand sides
+    Trav tr;        // nr threads per site
     int nr;
-    int cb_lo;      // sites with cb in [cb_lo, cb_lo + ncb) of parity blockIdx.y
-    int ncb;
     double alpha, norm;
 };
 
+// tot += U psi (forward hop) or U^dag psi (backward hop) on the four spins;
+// the link elements come from get(k)
+template <bool BWD, class GetU>
+__device__ __forceinline__ void gauss_hop(C tot[12], GetU get, const C ps[12])
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        C u0, u1, u2;   // row i of U, or column i of U for U^dag
+        if (BWD) { u0 = get(i); u1 = get(3 + i); u2 = get(6 + i); }
+        else { u0 = get(3 * i); u1 = get(3 * i + 1); u2 = get(3 * i + 2); }
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+            const C h = BWD ? cjmac(cjmac(cjmul_f(u0, ps[3 * sp]), u1, ps[3 * sp + 1]), u2, ps[3 * sp + 2])
+                            : cmac(cmac(cmul_f(u0, ps[3 * sp]), u1, ps[3 * sp + 1]), u2, ps[3 * sp + 2]);
+            tot[3 * sp + i] = tot[3 * sp + i] + h;
+        }
+    }
+}
+
+__device__ __forceinline__ void gauss_finish(const GaussArgs &a, int e, const C tot[12])
+{
+    C v[12];
+    a.in.load(e, v);
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        v[k] = {a.norm * (v[k].re + a.alpha * tot[k].re), a.norm * (v[k].im + a.alpha * tot[k].im)};
+    a.out.store(e, v);
+}
+
 // out(x) = norm (in(x) + alpha sum_{mu<3} [U_mu(x) in(x+mu) + U_mu(x-mu)^dag in(x-mu)])
+// One thread per (site, right-hand side); the right-hand sides of a site are
+// adjacent lanes and read the same links (one memory transaction).  Staging
+// the links in LDS, or ordering the six hops with register fences, measured
+// no faster.
 __global__ __launch_bounds__(kThreads) void gauss_step_kernel(GaussArgs a)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= a.ncb * a.nr) return;
-    const int cbl = fdiv(tid, a.dnr);
-    const int r = tid - cbl * a.nr;
-    const int p = blockIdx.y;
     const Geom &g = a.g;
-    const int cb = a.cb_lo + cbl;
-    const int s = p * g.Vh + cb;
-    const Site x = coords(g, p, cb);
+    int p, s, r;
+    Site x;
+    if (!trav_unit(g, a.tr, p, x, s, r)) return;
     const int V = g.V, nr = a.nr;
     C tot[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) tot[k] = {0.0, 0.0};
 #pragma unroll
-    for (int mu = 0; mu < 3; mu++) {
-        const int sf = blocked(g, step(g, x, mu, +1)), sb = blocked(g, step(g, x, mu, -1));
-        const Mat Uf = ldm(a.U, V, mu, s), Ub = ldm(a.U, V, mu, sb);
-        C pf[12], pb[12];
-        a.in.load(sf * nr + r, pf);
-        a.in.load(sb * nr + r, pb);
-#pragma unroll
-        for (int sp = 0; sp < 4; sp++)
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const C up = cmac(cmac(cmul_f(Uf.a[3 * i], pf[3 * sp]), Uf.a[3 * i + 1], pf[3 * sp + 1]),
-                                  Uf.a[3 * i + 2], pf[3 * sp + 2]);
-                const C ud = cjmac(cjmac(cjmul_f(Ub.a[i], pb[3 * sp]), Ub.a[3 + i], pb[3 * sp + 1]), Ub.a[6 + i],
-                                   pb[3 * sp + 2]);
-                tot[3 * sp + i] = tot[3 * sp + i] + (up + ud);
-            }
+    for (int h = 0; h < 6; h++) {
+        const int mu = h >> 1;
+        const bool bwd = h & 1;
+        const int sn = blocked(g, step(g, x, mu, bwd ? -1 : +1));
+        const Mat U = ldm(a.U, V, mu, bwd ? sn : s);
+        C ps[12];
+        a.in.load(sn * nr + r, ps);
+        auto get = [&](int k) { return U.a[k]; };
+        if (bwd) gauss_hop<true>(tot, get, ps);
+        else gauss_hop<false>(tot, get, ps);
     }
-    C v[12];
-    a.in.load(s * nr + r, v);
-#pragma unroll
-    for (int k = 0; k < 12; k++)
-        v[k] = {a.norm * (v[k].re + a.alpha * tot[k].re), a.norm * (v[k].im + a.alpha * tot[k].im)};
-    a.out.store(s * nr + r, v);
+    gauss_finish(a, s * nr + r, tot);
 }
 
 ProjConst proj_const()
@@ -422,7 +513,40 @@ ProjConst proj_const()
 
 bool fp64_links(const qkx_gauge_s *g) { return g && g->prec == QKX_PREC_DOUBLE && g->recon == QKX_RECON_18; }
 
+// Spatial-link workspace of the APE ping-pong, kept between calls (a smearing
+// run calls this once per configuration; re-allocating 432 B per site each
+// time would cost more than a step).  Grows on demand; work queued on the
+// library stream against the old buffer finishes before it is freed.
+double2 *g_ape_ws = nullptr;
+size_t g_ape_ws_bytes = 0;
+
+double2 *ape_workspace(size_t bytes)
+{
+    if (bytes <= g_ape_ws_bytes) return g_ape_ws;
+    if (g_ape_ws) {
+        (void)hipStreamSynchronize(qkx::g_stream);
+        (void)hipFree(g_ape_ws);
+    }
+    g_ape_ws = nullptr;
+    g_ape_ws_bytes = 0;
+    if (hipMalloc(&g_ape_ws, bytes) != hipSuccess) {
+        g_ape_ws = nullptr;
+        return nullptr;
+    }
+    g_ape_ws_bytes = bytes;
+    return g_ape_ws;
+}
+
 } // namespace
+
+namespace qkx {
+void smear_release()
+{
+    if (g_ape_ws) (void)hipFree(g_ape_ws);
+    g_ape_ws = nullptr;
+    g_ape_ws_bytes = 0;
+}
+} // namespace qkx
 
 extern "C" {
 
@@ -446,14 +570,15 @@ int qkx_ape_smear(qkx_gauge *out, const qkx_gauge *in, int nsmear, double alpha)
     QKX_CHECK_ARG(nsmear >= 0, "qkx_ape_smear: nsmear must be >= 0");
     if (nsmear == 0) return QKX_SUCCESS;   // the reference returns before touching the field
     const int V = in->V;
-    const size_t spatial = (size_t)3 * 9 * V * sizeof(double2);
     double2 *tmp = nullptr;
-    if (nsmear > 1 && hipMalloc(&tmp, spatial) != hipSuccess) {
+    if (nsmear > 1 && !(tmp = ape_workspace((size_t)3 * 9 * V * sizeof(double2)))) {
         qkx::set_error("qkx_ape_smear: out of device memory");
         return QKX_ERR_NOMEM;
     }
     ApeArgs a;
     a.g = qkx::make_geom(in->L, in->bc);
+    dim3 grid;
+    a.tr = make_trav(in->L, 3, -1, grid);
     a.alpha = alpha;
     a.pc = proj_const();
     // ping-pong so that the last step lands in out
@@ -463,21 +588,13 @@ int qkx_ape_smear(qkx_gauge *out, const qkx_gauge *in, int nsmear, double alpha)
         double2 *dst = bufs[(nsmear - 1 - it) & 1];
         a.in = src;
         a.out = dst;
-        hipLaunchKernelGGL(ape_step_kernel, dim3((V + kThreads - 1) / kThreads, 3), dim3(kThreads), 0, qkx::g_stream,
-                           a);
+        hipLaunchKernelGGL(ape_step_kernel, grid, dim3(kThreads), 0, qkx::g_stream, a);
         src = dst;
     }
-    hipError_t e = hipGetLastError();
+    QKX_HIP(hipGetLastError());
     // temporal links: in's
-    if (e == hipSuccess)
-        e = hipMemcpyAsync((double2 *)out->data + (size_t)3 * 9 * V, (const double2 *)in->data + (size_t)3 * 9 * V,
-                           (size_t)9 * V * sizeof(double2), hipMemcpyDeviceToDevice, qkx::g_stream);
-    if (tmp) {
-        hipError_t e2 = hipStreamSynchronize(qkx::g_stream);
-        (void)hipFree(tmp);
-        if (e == hipSuccess) e = e2;
-    }
-    QKX_HIP(e);
+    QKX_HIP(hipMemcpyAsync((double2 *)out->data + (size_t)3 * 9 * V, (const double2 *)in->data + (size_t)3 * 9 * V,
+                           (size_t)9 * V * sizeof(double2), hipMemcpyDeviceToDevice, qkx::g_stream));
     qkx::gauge_modified(out);
     return QKX_SUCCESS;
 }
@@ -514,14 +631,12 @@ int qkx_gauss_smear(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, i
     if (nsmear == 0) return QKX_SUCCESS;   // the reference returns before touching the field
     const int *L = in->L;
     const int nr = in->nrhs;
-    const int Vsh = L[0] * L[1] * L[2] / 2;   // sites of one parity on one time slice
     GaussArgs a;
     a.U = (const double2 *)U->data;
     a.g = qkx::make_geom(L, U->bc);
     a.nr = nr;
-    a.dnr = make_fastdiv(nr);
-    a.cb_lo = t < 0 ? 0 : t * Vsh;
-    a.ncb = t < 0 ? in->Vh : Vsh;
+    dim3 grid;
+    a.tr = make_trav(L, nr, t, grid);
     a.alpha = alpha;
     a.norm = 1.0 / (1 + 6 * alpha);
     int rc;
@@ -535,7 +650,6 @@ int qkx_gauss_smear(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, i
     }
     qkx_spinor_s *bufs[2] = {out, tmp ? tmp->f : nullptr};
     const qkx_spinor_s *src = in;
-    const dim3 grid((a.ncb * nr + kThreads - 1) / kThreads, 2);
     for (int it = 0; it < nsmear; it++) {
         qkx_spinor_s *dst = bufs[(nsmear - 1 - it) & 1];
         a.in = make_spinor<SpinorD>(src);

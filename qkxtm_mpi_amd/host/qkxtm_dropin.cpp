@@ -294,6 +294,47 @@ void Gauge::applyBoundaryCondition(LatticeInfo *latInfo)
     uploaded_ = false;
 }
 
+// Gauge::calculatePlaquette :923-946
+double Gauge::calculatePlaquette()
+{
+    double r = 0;
+    check(qkx_plaquette(&r, dev()), "calculatePlaquette");
+    return r;
+}
+
+// Gauge::APE_smearing :950-1078: this <- NsmearAPE steps of tmp's spatial
+// links, tmp's temporal links; nothing happens for NsmearAPE = 0
+void Gauge::APE_smearing(Gauge &tmp, LatticeInfo *latInfo)
+{
+    if (latInfo->NsmearAPE == 0) return;
+    printfQKXTM("Perform APE smearing\n");
+    qkx_gauge_s *in = tmp.dev();
+    if (!dev_) check(qkx_gauge_create(&dev_, L, QKX_PREC_DOUBLE, QKX_RECON_18, +1), "Gauge");
+    check(qkx_ape_smear(dev_, in, latInfo->NsmearAPE, latInfo->alphaAPE), "APE_smearing");
+    check(qkx_gauge_download(dev_, &(M[0][0][0][0].real())), "APE_smearing download");
+    uploaded_ = true;
+}
+
+// qcd_projectSU33d lib/blas.cpp:6-422
+void qcd_projectSU33d(qkxTM::Gauge &gf)
+{
+    qkx_gauge_s *g = gf.dev();
+    check(qkx_gauge_project_su3(g), "qcd_projectSU33d");
+    check(qkx_gauge_download(g, &(gf.M[0][0][0][0].real())), "qcd_projectSU33d download");
+}
+
+// Vector::Gaussian_smearing :1080-1146 (whole lattice, as the reference);
+// nothing happens for NsmearGauss = 0
+void Vector::Gaussian_smearing(Vector &vec_tmp, Gauge &gauge, LatticeInfo *latInfo)
+{
+    if (latInfo->NsmearGauss == 0) return;
+    printfQKXTM("Perform Gaussian smearing\n");
+    vec_tmp.upload();
+    check(qkx_gauss_smear(dev(), vec_tmp.dev(), gauge.dev(), latInfo->NsmearGauss, latInfo->alphaGauss, -1),
+          "Gaussian_smearing");
+    download();
+}
+
 // ---- CG: lib/solvers.cpp:8-81 ----------------------------------------------
 CG::CG(LatticeInfo *param, qkxTM::Gauge &gauge) : u(gauge), lastIterations(0)
 {

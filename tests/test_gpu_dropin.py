@@ -47,3 +47,29 @@ def test_driver_test_applyTM(q, tmp_path):
     U = q.gen_gauge(L, 0.4, 11)       # boundaryT = +1 in this driver
     ref = orc.tm_xx(np.ascontiguousarray(vin), U, L, 0.156, 0.1, 1, orc.UKQCD, orc.OO, 1)
     assert rel(vout, ref) <= 1e-13
+
+
+def test_driver_test_gauss_smearing(q, tmp_path):
+    """bin/test_gauss_smearing counterpart: plaquette, APE smearing, Gaussian
+    smearing of a point source through qkxTM::Gauge / qkxTM::Vector, against
+    the oracle's chain on the same links."""
+    print("driver test_gauss_smearing", flush=True)
+    dump = tmp_path / "slice.txt"
+    r = subprocess.run([os.path.join(HOST, "test_gauss_smearing"), str(dump)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    plaq = float(re.search(r"plaquette is (\S+)", r.stdout).group(1))
+    plaq_ape = float(re.search(r"plaquette smeared is (\S+)", r.stdout).group(1))
+    norm = float(re.search(r"the norm of vector is (\S+)", r.stdout).group(1))
+    L = (8, 8, 8, 16)
+    U = q.gen_gauge(L, 0.4, 1)
+    A = orc.ape_smear(U, L, 20, 0.5)
+    pt = np.zeros((int(np.prod(L)), 4, 3), complex)
+    pt[0, 0, 0] = 1.0
+    ref = orc.gauss_smear(pt, A, L, 50, 4.0)
+    assert abs(plaq - orc.plaquette(U, L)) <= 1e-14
+    assert abs(plaq_ape - orc.plaquette(A, L)) <= 1e-12
+    assert abs(norm - orc.norm2(ref)) <= 1e-10 * orc.norm2(ref)
+    a = np.loadtxt(dump)
+    got = (a[:, 5] + 1j * a[:, 6]).reshape(-1, 4, 3)
+    assert rel(got, ref[:L[0] * L[1] * L[2]]) <= 1e-10
