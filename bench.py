@@ -263,7 +263,8 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
         t_batch += t
         iters.append(sts[0].iterations)
         worst = max(worst, max(s.true_residual for s in sts))
-        del b, x
+        x_last = x if n == 12 else None
+        del b
     b1 = point_batch(q, L, x0, 0, 1)
     x1 = q.Spinor(L)
     t_single, st1 = timed(q, lambda: q.solve(x1, b1, u, p, eo=True, inner_prec=P, delta=delta, tol=TOL,
@@ -281,6 +282,15 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
     o = q.Spinor(L, P, q.EVEN)
     nlo, bylo = bytes_per_call(q, lambda: o.tm_xxDagTm_xx(e, ul, p, q.EE))
     uslo = loop_time(q, lambda: o.tm_xxDagTm_xx(e, ul, p, q.EE), nlo, calls=20, trials=3, warm=3)
+    # propagator assembly on the 12 columns (rescale(2 kappa), rotation to the
+    # physical basis): streaming kernels, 2 x 12 x 192 B per site each
+    Pa = q.Propagator(L)
+    if x_last is not None:
+        Pa.copy(x_last)
+    bya = int(np.prod(L)) * 12 * 384
+    us_rescale = loop_time(q, lambda: Pa.rescale(1.0), 1, calls=20, trials=3, warm=3)
+    us_rotate = loop_time(q, lambda: Pa.rotateToPhysicalBasePlus(), 1, calls=20, trials=3, warm=3)
+    del Pa
     mu_rows = []
     for Lm in mu_batches:
         Um = q.gen_gauge(Lm, 0.3, 3)
@@ -317,6 +327,9 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
         "dslash": {"fp64_tm_us": us64, "fp64_tm_GBs": by64 / n64 / (us64 * 1e-6) / 1e9,
                    f"{args.inner}_eo_hop_us": uslo, f"{args.inner}_eo_hop_GBs": bylo / nlo / (uslo * 1e-6) / 1e9},
         "multi_mu": mu_rows,
+        "assembly": {"rescale_us": us_rescale, "rotate_us": us_rotate,
+                     "rescale_GBs": bya / (us_rescale * 1e-6) / 1e9, "rotate_GBs": bya / (us_rotate * 1e-6) / 1e9,
+                     "note": "Vector::rescale(2 kappa) and rotateToPhysicalBasePlus on the 12-column propagator"},
     }
     print(json.dumps(out))
 

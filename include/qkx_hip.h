@@ -348,6 +348,25 @@ int qkx_plaquette(double *plaq, const qkx_gauge *g);
 int qkx_gauss_smear(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, int nsmear, double alpha,
                     int t);
 
+/* ---- propagators (Propagator, include/lattice_util.h:158-185) ---------------
+ * A propagator is a batch field of 12 members (qkx_spinor_create_batch, FP64,
+ * full lattice): member 3 nu + c2 is the column of source spin nu and source
+ * colour c2, i.e. Propagator::M[iv][mu][nu][c1][c2] = member(3 nu + c2)[iv][mu][c1].
+ * The batched solver's solution of the 12 spin-colour point sources (member
+ * 3 nu + c2 = point source (nu, c2)) is such a field.  Propagator::zero, copy
+ * and norm2 are qkx_spinor_zero, qkx_spinor_copy and qkx_norm2 on it. */
+/* Host Complex[V][4][4][3][3] (reference layout) <-> device. */
+int qkx_prop_upload(qkx_spinor *prop, const double *M_aos);
+int qkx_prop_download(const qkx_spinor *prop, double *M_aos);
+/* Propagator::absorbVector lib/lattice_util.cpp:2073-2082 (= copyVector
+ * :574-581): column (nu, c2) <- phi, a single full FP64 field. */
+int qkx_prop_absorb(qkx_spinor *prop, const qkx_spinor *phi, int nu, int c2);
+/* Propagator::rotateToPhysicalBasePlus / Minus :2323-2453, sign +1 / -1:
+ * S -> (1 + sign i g5) S (1 + sign i g5) / 2 per site and colour pair, UKQCD g5. */
+int qkx_prop_rotate_physical(qkx_spinor *prop, int sign);
+/* Vector::rescale :2314-2321: x <- a x (every member of a batch). */
+int qkx_spinor_rescale(qkx_spinor *x, double a);
+
 /* ---- synthetic inputs (stand-in for the LIME readers, lib/read_conf_lime.cpp) */
 /* Counter-hash RNG keyed by (seed, site, direction): thread-order independent.
  * Gauge: U = exp(i eps H), H Gaussian Hermitian traceless, reunitarised to

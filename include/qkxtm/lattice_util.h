@@ -93,6 +93,8 @@ class Vector : public LatticeGeometry {
     /* source preparation: include/lattice_util.h:101 (NsmearGauss, alphaGauss
      * from latInfo; this != &vec_tmp) */
     void Gaussian_smearing(Vector &vec_tmp, Gauge &gauge, LatticeInfo *latInfo);
+    /* include/lattice_util.h:138 */
+    void rescale(double a);
 
     /* device mirror (library internal) */
     qkx_spinor_s *dev();
@@ -132,6 +134,33 @@ class Gauge : public LatticeGeometry {
     bool uploaded_;
     Gauge(const Gauge &);
     Gauge &operator=(const Gauge &);
+};
+
+/* include/lattice_util.h:158-185.  M (host, reference layout) is
+ * authoritative: zero, copy, copyVector and absorbVector are host copies like
+ * the reference's; norm2 and the rotations run on the GPU. */
+class Propagator : public LatticeGeometry {
+  public:
+    long int size;
+    long int size_total;
+    Complex (*M)[NSPINS][NSPINS][NCOLORS][NCOLORS];
+
+    Propagator(LatticeInfo *latInfo);
+    virtual ~Propagator();
+
+    void zero();
+    void copy(Propagator &src);
+    void copyVector(Vector &v, int nu, int c2);
+    void absorbVector(Vector &phi, int nu, int c2);
+    double norm2();
+    void rotateToPhysicalBasePlus();
+    void rotateToPhysicalBaseMinus();
+
+  private:
+    qkx_spinor_s *dev_;
+    void rotate(int sign);
+    Propagator(const Propagator &);
+    Propagator &operator=(const Propagator &);
 };
 
 } // namespace qkxTM

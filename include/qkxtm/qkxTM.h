@@ -25,6 +25,7 @@ typedef std::complex<double> Complex;
 #define NDF 2
 #define NSPINOR (NSPINS * NCOLORS * NDF)
 #define NLINKS (NDIM * NCOLORS * NCOLORS * NDF)
+#define NPROPS (NSPINS * NSPINS * NCOLORS * NCOLORS * NDF)
 
 #define MS(spin, color) ((spin) * NCOLORS + color)
 #define MG(v, color1, color2) ((v) * NCOLORS * NCOLORS + (color1) * NCOLORS + color2)

@@ -55,6 +55,7 @@ def lib():
             "orc_ape_smear": ([P, P, IP, I, D], None),
             "orc_gauss_smear": ([P, P, P, IP, I, D], None),
             "orc_plaquette": ([P, IP], D),
+            "orc_prop_rotate": ([P, ctypes.c_long, I], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -244,3 +245,10 @@ def gauss_smear(psi, U, L, nsmear, alpha):
 
 def plaquette(U, L):
     return lib().orc_plaquette(_p(_c(U)), _L(L))
+
+
+def prop_rotate(M, sign):
+    """Propagator [V, 4, 4, 3, 3] rotated to the physical basis (sign +1 / -1)."""
+    M = np.array(M, dtype=np.complex128, copy=True)
+    lib().orc_prop_rotate(_p(M), M.shape[0], sign)
+    return M

@@ -93,6 +93,8 @@ void orc_project_su3_field(double *U, const int L[4]);
 void orc_ape_smear(double *out, const double *in, const int L[4], int nsmear, double alpha);
 void orc_gauss_smear(double *out, const double *in, const double *U, const int L[4], int nsmear, double alpha);
 double orc_plaquette(const double *U, const int L[4]);
+/* propagator M[V][4][4][3][3] rotated to the physical basis, sign +1 / -1 */
+void orc_prop_rotate(double *M, long V, int sign);
 
 #ifdef __cplusplus
 }

@@ -2,8 +2,9 @@
  * qkx_oracle_smear.c -- CPU restatement of the reference's source-preparation
  * steps: the SU(3) projection (lib/blas.cpp:6-422), APE smearing of the
  * spatial links (lib/lattice_util.cpp:950-1078), Gaussian smearing of a
- * spinor (lib/lattice_util.cpp:1080-1146) and the plaquette
- * (lib/lattice_util.cpp:923-946).
+ * spinor (lib/lattice_util.cpp:1080-1146), the plaquette
+ * (lib/lattice_util.cpp:923-946) and the rotation of a propagator to the
+ * physical basis (lib/lattice_util.cpp:2323-2453).
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the library's smearing kernels.
  *
@@ -392,4 +393,40 @@ double orc_plaquette(const double *U, const int L[4])
                 acc += P[0][0].re + P[1][1].re + P[2][2].re;
             }
     return acc / (V * 3.0 * 6.0);
+}
+
+/* Propagator::rotateToPhysicalBasePlus/Minus (lattice_util.cpp:2323-2453): per
+ * site and colour pair, S -> 0.5 R S R with R = 1 + sign i g5 (UKQCD g5:
+ * R[a][a] = 1, R[0][2] = R[1][3] = R[2][0] = R[3][1] = sign i), the products
+ * summed over mu' then nu' as the reference's loops do.  M[V][4][4][3][3]. */
+void orc_prop_rotate(double *M, long V, int sign)
+{
+    cplx R[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) R[a][b] = C(a == b ? 1. : 0., 0.);
+    R[0][2] = R[1][3] = R[2][0] = R[3][1] = C(0., (double)sign);
+    for (long iv = 0; iv < V; iv++) {
+        double *m = M + iv * 288;
+        for (int c1 = 0; c1 < 3; c1++)
+            for (int c2 = 0; c2 < 3; c2++) {
+                cplx S[4][4], T[4][4];
+                for (int mu = 0; mu < 4; mu++)
+                    for (int nu = 0; nu < 4; nu++) {
+                        const double *e = m + 2 * (((mu * 4 + nu) * 3 + c1) * 3 + c2);
+                        S[mu][nu] = C(e[0], e[1]);
+                        T[mu][nu] = C(0., 0.);
+                    }
+                for (int mu = 0; mu < 4; mu++)
+                    for (int mp = 0; mp < 4; mp++)
+                        for (int np = 0; np < 4; np++)
+                            for (int nu = 0; nu < 4; nu++)
+                                T[mu][nu] = cadd(T[mu][nu], cmulz(cmulz(R[mu][mp], S[mp][np]), R[np][nu]));
+                for (int mu = 0; mu < 4; mu++)
+                    for (int nu = 0; nu < 4; nu++) {
+                        double *e = m + 2 * (((mu * 4 + nu) * 3 + c1) * 3 + c2);
+                        e[0] = 0.5 * T[mu][nu].re;
+                        e[1] = 0.5 * T[mu][nu].im;
+                    }
+            }
+    }
 }

@@ -9,6 +9,7 @@
 //        ref_harness time_cg <in.bin> <iters>      (prints seconds per CG iteration)
 //        ref_harness proj <in.bin> <out.bin>       (qcd_projectSU33d of the gauge)
 //        ref_harness smear <in.bin> <out.bin> <nAPE> <alphaAPE> <nGauss> <alphaGauss>
+//        ref_harness prop <in.bin> <out.bin>       (Propagator assembly and rotations)
 //        ref_harness time_ape <in.bin> <iters>     (prints seconds per APE step)
 //        ref_harness time_gauss <in.bin> <iters>   (prints seconds per Gaussian step)
 //
@@ -41,6 +42,7 @@ static void put(const char *name, const double *d, long n)
 static void putv(const char *name, Vector &v) { put(name, &(v.M[0][0][0].real()), v.lV4d * 24); }
 static void puts1(const char *name, double x) { put(name, &x, 1); }
 static void putg(const char *name, Gauge &g) { put(name, &(g.M[0][0][0][0].real()), g.lV4d * 72); }
+static void putp(const char *name, Propagator &p) { put(name, &(p.M[0][0][0][0][0].real()), p.lV4d * 288); }
 
 int main(int argc, char **argv)
 {
@@ -157,6 +159,40 @@ int main(int argc, char **argv)
         qcd_projectSU33d(*u);
         putg("proj", *u);
         fclose(g_out);
+        MPI_Finalize();
+        return 0;
+    }
+
+    if (!strcmp(mode, "prop")) {
+        // Propagator::absorbVector of 12 columns built from psi and phi,
+        // column (nu, c2) = (nu + 1) psi + (c2 + 1) i phi, then the rotations
+        // to the physical basis (bin/test_seq_source.cpp:95-97, 143-144) and
+        // Vector::rescale(2 kappa).
+        Propagator *pr = new Propagator(&info);
+        Propagator *pm = new Propagator(&info);
+        pr->zero();
+        for (int nu = 0; nu < 4; nu++)
+            for (int c2 = 0; c2 < 3; c2++) {
+                for (long iv = 0; iv < V; iv++)
+                    for (int mu = 0; mu < 4; mu++)
+                        for (int c1 = 0; c1 < 3; c1++)
+                            tmp->M[iv][mu][c1] = (double)(nu + 1) * psi->M[iv][mu][c1] +
+                                                 Complex(0., (double)(c2 + 1)) * phi->M[iv][mu][c1];
+                pr->absorbVector(*tmp, nu, c2);
+            }
+        putp("prop_in", *pr);
+        puts1("prop_norm2", pr->norm2());
+        pm->copy(*pr);
+        pr->rotateToPhysicalBasePlus();
+        putp("prop_plus", *pr);
+        pm->rotateToPhysicalBaseMinus();
+        putp("prop_minus", *pm);
+        out->copy(*psi);
+        out->rescale(2 * info.kappa);
+        putv("rescale", *out);
+        fclose(g_out);
+        delete pr;
+        delete pm;
         MPI_Finalize();
         return 0;
     }

@@ -81,6 +81,8 @@ SIGNATURES = {
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
     "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
     "qkx_gauss_smear": [_P, _P, _P, _I, _D, _I],
+    "qkx_prop_upload": [_P, _P], "qkx_prop_download": [_P, _P], "qkx_prop_absorb": [_P, _P, _I, _I],
+    "qkx_prop_rotate_physical": [_P, _I], "qkx_spinor_rescale": [_P, _D],
     "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
     "qkx_spinor_create_batch": [ctypes.POINTER(_P), _IP, _I, _I, _I],
     "qkx_spinor_upload_rhs": [_P, _I, _P], "qkx_spinor_download_rhs": [_P, _I, _P],
@@ -311,6 +313,10 @@ class Spinor:
         _check(lib().qkx_norm2(ctypes.byref(v), self.h), "qkx_norm2")
         return v.value
 
+    def rescale(self, a):
+        """Vector::rescale (lib/lattice_util.cpp:2314): every member times a."""
+        _check(lib().qkx_spinor_rescale(self.h, float(a)), "qkx_spinor_rescale")
+
     def Gaussian_smearing(self, vec_tmp: "Spinor", gauge: Gauge, nsmear, alpha, t=-1):
         """Vector::Gaussian_smearing (lib/lattice_util.cpp:1080): this field
         becomes nsmear smearing steps of vec_tmp; t >= 0 smears only that time
@@ -378,6 +384,54 @@ def X_eq_aX_p_Y(x: Spinor, a: float, y: Spinor):
 
 
 # ---- solvers -----------------------------------------------------------------
+class Propagator(Spinor):
+    """Device propagator (reference qkxTM::Propagator, include/lattice_util.h:158-185):
+    a batch of 12 FP64 fields, member 3 nu + c2 = column (source spin nu,
+    source colour c2).  The solution of ``solve_batch`` on the 12 spin-colour
+    point sources (``point_sources``) is one."""
+
+    def __init__(self, L, host=None):
+        super().__init__(L, DOUBLE, FULL, nrhs=12)
+        if host is not None:
+            self.upload_prop(host)
+
+    def upload_prop(self, M):
+        M = np.ascontiguousarray(M, dtype=np.complex128)
+        assert M.size == volume(self.L) * 144
+        _check(lib().qkx_prop_upload(self.h, _ptr(M)), "qkx_prop_upload")
+
+    def download_prop(self):
+        """complex128 [V, 4, 4, 3, 3], the reference's Propagator::M layout."""
+        M = np.empty((volume(self.L), 4, 4, 3, 3), dtype=np.complex128)
+        _check(lib().qkx_prop_download(self.h, _ptr(M)), "qkx_prop_download")
+        return M
+
+    def absorbVector(self, phi: Spinor, nu, c2):
+        _check(lib().qkx_prop_absorb(self.h, phi.h, int(nu), int(c2)), "qkx_prop_absorb")
+
+    copyVector = absorbVector
+
+    def rotateToPhysicalBasePlus(self):
+        _check(lib().qkx_prop_rotate_physical(self.h, 1), "qkx_prop_rotate_physical")
+
+    def rotateToPhysicalBaseMinus(self):
+        _check(lib().qkx_prop_rotate_physical(self.h, -1), "qkx_prop_rotate_physical")
+
+
+def point_sources(L, x0, into=None):
+    """The 12 spin-colour point sources at site x0 as a 12-member batch (member
+    3 spin + colour), the right-hand sides of a propagator."""
+    V = volume(L)
+    iv = ((x0[3] * L[2] + x0[2]) * L[1] + x0[1]) * L[0] + x0[0]
+    b = into if into is not None else Spinor(L, nrhs=12)
+    psi = np.zeros((V, 4, 3), np.complex128)
+    for r in range(12):
+        psi[iv, r // 3, r % 3] = 1.0
+        b.upload_rhs(r, psi)
+        psi[iv, r // 3, r % 3] = 0.0
+    return b
+
+
 class CG:
     """Reference CG (include/solvers.h:3-22, lib/solvers.cpp:8-81): solves
     D^dag D out = in on the full lattice; ``out`` accumulates onto its value."""

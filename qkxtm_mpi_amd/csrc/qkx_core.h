@@ -1,5 +1,6 @@
 // qkx_core.h -- internal entry points shared by the library's translation units.
 #pragma once
+#include <cstddef>
 #include "qkx_hip.h"
 
 struct qkx_spinor_s;
@@ -57,6 +58,8 @@ void smear_release();
 
 // fields (qkx_fields.hip)
 int zero_parity(qkx_spinor_s *f, int parity);
+// device staging buffer for host transfers (grows; reused between calls)
+void *stage_buffer(size_t bytes);
 // the links of g changed: cached low-precision copies made from g are stale
 void gauge_modified(qkx_gauge_s *g);
 // *cache holds a conversion of U at precision prec / recon, current with U's links

@@ -9,21 +9,6 @@
 
 namespace {
 
-// full-lattice parity-blocked index s -> lexicographic index
-__device__ __forceinline__ int blocked_to_lex(int s, const Geom &g)
-{
-    int p = (s >= g.Vh) ? 1 : 0;
-    int cb = s - p * g.Vh;
-    int xh = cb % g.Lh0;
-    int rest = cb / g.Lh0;
-    int y = rest % g.L[1];
-    int zt = rest / g.L[1];
-    int z = zt % g.L[2];
-    int t = zt / g.L[2];
-    int x = 2 * xh + ((p + y + z + t) & 1);
-    return x + g.L[0] * rest;
-}
-
 template <class S>
 __global__ void spinor_from_aos(S dst, const double2 *src, Geom g, int par0, int n, int nrhs, int rhs)
 {
@@ -239,6 +224,8 @@ Geom make_geom(const int L[4], int bc)
 }
 
 void gauge_modified(qkx_gauge_s *g) { g->gen++; }
+
+void *stage_buffer(size_t bytes) { return stage(bytes); }
 
 int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon)
 {

@@ -97,6 +97,21 @@ struct Geom {
     FastDiv dLh0, dL1, dL2;   // divisions by Lh0, L[1], L[2]
 };
 
+// full-lattice parity-blocked index s -> lexicographic index
+__device__ __forceinline__ int blocked_to_lex(int s, const Geom &g)
+{
+    int p = (s >= g.Vh) ? 1 : 0;
+    int cb = s - p * g.Vh;
+    int xh = cb % g.Lh0;
+    int rest = cb / g.Lh0;
+    int y = rest % g.L[1];
+    int zt = rest / g.L[1];
+    int z = zt % g.L[2];
+    int t = zt / g.L[2];
+    int x = 2 * xh + ((p + y + z + t) & 1);
+    return x + g.L[0] * rest;
+}
+
 // ---------------------------------------------------------------------------
 // fp16 / bf16 pack helpers (4 values in a uint2)
 __device__ __forceinline__ void unpack_h4(uint2 w, float f[4])

@@ -335,6 +335,61 @@ void Vector::Gaussian_smearing(Vector &vec_tmp, Gauge &gauge, LatticeInfo *latIn
     download();
 }
 
+// Vector::rescale :2314-2321
+void Vector::rescale(double a)
+{
+    upload();
+    check(qkx_spinor_rescale(dev(), a), "rescale");
+    download();
+}
+
+// ---- Propagator: lib/lattice_util.cpp:528-598, 2073-2082, 2323-2453 ---------
+Propagator::Propagator(LatticeInfo *latInfo) : LatticeGeometry(latInfo), dev_(0)
+{
+    size = lV4d;
+    size_total = lV4d;
+    M = (Complex(*)[NSPINS][NSPINS][NCOLORS][NCOLORS])malloc(NPROPS * size_total * sizeof(double));
+    if (!M) errorQKXTM("Error out of memory");
+}
+
+Propagator::~Propagator()
+{
+    if (dev_) qkx_spinor_destroy(dev_);
+    free(M);
+}
+
+void Propagator::zero() { memset(&(M[0][0][0][0][0].real()), 0, NPROPS * size_total * sizeof(double)); }
+void Propagator::copy(Propagator &src)
+{
+    memcpy(&(M[0][0][0][0][0].real()), &(src.M[0][0][0][0][0].real()), lV4d * NPROPS * sizeof(double));
+}
+void Propagator::copyVector(Vector &v, int nu, int c2)
+{
+    for (long iv = 0; iv < lV4d; iv++)
+        for (int mu = 0; mu < NSPINS; mu++)
+            for (int c1 = 0; c1 < NCOLORS; c1++) M[iv][mu][nu][c1][c2] = v.M[iv][mu][c1];
+}
+void Propagator::absorbVector(Vector &phi, int nu, int c2) { copyVector(phi, nu, c2); }
+
+double Propagator::norm2()
+{
+    if (!dev_) check(qkx_spinor_create_batch(&dev_, L, QKX_PREC_DOUBLE, QKX_SUBSET_FULL, 12), "Propagator");
+    check(qkx_prop_upload(dev_, &(M[0][0][0][0][0].real())), "Propagator upload");
+    double r = 0;
+    check(qkx_norm2(&r, dev_), "Propagator norm2");
+    return r;
+}
+
+void Propagator::rotate(int sign)
+{
+    if (!dev_) check(qkx_spinor_create_batch(&dev_, L, QKX_PREC_DOUBLE, QKX_SUBSET_FULL, 12), "Propagator");
+    check(qkx_prop_upload(dev_, &(M[0][0][0][0][0].real())), "Propagator upload");
+    check(qkx_prop_rotate_physical(dev_, sign), "rotateToPhysicalBase");
+    check(qkx_prop_download(dev_, &(M[0][0][0][0][0].real())), "Propagator download");
+}
+void Propagator::rotateToPhysicalBasePlus() { rotate(+1); }
+void Propagator::rotateToPhysicalBaseMinus() { rotate(-1); }
+
 // ---- CG: lib/solvers.cpp:8-81 ----------------------------------------------
 CG::CG(LatticeInfo *param, qkxTM::Gauge &gauge) : u(gauge), lastIterations(0)
 {

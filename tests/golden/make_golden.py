@@ -13,7 +13,7 @@ Inputs are not stored: they are regenerated from (L, eps, seeds) by the
 library's counter-hash generator; a hash of them is kept to detect drift.
 
 usage (in the build container, after __graft_entry__.build()):
-    python tests/golden/make_golden.py [ops] [smear]     (default: all)
+    python tests/golden/make_golden.py [ops] [smear] [prop]     (default: all)
 """
 import os
 import struct
@@ -195,21 +195,55 @@ def run_proj_case(tmpdir):
     print(f"proj_{tag}.npz")
 
 
+# propagator assembly and rotations: (tag, L, kappa, psi seed, phi seed)
+PROP_CASE = ("r4448", (4, 4, 4, 8), 0.1632, 71, 72)
+
+
+def prop_columns(psi, phi):
+    """The 12 columns the harness builds: column (nu, c2) = (nu + 1) psi + (c2 + 1) i phi,
+    as a propagator [V, 4, 4, 3, 3]."""
+    M = np.zeros(psi.shape[:1] + (4, 4, 3, 3), np.complex128)
+    for nu in range(4):
+        for c2 in range(3):
+            M[:, :, nu, :, c2] = (nu + 1) * psi + (c2 + 1) * 1j * phi
+    return M
+
+
+def run_prop_case(tmpdir):
+    tag, L, kappa, sp, sf = PROP_CASE
+    psi, phi = q.gen_gaussian(L, sp), q.gen_gaussian(L, sf)
+    fin = os.path.join(tmpdir, "in.bin")
+    fout = os.path.join(tmpdir, "out.bin")
+    write_input(fin, L, kappa, 0.0, 1, 1, 1e-10, 1, q.gen_gauge(L, 0.3, 1), psi, phi)
+    subprocess.run([HARNESS, "prop", fin, fout], check=True, stdout=subprocess.DEVNULL)
+    recs = read_records(fout)
+    _, sites = probes(L)
+    data = {"L": np.array(L), "params": np.array([kappa, sp, sf]), "input_hash": np.array(input_hash(psi, phi)),
+            "sites": sites, "prop_norm2": recs["prop_norm2"]}
+    for name in ("prop_in", "prop_plus", "prop_minus"):
+        M = recs[name].view(np.complex128).reshape(-1, 4, 4, 3, 3)
+        data["site_" + name] = M[sites]
+        data["norm2_" + name] = np.array(np.vdot(M.ravel(), M.ravel()).real)
+    data["site_rescale"] = recs["rescale"].view(np.complex128).reshape(-1, 4, 3)[sites]
+    np.savez_compressed(os.path.join(OUT, f"prop_{tag}.npz"), **data)
+    print(f"prop_{tag}.npz")
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: make -C oracle/ref (needs the reference sources)")
-    only = set(sys.argv[1:])   # e.g. "smear": only the smearing fixtures
+    only = set(sys.argv[1:])   # sections: ops, smear, prop (default: all)
     with tempfile.TemporaryDirectory() as d:
         if not only or "ops" in only:
             for c in CASES:
                 run_ops_case(c, d)
             run_cg_case(d)
-        if only and "smear" not in only:
-            return
-        for c in SMEAR_CASES:
-            run_smear_case(c, d)
-        run_proj_case(d)
-
+        if not only or "smear" in only:
+            for c in SMEAR_CASES:
+                run_smear_case(c, d)
+            run_proj_case(d)
+        if not only or "prop" in only:
+            run_prop_case(d)
 
 if __name__ == "__main__":
     main()
