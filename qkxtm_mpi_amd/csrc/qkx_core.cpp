@@ -59,6 +59,8 @@ unsigned int *reduce_counter() { return g_counter; }
 double *reduce_result() { return g_result; }
 double *result_slot(int k) { return g_result + k; }
 
+bool prof_active() { return g_prof; }
+
 void prof_begin()
 {
     if (!g_prof) return;

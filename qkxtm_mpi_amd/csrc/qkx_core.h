@@ -53,6 +53,9 @@ int finish_sum_dev(const double *partial, int n, double *result);
 // x = a x + b y; sc_mode 1: b = sc[0], 2: a = sc[0], 3: b = -sc[0]
 int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode);
 
+// instrumentation on (qkx_core.cpp)
+bool prof_active();
+
 // smearing (qkx_smear.hip): frees the cached APE link workspace
 void smear_release();
 

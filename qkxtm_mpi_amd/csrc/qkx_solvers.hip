@@ -43,6 +43,7 @@ struct CGSlot {
 };
 
 int g_cg_fuse = 1;               // fuse the CG update into the operator's last hop
+int g_cg_graph = 0;              // replay pairs of CG iterations from captured graphs (off: measured slower)
 CGSlot *g_slots = nullptr;       // device, two slots
 CGSlot *g_slots_host = nullptr;  // pinned mirror
 double *g_part = nullptr;        // partials: [0, cap) p.Ap, [cap, 2 cap) |r|^2
@@ -386,6 +387,81 @@ int enqueue_iteration(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_
     return rc;
 }
 
+// Pairs of CG iterations replayed from a captured graph.  After two
+// iterations the slot parity is back where it started, so one executable per
+// starting parity serves a whole solve; a replay costs one launch instead of
+// the ten or so kernel launches of two iterations.  On MI355X the eager
+// launches already run ahead of the GPU inside an iteration batch, so replay
+// saves no GPU time and the capture adds about 0.2 ms per solve: off by
+// default (qkx_set_cg_graph), kept for hosts whose launch cost is higher.  An executable bakes
+// in every kernel argument: it is captured again when one changes (the
+// stopping threshold after an early iterated stop) and freed with the solve.
+// The captured work is kernels only (no memset or copy nodes).  Captures that
+// fail (or instrumentation, whose events cannot be captured) fall back to
+// launching the iterations one by one.
+struct IterGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    double stop2[2] = {0.0, 0.0};
+    bool broken = false;
+    IterGraphs() = default;
+    IterGraphs(const IterGraphs &) = delete;
+    IterGraphs &operator=(const IterGraphs &) = delete;
+    ~IterGraphs()
+    {
+        for (auto &e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
+    // capture two iterations from parity par; false if capture is unavailable
+    bool capture(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap,
+                 int par)
+    {
+        if (exec[par]) {
+            (void)hipGraphExecDestroy(exec[par]);
+            exec[par] = nullptr;
+        }
+        if (hipStreamBeginCapture(qkx::g_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const int j0 = E.j;
+        int rc = enqueue_iteration(E, A, x, r, p, Ap);
+        if (!rc) rc = enqueue_iteration(E, A, x, r, p, Ap);
+        E.j = j0;   // nothing ran yet: the launch of the graph runs the pair
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(qkx::g_stream, &g);
+        bool ok = !rc && ec == hipSuccess && g;
+        if (ok && hipGraphInstantiate(&exec[par], g, nullptr, nullptr, 0) != hipSuccess) {
+            exec[par] = nullptr;
+            ok = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        stop2[par] = E.stop2;
+        return ok;
+    }
+    int run(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap,
+            int n)
+    {
+        int rc;
+        const bool use = g_cg_graph && !qkx::prof_active();
+        while (n > 0) {
+            const int par = E.j & 1;
+            if (use && !broken && n >= 2) {
+                if ((!exec[par] || stop2[par] != E.stop2) && !capture(E, A, x, r, p, Ap, par)) broken = true;
+                if (!broken) {
+                    QKX_HIP(hipGraphLaunch(exec[par], qkx::g_stream));
+                    E.j += 2;
+                    n -= 2;
+                    continue;
+                }
+            }
+            if ((rc = enqueue_iteration(E, A, x, r, p, Ap))) return rc;
+            n--;
+        }
+        return QKX_SUCCESS;
+    }
+};
+
 // Start (or restart after a reliable update) the recurrence from residual r:
 // mode 0 -> p = r, slot = {|r|^2, flag 0, k}; mode 1 -> p = r + beta p with
 // beta = |r|^2 / r2_old of the current slot.
@@ -466,6 +542,12 @@ int qkx_set_cg_fuse(int on)
     return QKX_SUCCESS;
 }
 
+int qkx_set_cg_graph(int on)
+{
+    g_cg_graph = on ? 1 : 0;
+    return QKX_SUCCESS;
+}
+
 
 // Reference CG (lib/solvers.cpp:16-81), FP64, full lattice.  p.Ap is
 // evaluated as |D p|^2, which equals <p, D^dag D p>.  With nstore > 0 the
@@ -499,11 +581,11 @@ static int cg_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const
     const int maxIter = p->maxIter;
     if (h.r2 > E.stop2) {
         int saved = 0;
+        IterGraphs graphs;
         while (true) {
             int batch = std::min(nstore > 0 ? every : qkx::cg_batch_size, maxIter - h.k);
             if (batch <= 0) break;
-            for (int i = 0; i < batch; i++)
-                if ((rc = enqueue_iteration(E, A, x, r.f, pp.f, Ap.f))) return rc;
+            if ((rc = graphs.run(E, A, x, r.f, pp.f, Ap.f, batch))) return rc;
             if ((rc = read_slot(E, h))) return rc;
             if (h.flag == 3) { status = QKX_ERR_BREAKDOWN; break; }
             if (h.flag == 2) break;
@@ -688,6 +770,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     qkx_spinor_s *xiter = mixed ? xl.f : xs.f;
     int k_prev = -1;
     double r2_prev = 0;
+    IterGraphs graphs;
     while (h.k < o->maxIter) {
         // Enqueue about as many iterations as the residual needs to reach the
         // next event (stop or reliable update) at its recent rate of decrease:
@@ -703,8 +786,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         }
         k_prev = h.k;
         r2_prev = h.r2;
-        for (int i = 0; i < batch; i++)
-            if ((rc = enqueue_iteration(E, Al, xiter, r.f, pp.f, Ap.f))) return rc;
+        if ((rc = graphs.run(E, Al, xiter, r.f, pp.f, Ap.f, batch))) return rc;
         if ((rc = read_slot(E, h))) return rc;
         if (h.flag == 0) continue;
         if (h.flag == 3) {

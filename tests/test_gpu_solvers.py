@@ -149,3 +149,33 @@ def test_solve_with_separate_update_kernel(q, cfg1, prec):
         assert st.converged
     finally:
         q.set_cg_fuse(True)
+
+
+def test_graph_replay_identical_to_eager(q):
+    """CG iterations replayed from captured HIP graphs give the same solution,
+    bit for bit, as the same iterations launched one by one (reference CG and
+    the mixed-precision even-odd solver, two solves in a row each)."""
+    L = (8, 8, 8, 16)
+    U = q.gen_gauge(L, 0.4, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    src = q.gen_z2(L, 2)
+    p = q.params(L, 0.156, 0.01, 1, -1)
+    b = q.Spinor(L, host=src)
+    out = {}
+    try:
+        for on in (False, True):
+            q.set_cg_graph(on)
+            for rep in range(2):
+                print(f"graph={on} rep={rep}", flush=True)
+                x = q.Spinor(L)
+                st = q.solve(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.05, tol=1e-10)
+                y = q.Spinor(L)
+                cg = q.CG(p, u)
+                sc = cg(y, b)
+                out[(on, rep)] = (x.download(), st.iterations, st.reliable_updates, y.download(), sc.iterations)
+    finally:
+        q.set_cg_graph(False)
+    ref = out[(False, 0)]
+    for key, v in out.items():
+        assert np.array_equal(v[0], ref[0]) and v[1:3] == ref[1:3], key
+        assert np.array_equal(v[3], ref[3]) and v[4] == ref[4], key
