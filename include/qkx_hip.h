@@ -371,7 +371,24 @@ int qkx_prop_rotate_physical(qkx_spinor *prop, int sign);
 /* Vector::rescale :2314-2321: x <- a x (every member of a batch). */
 int qkx_spinor_rescale(qkx_spinor *x, double a);
 
-/* ---- synthetic inputs (stand-in for the LIME readers, lib/read_conf_lime.cpp) */
+/* ---- ILDG/LIME gauge files (include/lattice_util.h:319-320) -----------------
+ * Our own LIME record parser (the reference uses c-lime).  Host arrays in the
+ * reference layout Complex[V][4][3][3]; the file's big-endian order t,z,y,x,
+ * mu,row,col is that layout byte-swapped. */
+/* qkxTM_MPI_getGaugeLime lib/read_conf_lime.cpp:111-264: the links of the
+ * "ildg-binary-data" record, precision from "ildg-format" (64, as the
+ * reference; 32 also accepted and widened).  The lattice in the XML and the
+ * payload size must match L. */
+int qkx_lime_read_gauge(const char *path, const int L[4], double *U_aos);
+/* qkxTM_writeGaugeLime lib/write_conf_lime.cpp:49-199: the reference's three
+ * records ("ildg-format" with its XML text, "xlf-info" = message,
+ * "ildg-binary-data" in 64-bit), byte for byte. */
+int qkx_lime_write_gauge(const char *path, const int L[4], const double *U_aos, const char *message);
+/* The same into / from a device gauge field (FP64 recon 18 for writing). */
+int qkx_gauge_read_lime(qkx_gauge *g, const char *path);
+int qkx_gauge_write_lime(const qkx_gauge *g, const char *path, const char *message);
+
+/* ---- synthetic inputs (seeded stand-ins for configurations and sources) ---- */
 /* Counter-hash RNG keyed by (seed, site, direction): thread-order independent.
  * Gauge: U = exp(i eps H), H Gaussian Hermitian traceless, reunitarised to
  * SU(3).  Z2: (+-1 +-i)/sqrt2 per component.  All in host reference layout. */

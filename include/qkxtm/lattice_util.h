@@ -126,8 +126,10 @@ class Gauge : public LatticeGeometry {
     double calculatePlaquette();
     void APE_smearing(Gauge &tmp, LatticeInfo *latInfo);
 
-    /* device mirror, uploaded as needed (see constGauge above) */
+    /* device mirror, uploaded as needed (see constGauge above); hostModified()
+     * marks it stale after M was rewritten outside the class (library internal) */
     qkx_gauge_s *dev();
+    void hostModified();
 
   private:
     qkx_gauge_s *dev_;
@@ -167,5 +169,10 @@ class Propagator : public LatticeGeometry {
 
 /* include/lattice_util.h:325: SU(3) projection of the spatial links, in place */
 void qcd_projectSU33d(qkxTM::Gauge &gf);
+
+/* include/lattice_util.h:319-320: ILDG/LIME gauge files (returns 0; errors
+ * abort like errorQKXTM) */
+int qkxTM_MPI_getGaugeLime(char *fname, qkxTM::Gauge &u);
+int qkxTM_writeGaugeLime(char *fname, qkxTM::Gauge &u, char *message);
 
 #endif

@@ -83,6 +83,9 @@ SIGNATURES = {
     "qkx_gauss_smear": [_P, _P, _P, _I, _D, _I],
     "qkx_prop_upload": [_P, _P], "qkx_prop_download": [_P, _P], "qkx_prop_absorb": [_P, _P, _I, _I],
     "qkx_prop_rotate_physical": [_P, _I], "qkx_spinor_rescale": [_P, _D],
+    "qkx_lime_read_gauge": [ctypes.c_char_p, _IP, _P],
+    "qkx_lime_write_gauge": [ctypes.c_char_p, _IP, _P, ctypes.c_char_p],
+    "qkx_gauge_read_lime": [_P, ctypes.c_char_p], "qkx_gauge_write_lime": [_P, ctypes.c_char_p, ctypes.c_char_p],
     "qkx_spinor_create": [ctypes.POINTER(_P), _IP, _I, _I], "qkx_spinor_destroy": [_P],
     "qkx_spinor_create_batch": [ctypes.POINTER(_P), _IP, _I, _I, _I],
     "qkx_spinor_upload_rhs": [_P, _I, _P], "qkx_spinor_download_rhs": [_P, _I, _P],
@@ -236,6 +239,14 @@ class Gauge:
         U = np.empty((volume(self.L), 4, 3, 3), dtype=np.complex128)
         _check(lib().qkx_gauge_download(self.h, _ptr(U)), "qkx_gauge_download")
         return U
+
+    def read_lime(self, path):
+        """Links from an ILDG/LIME file (qkxTM_MPI_getGaugeLime, lib/read_conf_lime.cpp:111)."""
+        _check(lib().qkx_gauge_read_lime(self.h, os.fsencode(path)), "qkx_gauge_read_lime")
+
+    def write_lime(self, path, message=""):
+        """ILDG/LIME file of the links (qkxTM_writeGaugeLime, lib/write_conf_lime.cpp:49)."""
+        _check(lib().qkx_gauge_write_lime(self.h, os.fsencode(path), message.encode()), "qkx_gauge_write_lime")
 
     # ---- source preparation, named after the reference's Gauge methods ------
     def APE_smearing(self, U_tmp: "Gauge", nsmear, alpha):
@@ -416,6 +427,21 @@ class Propagator(Spinor):
 
     def rotateToPhysicalBaseMinus(self):
         _check(lib().qkx_prop_rotate_physical(self.h, -1), "qkx_prop_rotate_physical")
+
+
+def read_lime(path, L):
+    """Host links complex128 [V, 4, 3, 3] from an ILDG/LIME file."""
+    U = np.empty((volume(L), 4, 3, 3), dtype=np.complex128)
+    _check(lib().qkx_lime_read_gauge(os.fsencode(path), _ivec(L), _ptr(U)), "qkx_lime_read_gauge")
+    return U
+
+
+def write_lime(path, L, U, message=""):
+    """ILDG/LIME file (the reference writer's three records) from host links."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    assert U.size == volume(L) * 36
+    _check(lib().qkx_lime_write_gauge(os.fsencode(path), _ivec(L), _ptr(U), message.encode()),
+           "qkx_lime_write_gauge")
 
 
 def point_sources(L, x0, into=None):

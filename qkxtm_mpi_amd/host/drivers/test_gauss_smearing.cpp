@@ -4,7 +4,8 @@
 // point source at site 0, Gaussian smearing with the APE links (NsmearGauss
 // 50, alpha 4), its norm, and the time slice t = 0 of the result written to a
 // file as "x y z mu c re im" (the reference writes the same slice to kale.dat).
-// Synthetic gauge (generator, eps 0.4, 8^3x16) instead of a LIME file.
+// Synthetic gauge (generator, eps 0.4, 8^3x16), passed through an ILDG/LIME
+// file when a second argument names one.
 #include "qkxtm/qkxTM.h"
 #include "qkxtm/lattice_util.h"
 #include "qkx_hip.h"
@@ -34,6 +35,14 @@ int main(int argc, char *argv[])
 
     gauge->zero();
     if (qkx_gen_gauge(&(gauge->M[0][0][0][0].real()), latInfo.L, 0.4, 1)) errorQKXTM("gauge generation failed\n");
+    if (argc > 2) {
+        // through an ILDG/LIME file, as the reference driver reads its
+        // configuration (bin/test_gauss_smearing.cpp:59)
+        char msg[] = "synthetic configuration, eps 0.4, seed 1";
+        qkxTM_writeGaugeLime(argv[2], *gauge, msg);
+        gauge->zero();
+        qkxTM_MPI_getGaugeLime(argv[2], *gauge);
+    }
     gauge_tmp->copy(*gauge);
     double plaq = gauge->calculatePlaquette();
     printfQKXTM("plaquette is %.15f\n", plaq);

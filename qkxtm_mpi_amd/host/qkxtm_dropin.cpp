@@ -264,6 +264,8 @@ qkx_gauge_s *Gauge::dev()
     return dev_;
 }
 
+void Gauge::hostModified() { uploaded_ = false; }
+
 void Gauge::zero()
 {
     memset(&(M[0][0][0][0].real()), 0, NLINKS * size_total * sizeof(double));
@@ -321,6 +323,22 @@ void qcd_projectSU33d(qkxTM::Gauge &gf)
     qkx_gauge_s *g = gf.dev();
     check(qkx_gauge_project_su3(g), "qcd_projectSU33d");
     check(qkx_gauge_download(g, &(gf.M[0][0][0][0].real())), "qcd_projectSU33d download");
+}
+
+// qkxTM_MPI_getGaugeLime lib/read_conf_lime.cpp:111-264 into the host links
+int qkxTM_MPI_getGaugeLime(char *fname, qkxTM::Gauge &u)
+{
+    if (u.initialized == false) errorQKXTM("gauge not initialized\n");
+    check(qkx_lime_read_gauge(fname, u.L, &(u.M[0][0][0][0].real())), "qkxTM_MPI_getGaugeLime");
+    u.hostModified();
+    return 0;
+}
+
+// qkxTM_writeGaugeLime lib/write_conf_lime.cpp:49-199 from the host links
+int qkxTM_writeGaugeLime(char *fname, qkxTM::Gauge &u, char *message)
+{
+    check(qkx_lime_write_gauge(fname, u.L, &(u.M[0][0][0][0].real()), message), "qkxTM_writeGaugeLime");
+    return 0;
 }
 
 // Vector::Gaussian_smearing :1080-1146 (whole lattice, as the reference);

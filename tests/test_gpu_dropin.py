@@ -55,8 +55,9 @@ def test_driver_test_gauss_smearing(q, tmp_path):
     the oracle's chain on the same links."""
     print("driver test_gauss_smearing", flush=True)
     dump = tmp_path / "slice.txt"
-    r = subprocess.run([os.path.join(HOST, "test_gauss_smearing"), str(dump)], capture_output=True, text=True,
-                       timeout=120)
+    lime = tmp_path / "conf.lime"
+    r = subprocess.run([os.path.join(HOST, "test_gauss_smearing"), str(dump), str(lime)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     plaq = float(re.search(r"plaquette is (\S+)", r.stdout).group(1))
     plaq_ape = float(re.search(r"plaquette smeared is (\S+)", r.stdout).group(1))
@@ -73,6 +74,7 @@ def test_driver_test_gauss_smearing(q, tmp_path):
     a = np.loadtxt(dump)
     got = (a[:, 5] + 1j * a[:, 6]).reshape(-1, 4, 3)
     assert rel(got, ref[:L[0] * L[1] * L[2]]) <= 1e-10
+    assert np.array_equal(q.read_lime(lime, L), U)   # the file the driver wrote and read back
 
 
 def test_driver_test_prop(q, tmp_path):
