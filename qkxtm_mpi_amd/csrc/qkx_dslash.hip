@@ -75,6 +75,18 @@ template <int K, class T> __device__ __forceinline__ Cx<T> umul(Cx<T> z)
     else return {T(0), T(0)};
 }
 
+// acc + n K z for the reconstruction constant K: one FMA per real, so a
+// per-site scale n costs nothing on top of the accumulation
+template <int K, class T> __device__ __forceinline__ Cx<T> umul_acc(Cx<T> acc, Cx<T> z, T n)
+{
+    if constexpr (K == KP1) return {fma(n, z.re, acc.re), fma(n, z.im, acc.im)};
+    else if constexpr (K == KM1) return {fma(-n, z.re, acc.re), fma(-n, z.im, acc.im)};
+    else if constexpr (K == KPI) return {fma(-n, z.im, acc.re), fma(n, z.re, acc.im)};
+    else if constexpr (K == KMI) return {fma(n, z.im, acc.re), fma(-n, z.re, acc.im)};
+    else if constexpr (K == K2) return {fma(n + n, z.re, acc.re), fma(n + n, z.im, acc.im)};
+    else return acc;
+}
+
 template <int B, int MU, int KIND> constexpr bool uses_spin_pair(int pair)
 {
     constexpr Proj P = kProj[B][MU][KIND];
@@ -105,6 +117,17 @@ __device__ __forceinline__ void collect_s(Cx<T> R[12], const Cx<T> xi[6])
     if constexpr (P.q[s] != K0) {
 #pragma unroll
         for (int c = 0; c < 3; c++) R[s * 3 + c] = R[s * 3 + c] + umul<P.q[s]>(xi[P.kq[s] * 3 + c]);
+    }
+}
+
+// R_s += n q_s xi[kq_s]: the reconstruction with a scale folded in
+template <int B, int MU, int KIND, int s, class T>
+__device__ __forceinline__ void collect_scaled(Cx<T> R[12], const Cx<T> xi[6], T n)
+{
+    constexpr Proj P = kProj[B][MU][KIND];
+    if constexpr (P.q[s] != K0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) R[s * 3 + c] = umul_acc<P.q[s]>(R[s * 3 + c], xi[P.kq[s] * 3 + c], n);
     }
 }
 
@@ -164,12 +187,148 @@ __device__ __forceinline__ HopOps<SIn, G> fetch_dir(const SIn &in, int s_nb, con
     return o;
 }
 
+// ---- FP16 spinors with FP16 links: packed half arithmetic into dot2 --------
+// v_dot2_f32_f16 (d = a.lo b.lo + a.hi b.hi + c, FP32 accumulation) takes a
+// complex product's real or imaginary part straight from two (re, im) half
+// pairs: re(u h) = dot2(u, (h.re, -h.im)), im(u h) = dot2(u, (h.im, h.re)),
+// re(conj(u) h) = dot2(u, h), im(conj(u) h) = dot2(u, (h.im, -h.re)).  The
+// spinor and the first two link rows are used as stored (no conversions).
+// Each needed view of the projected half spinor is formed in half precision
+// by one packed add with source modifiers (one rounding, the size of the
+// storage rounding); the third link row is reconstructed with dot2 and
+// rounded to half.  Written with the compiler builtin, whose hazard tracking
+// keeps the required wait states between dependent dot instructions (inline
+// assembly would hide them).
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ h2v h2_of(uint32_t w) { return __builtin_bit_cast(h2v, w); }
+__device__ __forceinline__ float dot2(h2v a, h2v b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+__device__ __forceinline__ h2v cj16(h2v z) { return h2v{z.x, -z.y}; }
+__device__ __forceinline__ h2v sw16(h2v z) { return h2v{z.y, z.x}; }
+__device__ __forceinline__ h2v swn16(h2v z) { return h2v{z.y, -z.x}; }
+template <int K> __device__ __forceinline__ h2v kmul16(h2v b)
+{
+    if constexpr (K == KP1) return b;
+    else if constexpr (K == KM1) return h2v{-b.x, -b.y};
+    else if constexpr (K == KPI) return h2v{-b.y, b.x};
+    else if constexpr (K == KMI) return h2v{b.y, -b.x};
+    else return h2v{(_Float16)0, (_Float16)0};
+}
+
+// views of projected component k: (h, (h.im, -h.re)) for a backward hop,
+// ((h.re, -h.im), (h.im, h.re)) for a forward one, h = psi[sa] + b psi[sb]
+// (a = 2 is left to the caller's scale)
+template <int B, int MU, int KIND, bool DAGL, int k>
+__device__ __forceinline__ void views16(const uint32_t w[12], h2v v0[6], h2v v1[6])
+{
+    constexpr Proj P = kProj[B][MU][KIND];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const h2v a = h2_of(w[P.sa[k] * 3 + c]);
+        if constexpr (P.b[k] == K0) {
+            v0[k * 3 + c] = DAGL ? a : cj16(a);
+            v1[k * 3 + c] = DAGL ? swn16(a) : sw16(a);
+        } else {
+            const h2v kb = kmul16<P.b[k]>(h2_of(w[P.sb[k] * 3 + c]));
+            if constexpr (DAGL) {
+                v0[k * 3 + c] = a + kb;
+                v1[k * 3 + c] = swn16(a) + swn16(kb);
+            } else {
+                v0[k * 3 + c] = cj16(a) + cj16(kb);
+                v1[k * 3 + c] = sw16(a) + sw16(kb);
+            }
+        }
+    }
+}
+
+template <int B, int MU, int KIND, bool DAGL, class T>
+__device__ __forceinline__ void apply_dir_h16(Cx<T> R[12], const typename SpinorH::Raw &ps,
+                                              const typename Gauge16<false>::Raw &lk, T sign)
+{
+    constexpr Proj P = kProj[B][MU][KIND];
+    static_assert(P.a[0] == P.a[1] && (P.a[0] == KP1 || P.a[0] == K2), "projector shape");
+    uint32_t w[12];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        w[4 * j + 0] = ps.c[j].x;
+        w[4 * j + 1] = ps.c[j].y;
+        w[4 * j + 2] = ps.c[j].z;
+        w[4 * j + 3] = ps.c[j].w;
+    }
+    h2v v0[6], v1[6];
+    views16<B, MU, KIND, DAGL, 0>(w, v0, v1);
+    views16<B, MU, KIND, DAGL, 1>(w, v0, v1);
+    h2v U[9];
+    U[0] = h2_of(lk.a.x);
+    U[1] = h2_of(lk.a.y);
+    U[2] = h2_of(lk.a.z);
+    U[3] = h2_of(lk.a.w);
+    U[4] = h2_of(lk.b.x);
+    U[5] = h2_of(lk.b.y);
+    // third row conj(x y - z w): views of row 1 and the negated row 0, once
+    {
+        const h2v c3 = cj16(U[3]), c4 = cj16(U[4]), c5 = cj16(U[5]);
+        const h2v s3 = sw16(U[3]), s4 = sw16(U[4]), s5 = sw16(U[5]);
+        const h2v n0 = -U[0], n1 = -U[1], n2 = -U[2];
+        auto cross = [&](h2v x, h2v nx, h2v yc, h2v ys, h2v nz, h2v z, h2v wc, h2v ws) {
+            const float re = dot2(x, yc, dot2(nz, wc, 0.f));
+            const float im = dot2(z, ws, dot2(nx, ys, 0.f));
+            return h2v{(_Float16)re, (_Float16)im};
+        };
+        U[6] = cross(U[1], n1, c5, s5, n2, U[2], c4, s4);
+        U[7] = cross(U[2], n2, c3, s3, n0, U[0], c5, s5);
+        U[8] = cross(U[0], n0, c4, s4, n1, U[1], c3, s3);
+    }
+    Cx<T> xi[6];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            // forward: sum_b U[a][b] h[b]; backward: sum_b conj(U[b][a]) h[b]
+            const h2v u0 = DAGL ? U[a] : U[3 * a], u1 = DAGL ? U[3 + a] : U[3 * a + 1],
+                      u2 = DAGL ? U[6 + a] : U[3 * a + 2];
+            xi[k * 3 + a] = {dot2(u2, v0[k * 3 + 2], dot2(u1, v0[k * 3 + 1], dot2(u0, v0[k * 3], 0.f))),
+                             dot2(u2, v1[k * 3 + 2], dot2(u1, v1[k * 3 + 1], dot2(u0, v1[k * 3], 0.f)))};
+        }
+    T n = ps.n * sign;
+    if constexpr (P.a[0] == K2) n = n + n;
+    collect_scaled<B, MU, KIND, 0>(R, xi, n);
+    collect_scaled<B, MU, KIND, 1>(R, xi, n);
+    collect_scaled<B, MU, KIND, 2>(R, xi, n);
+    collect_scaled<B, MU, KIND, 3>(R, xi, n);
+}
+
+#ifndef QKX_HOP16_DOT2
+#define QKX_HOP16_DOT2 1
+#endif
+
 // Project, multiply by the link (or its adjoint), reconstruct, accumulate.
 template <int B, int MU, int KIND, bool DAGL, class SIn, class G, class T>
 __device__ __forceinline__ void apply_dir(Cx<T> R[12], const SIn &in, const G &U, const HopOps<SIn, G> &o,
                                           T sign)
 {
     Cx<T> psi[12];
+    if constexpr (QKX_HOP16_DOT2 && std::is_same_v<SIn, SpinorH> && std::is_same_v<G, Gauge16<false>>) {
+        apply_dir_h16<B, MU, KIND, DAGL>(R, o.psi, o.link, sign);
+        return;
+    } else if constexpr (std::is_same_v<SIn, SpinorH>) {
+        // FP16 neighbours carry a per-site scale: project and multiply the
+        // unscaled values and fold scale and boundary sign into the final FMAs
+        // (24 multiplies fewer per hop in an ALU-bound kernel)
+        in.template unpack_raw<spin_mask<B, MU, KIND>()>(o.psi, psi);
+        Cx<T> h[6];
+        project_k<B, MU, KIND, 0>(psi, h);
+        project_k<B, MU, KIND, 1>(psi, h);
+        Cx<T> Ul[9];
+        U.unpack(o.link, Ul);
+        Cx<T> xi[6];
+        link_mul<DAGL, kFmaChain<SIn>>(Ul, h, xi);
+        const T n = in.scale(o.psi) * sign;
+        collect_scaled<B, MU, KIND, 0>(R, xi, n);
+        collect_scaled<B, MU, KIND, 1>(R, xi, n);
+        collect_scaled<B, MU, KIND, 2>(R, xi, n);
+        collect_scaled<B, MU, KIND, 3>(R, xi, n);
+        return;
+    }
     in.template unpack<spin_mask<B, MU, KIND>()>(o.psi, psi);
     Cx<T> h[6];
     project_k<B, MU, KIND, 0>(psi, h);

@@ -317,6 +317,22 @@ struct SpinorH {
                 }
             }
     }
+    // the stored values without the site scale, and the scale
+    template <int MASK> __device__ __forceinline__ void unpack_raw(const Raw &r, Cx<float> v[12]) const
+    {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (need_chunk16<MASK>(j)) {
+                float f[8];
+                unpack_h8(r.c[j], f);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int k = 4 * j + i;
+                    if ((MASK >> (k / 6)) & 1) v[k] = {f[2 * i], f[2 * i + 1]};
+                }
+            }
+    }
+    __device__ __forceinline__ float scale(const Raw &r) const { return r.n; }
     __device__ __forceinline__ void load(int s, Cx<float> v[12]) const { unpack<3>(fetch<3>(s), v); }
     __device__ __forceinline__ void store(int s, const Cx<float> v[12]) const
     {
