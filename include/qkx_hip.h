@@ -120,6 +120,12 @@ int  qkx_prof_reset(void);
  * the software-pipeline depth (1 or 2 hops in flight) of the FP64 and of the
  * FP32/FP16/BF16 kernels. */
 int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
+/* Order in which the hopping kernels walk the sites of a parity: 1 (default)
+ * blocked -- x, then blocks of up to 8 y rows and up to 8 t slices, then z --
+ * so neighbour spinors and backward links are reused from L2; 0: checkerboard
+ * index order.  Results per site are identical; reductions sum in another
+ * order. */
+int  qkx_set_site_order(int blocked);
 /* CG iterations enqueued between host reads of the solver state (default 16;
  * 1 gives no launches past the stopping iteration). */
 int  qkx_set_cg_batch(int iterations);

@@ -269,6 +269,13 @@ int qkx_set_cg_batch(int iterations)
     return QKX_SUCCESS;
 }
 
+int qkx_set_site_order(int blocked)
+{
+    QKX_CHECK_ARG(blocked == 0 || blocked == 1, "qkx_set_site_order: 0 or 1");
+    qkx::set_site_order(blocked);
+    return QKX_SUCCESS;
+}
+
 int qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
     QKX_CHECK_ARG(block == 64 || block == 128 || block == 256, "qkx_set_tuning: block must be 64, 128 or 256");

@@ -39,6 +39,7 @@ int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);
+void set_site_order(int blocked);
 // even-odd Schur operator on parity pe (qkx_ops.cpp):
 //   out_pe = [A in - kappa^2 H A^-1 H in]_pe (then g5 if g5out); the optional
 //   norm_partial collects |out|^2 per workgroup of the second hop

@@ -396,6 +396,18 @@ __device__ __forceinline__ void epilogue(const Epi &e, const Cx<T> R[12], const 
     }
 }
 
+// Site order of the hop kernels.  Plain: checkerboard index order (x fastest,
+// then y, z, t).  Blocked: x, then yb rows of y, then tb slices of t, then z,
+// then the next y block, then the next t block, so a site's +-y, +-z and +-t
+// neighbours (and the backward links they carry) are reused after about
+// L0 yb tb sites instead of a whole t slice; yb and tb are chosen on the
+// host so that this reuse window stays well inside one XCD's L2.
+struct HopTrav {
+    int on;        // 0: plain order
+    int yb, tb, nyb;
+    FastDiv dyb, dtb, dnyb;
+};
+
 template <class SIn, class SOut, class SAux, class G>
 struct HopArgs {
     Geom g;
@@ -408,6 +420,7 @@ struct HopArgs {
     int nout;      // output entries (sites x right-hand sides)
     int nrhs;      // right-hand sides; thread unit u -> site u / nrhs, rhs u % nrhs
     int nblk8;     // grid size / 8 when XCD remapping is on, else 0
+    HopTrav tr;    // order in which the sites of a parity are walked
     const int *skip;   // when non-null and *skip != 0 the launch does nothing
     double *norm_partial;  // when non-null: per-workgroup (and per-rhs) sum of |out|^2
     Epi epi[QKX_MAX_RHS];  // epilogue coefficients per right-hand side
@@ -466,21 +479,16 @@ struct HopSite {
     int nb[8];
     bool tw_p, tw_m;
 };
-__device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
+__device__ __forceinline__ HopSite hop_site_at(const Geom &g, int p, int cb, int xh, int y, int z, int t)
 {
     HopSite h;
     const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
-    const int rest = fdiv(cb, g.dLh0);
-    const int xh = cb - rest * g.Lh0;
-    const int zt = fdiv(rest, g.dL1);
-    const int y = rest - zt * L1;
-    const int t = fdiv(zt, g.dL2);
-    const int z = zt - t * L2;
+    const int row = cb - xh;   // Lh0 * (y + L1 (z + L2 t))
     const int x = 2 * xh + ((p + y + z + t) & 1);
     const int xp = (x + 1 == L0) ? 0 : x + 1;
     const int xm = (x == 0) ? L0 - 1 : x - 1;
-    h.nb[0] = (xp >> 1) + g.Lh0 * rest;
-    h.nb[1] = (xm >> 1) + g.Lh0 * rest;
+    h.nb[0] = (xp >> 1) + row;
+    h.nb[1] = (xm >> 1) + row;
     h.nb[2] = cb + g.Lh0 * ((y + 1 == L1) ? -(L1 - 1) : 1);
     h.nb[3] = cb + g.Lh0 * ((y == 0) ? (L1 - 1) : -1);
     const int LY = L1 * g.Lh0;
@@ -492,6 +500,35 @@ __device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
     h.nb[6] = cb + (h.tw_p ? -(L3 - 1) * LZ : LZ);
     h.nb[7] = cb + (h.tw_m ? (L3 - 1) * LZ : -LZ);
     return h;
+}
+
+__device__ __forceinline__ HopSite hop_site(const Geom &g, int p, int cb)
+{
+    const int rest = fdiv(cb, g.dLh0);
+    const int xh = cb - rest * g.Lh0;
+    const int zt = fdiv(rest, g.dL1);
+    const int y = rest - zt * g.L[1];
+    const int t = fdiv(zt, g.dL2);
+    const int z = zt - t * g.L[2];
+    return hop_site_at(g, p, cb, xh, y, z, t);
+}
+
+// The v-th site of parity p in the walk order of tr: its checkerboard index
+// cb and its neighbours.  v < Vh.
+__device__ __forceinline__ HopSite hop_site_walk(const Geom &g, const HopTrav &tr, int p, int v, int &cb)
+{
+    if (!tr.on) {
+        cb = v;
+        return hop_site(g, p, v);
+    }
+    const int q1 = fdiv(v, g.dLh0), xh = v - q1 * g.Lh0;
+    const int q2 = fdiv(q1, tr.dyb), yi = q1 - q2 * tr.yb;
+    const int q3 = fdiv(q2, tr.dtb), ti = q2 - q3 * tr.tb;
+    const int q4 = fdiv(q3, g.dL2), z = q3 - q4 * g.L[2];
+    const int tbk = fdiv(q4, tr.dnyb), ybk = q4 - tbk * tr.nyb;
+    const int y = ybk * tr.yb + yi, t = tbk * tr.tb + ti;
+    cb = xh + g.Lh0 * (y + g.L[1] * (z + g.L[2] * t));
+    return hop_site_at(g, p, cb, xh, y, z, t);
 }
 
 // What a hop thread does with its result.  StoreFin: write it (and, on
@@ -551,11 +588,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     const int nr = a.nrhs;
     int p, u;   // parity and unit (site x rhs) index within it
     hop_unit(g, a.par_out, a.nblk8, nr, p, u);
-    const int cb = nr == 1 ? u : u / nr;
-    const int rr = u - cb * nr;
+    const int v = nr == 1 ? u : u / nr;   // position of the site in the walk
+    const int rr = u - v * nr;
     double nrm2 = 0.0;
-    if (cb < g.Vh) {
-    const HopSite hs = hop_site(g, p, cb);
+    int cb = v;
+    if (v < g.Vh) {
+    const HopSite hs = hop_site_walk(g, a.tr, p, v, cb);
     const int q = p ^ 1;
     const int qoff = q * g.Vh, poff = p * g.Vh;
     const int in_q = a.in_off[q];
@@ -672,7 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
             __shared__ double acc[QKX_MAX_RHS];
             if (threadIdx.x < QKX_MAX_RHS) acc[threadIdx.x] = 0.0;
             __syncthreads();
-            if (cb < g.Vh) atomicAdd(&acc[rr], nrm2);
+            if (v < g.Vh) atomicAdd(&acc[rr], nrm2);
             __syncthreads();
             if ((int)threadIdx.x < nr) a.norm_partial[blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
         }
@@ -836,7 +874,35 @@ int g_block = 256;
 int g_depth64 = 2;   // pipeline depth of the FP64 hop (2 measured faster than 1 at 24^3x48)
 int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 int g_rhs_tile = 0;  // right-hand sides per thread in batched hops (0 or 1: one)
+int g_site_order = 0;   // 1: blocked walk (HopTrav), 0 (default): checkerboard index order
 const int *g_skip = nullptr;
+
+// Blocked walk for a launch moving site_bytes per output site: rows of y in
+// blocks of up to 8, and as many t slices per block (up to 8, dividing L3)
+// as keep one z plane of a block -- the traffic between a site's use as a
+// +z neighbour and its use as a -z neighbour -- under 1.5 MB, well inside
+// the 4 MB L2 of one XCD.
+constexpr double kTravWindow = 1.5e6;
+HopTrav hop_trav(const Geom &g, int par_out, double site_bytes)
+{
+    HopTrav tr{};
+    if (!g_site_order) return tr;
+    int yb = 1;
+    for (int d = 2; d <= 8; d++)
+        if (g.L[1] % d == 0) yb = d;
+    const double row = (par_out < 0 ? g.L[0] : g.Lh0) * site_bytes;
+    int tb = 1;
+    for (int d = 2; d <= 8; d++)
+        if (g.L[3] % d == 0 && row * yb * d <= kTravWindow) tb = d;
+    tr.yb = yb;
+    tr.tb = tb;
+    tr.nyb = g.L[1] / yb;
+    tr.dyb = make_fastdiv(yb);
+    tr.dtb = make_fastdiv(tb);
+    tr.dnyb = make_fastdiv(tr.nyb);
+    tr.on = (yb > 1 || tb > 1) ? 1 : 0;
+    return tr;
+}
 
 template <class S, class G, int B, class Fin = StoreFin>
 int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
@@ -888,6 +954,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
                 (U->recon == QKX_RECON_18 ? 9 : 6);
     double bytes = (double)sites_out * (par_out < 0 ? 4 : 8) * wg + (double)nout * 2 * ws;
     if (aux && aux != in) bytes += (double)nout * ws;
+    a.tr = hop_trav(a.g, par_out, bytes / sites_out);
     qkx::prof_begin();
     const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
     const dim3 grid(nb), blk(g_block);
@@ -948,6 +1015,8 @@ namespace qkx {
 void set_hop_skip(const int *flag) { g_skip = flag; }
 
 void set_rhs_tile(int rt) { g_rhs_tile = rt; }
+
+void set_site_order(int blocked) { g_site_order = blocked; }
 
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
