@@ -58,6 +58,9 @@ class Vector : public LatticeGeometry {
     void zero();
     void copy(Vector &src);
     double norm2();
+    void normalize();
+    void communicateToPlus();
+    void communicateToMinus();
 
     void applyDslash(Vector &psi, Gauge &u);
     void applyDslash_chiral(Vector &psi, Gauge &u);
@@ -120,6 +123,8 @@ class Gauge : public LatticeGeometry {
     void zero();
     void copy(Gauge &src);
     double norm2();
+    void communicateToPlus();
+    void communicateToMinus();
     void applyBoundaryCondition(LatticeInfo *latInfo);
     /* include/lattice_util.h:209-210 (NsmearAPE, alphaAPE from latInfo; tmp is
      * not modified) */
@@ -152,6 +157,8 @@ class Propagator : public LatticeGeometry {
 
     void zero();
     void copy(Propagator &src);
+    void communicateToPlus();
+    void communicateToMinus();
     void copyVector(Vector &v, int nu, int c2);
     void absorbVector(Vector &phi, int nu, int c2);
     double norm2();

@@ -572,7 +572,11 @@ template <class SX, class SR, class SP> struct CgFin {
 #ifndef QKX_HOP16_WAVES
 #define QKX_HOP16_WAVES 1
 #endif
+#ifndef QKX_HOP64_WAVES
+#define QKX_HOP64_WAVES 1
+#endif
 template <class S> struct HopWaves { static constexpr int v = 1; };
+template <> struct HopWaves<SpinorD> { static constexpr int v = QKX_HOP64_WAVES; };
 template <> struct HopWaves<SpinorH> { static constexpr int v = QKX_HOP16_WAVES; };
 template <> struct HopWaves<SpinorB> { static constexpr int v = QKX_HOP16_WAVES; };
 

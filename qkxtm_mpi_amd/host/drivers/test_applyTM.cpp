@@ -34,6 +34,10 @@ int main(int argc, char *argv[])
 
     vecIn->zero();
     if (qkx_gen_z2(&(vecIn->M[0][0][0].real()), latInfo.L, 12)) errorQKXTM("source generation failed\n");
+    gauge->communicateToPlus();
+    gauge->communicateToMinus();
+    vecIn->communicateToPlus();
+    vecIn->communicateToMinus();
     vec->tm_xxDagTm_xx(*vecIn, *gauge, &latInfo, OO);
 
     FILE *out = argc > 1 ? fopen(argv[1], "w") : stdout;
@@ -50,6 +54,8 @@ int main(int argc, char *argv[])
                                     vec->M[LEXIC(t, z, y, x, vecIn->lL)][mu][c1].real(),
                                     vec->M[LEXIC(t, z, y, x, vecIn->lL)][mu][c1].imag());
     if (out != stdout) fclose(out);
+    vec->normalize();
+    printfQKXTM("norm of the normalized vector is %.17e\n", vec->norm2());
     delete gauge;
     delete vecIn;
     delete vec;

@@ -105,6 +105,25 @@ double Vector::norm2()
     return r;
 }
 
+// Vector::normalize :1915-1923: divide by the square root of norm2()
+void Vector::normalize()
+{
+    const double n2 = norm2();   // uploads
+    check(qkx_spinor_rescale(dev(), 1.0 / sqrt(n2)), "normalize");
+    download();
+}
+
+// Ghost-zone exchange, Vector :479-526, Propagator :602-650, Gauge :726-778.
+// The reference exchanges a face only in a direction whose local extent is
+// below the global one; one device holds the whole lattice and its kernels
+// wrap periodically, so there is never a face to exchange.
+void Vector::communicateToPlus() {}
+void Vector::communicateToMinus() {}
+void Propagator::communicateToPlus() {}
+void Propagator::communicateToMinus() {}
+void Gauge::communicateToPlus() {}
+void Gauge::communicateToMinus() {}
+
 // hopping terms: applyDslash :1158, _chiral :1215, _xx :1272, _xx_chiral :1346, Dag :1643
 void Vector::applyDslash(Vector &psi, Gauge &u)
 {

@@ -47,6 +47,8 @@ def test_driver_test_applyTM(q, tmp_path):
     U = q.gen_gauge(L, 0.4, 11)       # boundaryT = +1 in this driver
     ref = orc.tm_xx(np.ascontiguousarray(vin), U, L, 0.156, 0.1, 1, orc.UKQCD, orc.OO, 1)
     assert rel(vout, ref) <= 1e-13
+    # Vector::normalize after the output was written
+    assert abs(float(re.search(r"norm of the normalized vector is (\S+)", r.stdout).group(1)) - 1) <= 1e-14
 
 
 def test_driver_test_gauss_smearing(q, tmp_path):
