@@ -1,5 +1,6 @@
 """GPU checks at the benchmark sizes (24^3x48 single source, the 32^3x64
-12-source batch), through properties that do not depend on the lattice size:
+12-source batch, the 48^3x96 production lattice), through properties that do
+not depend on the lattice size:
 linearity and adjointness of the operator, solutions verified by an FP64
 residual formed outside the solver, SU(3) links after APE smearing,
 Gaussian smearing as a contraction confined to its time slice, and the
@@ -120,3 +121,30 @@ def test_smearing_full_size(q, cfg2):
     o = out.download()
     assert out.norm2() <= src.norm2() * (1 + 1e-12)
     assert not np.any(o[np.arange(L2[3] * V3) // V3 != 7])
+
+
+def test_largest_lattice_48x48x48x96(q):
+    """The production lattice (10.6M sites, 6.1 GB of FP64 links): adjointness
+    through device inner products, and an FP16-inner solve to 1e-10 verified
+    by an FP64 residual formed outside the solver."""
+    L = (48, 48, 48, 96)
+    print("48^3x96: generating links", flush=True)
+    U = q.gen_gauge(L, 0.25, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    del U
+    p = q.params(L, 0.1632, 0.004, 1, -1)
+    X, Y = q.Spinor(L, host=q.gen_gaussian(L, 41)), q.Spinor(L, host=q.gen_gaussian(L, 42))
+    DX, DtY = q.Spinor(L), q.Spinor(L)
+    print("48^3x96: adjointness", flush=True)
+    DX.tm(X, u, p)
+    DtY.tmDag(Y, u, p)
+    lhs, rhs = q.reCdotXdagY(Y, DX), q.reCdotXdagY(DtY, X)
+    assert abs(lhs - rhs) <= 1e-12 * np.sqrt(DX.norm2() * Y.norm2())
+    del X, Y, DtY
+    b, x = q.Spinor(L, host=q.gen_z2(L, 2)), q.Spinor(L)
+    print("48^3x96: FP16-inner solve", flush=True)
+    st = q.solve(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=1e-10)
+    DX.tm(x, u, p)
+    q.X_eq_aX_p_Y(DX, -1.0, b)
+    res = np.sqrt(DX.norm2() / b.norm2())
+    assert st.converged and res <= 1e-10, (st.iterations, res)
