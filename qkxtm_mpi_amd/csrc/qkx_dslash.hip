@@ -550,18 +550,21 @@ template <class SX, class SR, class SP> struct CgFin {
     int npap;
     double *r2_part;          // out: per-workgroup |r|^2 after the update
     double alpha;
+    // p.Ap in the same fixed order in every workgroup: strided per-thread
+    // sums, an xor butterfly (the same bits in every lane), then the waves in
+    // order
     template <class A> __device__ __forceinline__ void begin(const A &)
     {
-        __shared__ double lds[256];
+        __shared__ double lds[4];
         double v = 0;
         for (int i = threadIdx.x; i < npap; i += blockDim.x) v += pap_part[i];
-        lds[threadIdx.x] = v;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
         __syncthreads();
-        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
-            __syncthreads();
-        }
-        alpha = *r2cur / lds[0];
+        double pap = lds[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) pap += lds[w];
+        alpha = *r2cur / pap;
         __syncthreads();
     }
 };

@@ -70,19 +70,22 @@ int ensure_state(int nblocks)
 }
 
 // Every thread of the workgroup gets sum(v[0..n)), added in a fixed order.
+// Sum of v[0..n) seen by every thread of the workgroup, in an order fixed by
+// n and the workgroup size only, so all workgroups of a launch agree bit for
+// bit.  The xor butterfly leaves the same bits in every lane (each step adds
+// the same two operands, in either order).
 __device__ double block_sum_array(const double *v, int n)
 {
-    __shared__ double lds[kThreads];
+    __shared__ double lds[kThreads / 64];
     double s = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
-    lds[threadIdx.x] = s;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
-        __syncthreads();
-    }
     double r = lds[0];
-    __syncthreads();
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r += lds[w];
+    __syncthreads();   // lds is reused by the next call
     return r;
 }
 
@@ -176,6 +179,16 @@ __global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, int n, cons
         if (blockIdx.x == 0 && threadIdx.x == 0) *next = *cur;
         return;
     }
+    // this site's p and r are fetched before the partial sums are reduced,
+    // so their HBM latency overlaps the reduction
+    using T = typename SR::Real;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    Cx<typename S::Real> pv[12];
+    Cx<T> rv[12];
+    if (s < n) {
+        p.load(s, pv);
+        r.load(s, rv);
+    }
     const double r2n = block_sum_array(r2_part, nr2);
     double beta;
     int flag = 0;
@@ -193,14 +206,8 @@ __global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, int n, cons
             *next = CGSlot{r2n, cur->r2_old, cur->pAp, r2n, isfinite(r2n) ? 0 : 3, cur->k};
     }
     if (flag) return;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    using T = typename SR::Real;
     const T b = T(beta);
-    Cx<typename S::Real> pv[12];
-    Cx<T> rv[12];
-    p.load(s, pv);
-    r.load(s, rv);
     Cx<typename S::Real> out[12];
 #pragma unroll
     for (int k = 0; k < 12; k++)
