@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the measured kernels a few times each (for hardware-counter runs): "
                          "FP64 tm, FP16 even-odd normal operator, FP64 field copy (counter calibration)")
+    ap.add_argument("--batch-kernels-only", action="store_true",
+                    help="only apply the FP16 even-odd normal operator to a batch of 12 right-hand sides at "
+                         "24^3x48, three times (hardware-counter runs of the batched hop)")
     return ap.parse_args()
 
 
@@ -478,6 +481,24 @@ def run_smear(q, args):
     print(json.dumps(res))
 
 
+def batch_kernels(q):
+    """The 12-RHS FP16 even-odd normal operator at 24^3x48 (4 hops), three
+    times: 12 seeded Gaussian sources in one batch field, FP16 12-real links."""
+    U = q.gen_gauge(L, EPS, 1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    del U
+    uh = q.Gauge(L, q.HALF, q.RECON_12, -1)
+    uh.convert_from(u)
+    p = q.params(L, KAPPA, MU, 1, -1)
+    src = np.stack([q.gen_gaussian(L, 100 + r) for r in range(12)])
+    b = q.Spinor(L, q.HALF, q.EVEN, host=src, nrhs=12)
+    o = q.Spinor(L, q.HALF, q.EVEN, nrhs=12)
+    for _ in range(3):
+        o.tm_xxDagTm_xx(b, uh, p, q.EE)
+    q.sync()
+    print("batch kernels done", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -496,6 +517,9 @@ def main():
         return
     if args.workload == "smear":
         run_smear(q, args)
+        return
+    if args.batch_kernels_only:
+        batch_kernels(q)
         return
 
     U = q.gen_gauge(L, EPS, 1)
