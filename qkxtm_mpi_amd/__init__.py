@@ -76,7 +76,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
-    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
+    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_set_batch_link_sharing": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
     "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
@@ -652,6 +652,11 @@ def set_cg_graph(on=True):
 
 def set_rhs_tile(rt=0):
     _check(lib().qkx_set_rhs_tile(int(rt)), "qkx_set_rhs_tile")
+
+
+def set_batch_link_sharing(on=True):
+    """Share each site's links between the RHS threads of FP16 batches (default on)."""
+    _check(lib().qkx_set_batch_link_sharing(int(on)), "qkx_set_batch_link_sharing")
 
 
 def set_site_order(blocked=True):

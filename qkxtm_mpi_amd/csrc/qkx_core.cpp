@@ -269,6 +269,13 @@ int qkx_set_cg_batch(int iterations)
     return QKX_SUCCESS;
 }
 
+int qkx_set_batch_link_sharing(int on)
+{
+    QKX_CHECK_ARG(on == 0 || on == 1, "qkx_set_batch_link_sharing: 0 or 1");
+    qkx::set_batch_link_sharing(on);
+    return QKX_SUCCESS;
+}
+
 int qkx_set_site_order(int blocked)
 {
     QKX_CHECK_ARG(blocked == 0 || blocked == 1, "qkx_set_site_order: 0 or 1");

@@ -40,6 +40,7 @@ void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);
 void set_site_order(int blocked);
+void set_batch_link_sharing(int on);
 // even-odd Schur operator on parity pe (qkx_ops.cpp):
 //   out_pe = [A in - kappa^2 H A^-1 H in]_pe (then g5 if g5out); the optional
 //   norm_partial collects |out|^2 per workgroup of the second hop

@@ -240,9 +240,34 @@ __device__ __forceinline__ void views16(const uint32_t w[12], h2v v0[6], h2v v1[
     }
 }
 
+// The 9 (re, im) half pairs of a stored FP16 12-real link: rows 0 and 1 as
+// stored, row 2 = conj(row0 x row1) formed with dot2 and rounded to half.
+__device__ __forceinline__ void link_rows16(const typename Gauge16<false>::Raw &lk, h2v U[9])
+{
+    U[0] = h2_of(lk.a.x);
+    U[1] = h2_of(lk.a.y);
+    U[2] = h2_of(lk.a.z);
+    U[3] = h2_of(lk.a.w);
+    U[4] = h2_of(lk.b.x);
+    U[5] = h2_of(lk.b.y);
+    // third row conj(x y - z w): views of row 1 and the negated row 0, once
+    const h2v c3 = cj16(U[3]), c4 = cj16(U[4]), c5 = cj16(U[5]);
+    const h2v s3 = sw16(U[3]), s4 = sw16(U[4]), s5 = sw16(U[5]);
+    const h2v n0 = -U[0], n1 = -U[1], n2 = -U[2];
+    auto cross = [&](h2v x, h2v nx, h2v yc, h2v ys, h2v nz, h2v z, h2v wc, h2v ws) {
+        const float re = dot2(x, yc, dot2(nz, wc, 0.f));
+        const float im = dot2(z, ws, dot2(nx, ys, 0.f));
+        return h2v{(_Float16)re, (_Float16)im};
+    };
+    U[6] = cross(U[1], n1, c5, s5, n2, U[2], c4, s4);
+    U[7] = cross(U[2], n2, c3, s3, n0, U[0], c5, s5);
+    U[8] = cross(U[0], n0, c4, s4, n1, U[1], c3, s3);
+}
+
+// One hop of an FP16 neighbour with the link given as its 9 half pairs.
 template <int B, int MU, int KIND, bool DAGL, class T>
-__device__ __forceinline__ void apply_dir_h16(Cx<T> R[12], const typename SpinorH::Raw &ps,
-                                              const typename Gauge16<false>::Raw &lk, T sign)
+__device__ __forceinline__ void apply_rows_h16(Cx<T> R[12], const typename SpinorH::Raw &ps, const h2v U[9],
+                                               T sign)
 {
     constexpr Proj P = kProj[B][MU][KIND];
     static_assert(P.a[0] == P.a[1] && (P.a[0] == KP1 || P.a[0] == K2), "projector shape");
@@ -257,27 +282,6 @@ __device__ __forceinline__ void apply_dir_h16(Cx<T> R[12], const typename Spinor
     h2v v0[6], v1[6];
     views16<B, MU, KIND, DAGL, 0>(w, v0, v1);
     views16<B, MU, KIND, DAGL, 1>(w, v0, v1);
-    h2v U[9];
-    U[0] = h2_of(lk.a.x);
-    U[1] = h2_of(lk.a.y);
-    U[2] = h2_of(lk.a.z);
-    U[3] = h2_of(lk.a.w);
-    U[4] = h2_of(lk.b.x);
-    U[5] = h2_of(lk.b.y);
-    // third row conj(x y - z w): views of row 1 and the negated row 0, once
-    {
-        const h2v c3 = cj16(U[3]), c4 = cj16(U[4]), c5 = cj16(U[5]);
-        const h2v s3 = sw16(U[3]), s4 = sw16(U[4]), s5 = sw16(U[5]);
-        const h2v n0 = -U[0], n1 = -U[1], n2 = -U[2];
-        auto cross = [&](h2v x, h2v nx, h2v yc, h2v ys, h2v nz, h2v z, h2v wc, h2v ws) {
-            const float re = dot2(x, yc, dot2(nz, wc, 0.f));
-            const float im = dot2(z, ws, dot2(nx, ys, 0.f));
-            return h2v{(_Float16)re, (_Float16)im};
-        };
-        U[6] = cross(U[1], n1, c5, s5, n2, U[2], c4, s4);
-        U[7] = cross(U[2], n2, c3, s3, n0, U[0], c5, s5);
-        U[8] = cross(U[0], n0, c4, s4, n1, U[1], c3, s3);
-    }
     Cx<T> xi[6];
 #pragma unroll
     for (int k = 0; k < 2; k++)
@@ -295,6 +299,15 @@ __device__ __forceinline__ void apply_dir_h16(Cx<T> R[12], const typename Spinor
     collect_scaled<B, MU, KIND, 1>(R, xi, n);
     collect_scaled<B, MU, KIND, 2>(R, xi, n);
     collect_scaled<B, MU, KIND, 3>(R, xi, n);
+}
+
+template <int B, int MU, int KIND, bool DAGL, class T>
+__device__ __forceinline__ void apply_dir_h16(Cx<T> R[12], const typename SpinorH::Raw &ps,
+                                              const typename Gauge16<false>::Raw &lk, T sign)
+{
+    h2v U[9];
+    link_rows16(lk, U);
+    apply_rows_h16<B, MU, KIND, DAGL>(R, ps, U, sign);
 }
 
 #ifndef QKX_HOP16_DOT2
@@ -438,6 +451,22 @@ __device__ __forceinline__ void block_partial(double v, double *dst)
         double s = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[w];
         *dst = s;
+    }
+}
+
+// Per right-hand side sums of a workgroup's values, in thread order (fixed,
+// whatever order the waves run in): thread t's value belongs to right-hand
+// side (u0 + t) mod nr, u0 the unit index of thread 0 modulo nr.  Threads
+// 0 .. nr-1 write dst[0 .. nr-1].
+__device__ __forceinline__ void block_partial_rhs(double v, int nr, int u0, double *dst)
+{
+    __shared__ double vals[256];
+    vals[threadIdx.x] = v;
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+        double s = 0;
+        for (int t = ((int)threadIdx.x - u0 + nr) % nr; t < (int)blockDim.x; t += nr) s += vals[t];
+        dst[threadIdx.x] = s;
     }
 }
 
@@ -713,14 +742,129 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
         if (nr == 1) {
             block_partial(nrm2, a.norm_partial + blockIdx.x);
         } else {
-            // per right-hand side: LDS accumulation, then one partial per rhs
-            __shared__ double acc[QKX_MAX_RHS];
-            if (threadIdx.x < QKX_MAX_RHS) acc[threadIdx.x] = 0.0;
-            __syncthreads();
-            if (v < g.Vh) atomicAdd(&acc[rr], nrm2);
-            __syncthreads();
-            if ((int)threadIdx.x < nr) a.norm_partial[blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+            // one partial per right-hand side, summed in thread order
+            const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+            block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
         }
+    }
+}
+
+// Batches of FP16 spinors on FP16 12-real links, 8 to 16 right-hand sides:
+// a workgroup holds whole sites (nr * (sites per workgroup) threads, the RHS
+// of a site on consecutive threads).  The 8 links of each of its sites are
+// fetched and reconstructed once per workgroup, by one thread per (site,
+// direction), into LDS; every (site, RHS) thread then reads them from there.
+// In hop_kernel each thread fetches and reconstructs its site's links itself,
+// and a wave repeats that work for the ~5 sites it covers 12 times over.
+constexpr int kShareSites = 32;   // sites per workgroup at most (nr >= 8, <= 256 threads)
+
+template <int B, bool DAG, bool AUX>
+__global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH, SpinorH, Gauge16<false>> a)
+{
+    using T = float;
+    if (a.skip && *a.skip) return;   // uniform over the grid
+    __shared__ int lsite[kShareSites][8];          // link site of each (site, direction); -1: no site
+    __shared__ uint32_t lrow[kShareSites][8][9];   // the link's 9 (re, im) half pairs
+    const Geom &g = a.g;
+    const int nr = a.nrhs;
+    int p, u;
+    hop_unit(g, a.par_out, a.nblk8, nr, p, u);
+    const int v = u / nr;
+    const int rr = u - v * nr;
+    const int j = threadIdx.x / nr;   // site within the workgroup
+    int cb = v;
+    HopSite hs{};
+    const bool valid = v < g.Vh;
+    if (valid) hs = hop_site_walk(g, a.tr, p, v, cb);
+    const int q = p ^ 1;
+    const int qoff = q * g.Vh, poff = p * g.Vh;
+    if (rr == 0) {
+#pragma unroll
+        for (int d = 0; d < 8; d++)
+            lsite[j][d] = !valid ? -1 : (d & 1) ? qoff + hs.nb[d] : poff + cb;
+    }
+    __syncthreads();
+    // (site, direction) tasks, dense over the workgroup's threads
+    const int nsites = blockDim.x / nr;
+    for (int t = threadIdx.x; t < nsites * 8; t += blockDim.x) {
+        const int js = t >> 3, d = t & 7;
+        const int sl = lsite[js][d];
+        if (sl >= 0) {
+            h2v Ul[9];
+            link_rows16(a.U.fetch(d >> 1, sl), Ul);
+#pragma unroll
+            for (int k = 0; k < 9; k++) lrow[js][d][k] = __builtin_bit_cast(uint32_t, Ul[k]);
+        }
+    }
+    __syncthreads();
+    double nrm2 = 0.0;
+    if (valid) {
+        const int in_q = a.in_off[q];
+        const T bc = T(g.bc);
+        Cx<T> R[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) R[k] = {T(0), T(0)};
+        constexpr int FW = DAG ? 1 : 0;
+        constexpr int BW = DAG ? 0 : 1;
+        const SpinorH &in = a.in;
+        auto rows = [&](int d, h2v U[9]) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) U[k] = h2_of(lrow[j][d][k]);
+        };
+        // neighbour spinors two hops ahead of the arithmetic
+        auto f0 = in.template fetch<spin_mask<B, 0, FW>()>(in_q + hs.nb[0] * nr + rr);
+        auto f1 = in.template fetch<spin_mask<B, 0, BW>()>(in_q + hs.nb[1] * nr + rr);
+        h2v U[9];
+        rows(0, U);
+        apply_rows_h16<B, 0, FW, false>(R, f0, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f2 = in.template fetch<spin_mask<B, 1, FW>()>(in_q + hs.nb[2] * nr + rr);
+        rows(1, U);
+        apply_rows_h16<B, 0, BW, true>(R, f1, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f3 = in.template fetch<spin_mask<B, 1, BW>()>(in_q + hs.nb[3] * nr + rr);
+        rows(2, U);
+        apply_rows_h16<B, 1, FW, false>(R, f2, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f4 = in.template fetch<spin_mask<B, 2, FW>()>(in_q + hs.nb[4] * nr + rr);
+        rows(3, U);
+        apply_rows_h16<B, 1, BW, true>(R, f3, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f5 = in.template fetch<spin_mask<B, 2, BW>()>(in_q + hs.nb[5] * nr + rr);
+        rows(4, U);
+        apply_rows_h16<B, 2, FW, false>(R, f4, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f6 = in.template fetch<spin_mask<B, 3, FW>()>(in_q + hs.nb[6] * nr + rr);
+        rows(5, U);
+        apply_rows_h16<B, 2, BW, true>(R, f5, U, T(1));
+        QKX_SCHED_FENCE();
+        auto f7 = in.template fetch<spin_mask<B, 3, BW>()>(in_q + hs.nb[7] * nr + rr);
+        rows(6, U);
+        apply_rows_h16<B, 3, FW, false>(R, f6, U, hs.tw_p ? bc : T(1));
+        QKX_SCHED_FENCE();
+        rows(7, U);
+        apply_rows_h16<B, 3, BW, true>(R, f7, U, hs.tw_m ? bc : T(1));
+
+        Cx<T> res[12];
+        const Epi ep = a.epi[rr];
+        if constexpr (AUX) {
+            Cx<T> xv[12];
+            a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
+            epilogue<B>(ep, R, xv, res);
+        } else {
+            epilogue<B, T>(ep, R, nullptr, res);
+        }
+        a.out.store(a.out_off[p] + cb * nr + rr, res);
+        if (a.norm_partial) {
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+        }
+    }
+    if (a.norm_partial) {
+        // one partial per right-hand side, summed in thread order
+        const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+        block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
     }
 }
 
@@ -829,15 +973,19 @@ __global__ __launch_bounds__(256) void hop_tile_kernel(HopArgs<SIn, SOut, SAux, 
         }
     }
     if (a.norm_partial) {
-        __shared__ double acc[QKX_MAX_RHS];
-        if (threadIdx.x < QKX_MAX_RHS) acc[threadIdx.x] = 0.0;
-        __syncthreads();
-        if (cb < g.Vh) {
+        // one partial per right-hand side, summed in thread order: thread t
+        // holds right-hand sides ((u0 + t) mod ng) RT + j
+        __shared__ double vals[256 * RT];
 #pragma unroll
-            for (int j = 0; j < RT; j++) atomicAdd(&acc[r0 + j], nrm2[j]);
-        }
+        for (int j = 0; j < RT; j++) vals[threadIdx.x * RT + j] = nrm2[j];
         __syncthreads();
-        if ((int)threadIdx.x < nr) a.norm_partial[blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+        if ((int)threadIdx.x < nr) {
+            const int gi = threadIdx.x / RT, j = threadIdx.x - gi * RT;
+            const int u0 = (((u - (int)threadIdx.x) % ng) + ng) % ng;
+            double sum = 0;
+            for (int t = (gi - u0 + ng) % ng; t < (int)blockDim.x; t += ng) sum += vals[t * RT + j];
+            a.norm_partial[blockIdx.x * nr + threadIdx.x] = sum;
+        }
     }
 }
 
@@ -881,6 +1029,7 @@ int g_block = 256;
 int g_depth64 = 2;   // pipeline depth of the FP64 hop (2 measured faster than 1 at 24^3x48)
 int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 int g_rhs_tile = 0;  // right-hand sides per thread in batched hops (0 or 1: one)
+int g_share_links = 1;  // FP16 batches of >= 8 RHS: links shared per workgroup through LDS
 int g_site_order = 0;   // 1: blocked walk (HopTrav), 0 (default): checkerboard index order
 const int *g_skip = nullptr;
 
@@ -938,15 +1087,22 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     int rt = 1;
     if (a.nrhs > 1 && g_rhs_tile > 1 && g_rhs_tile <= 4 && a.nrhs % g_rhs_tile == 0) rt = g_rhs_tile;
     const int upsite = a.nrhs / rt;   // threads per output site
+    // FP16 batches of 8 or more RHS on FP16 links: each workgroup fetches and
+    // reconstructs its sites' links once into LDS (hop_share_kernel); its
+    // size is a whole number of sites
+    constexpr bool share_ok = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> &&
+                              std::is_same_v<Fin, StoreFin> && QKX_HOP16_DOT2;
+    const bool share = share_ok && g_share_links && rt == 1 && a.nrhs >= 8 && g_block / a.nrhs <= kShareSites;
+    const int bs = share ? a.nrhs * (g_block / a.nrhs) : g_block;   // threads per workgroup
     const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
-    int nb = (sites_out * upsite + g_block - 1) / g_block;
+    int nb = (sites_out * upsite + bs - 1) / bs;
     a.nblk8 = 0;
     if (g_xcd_remap && nb >= 64) {
         if (par_out >= 0) {
             nb = (nb + 7) / 8 * 8;
             a.nblk8 = nb / 8;
         } else {
-            int nbh = ((a.g.Vh * upsite + g_block - 1) / g_block + 7) / 8 * 8;   // blocks per parity
+            int nbh = ((a.g.Vh * upsite + bs - 1) / bs + 7) / 8 * 8;   // blocks per parity
             a.nblk8 = nbh / 8;
             nb = 2 * nbh;
         }
@@ -964,7 +1120,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     a.tr = hop_trav(a.g, par_out, bytes / sites_out);
     qkx::prof_begin();
     const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
-    const dim3 grid(nb), blk(g_block);
+    const dim3 grid(nb), blk(bs);
     hipStream_t st = qkx::g_stream;
 #define QKX_LAUNCH_HOP(DG, AX, DP) \
     hipLaunchKernelGGL((hop_kernel<S, S, S, G, B, DG, AX, DP, Fin>), grid, blk, 0, st, a, fin)
@@ -979,7 +1135,17 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     constexpr bool store = std::is_same_v<Fin, StoreFin>;
     constexpr bool tiles = store && B == QKX_BASIS_UKQCD;
     if (!tiles) rt = 1;
-    if (rt == 4) {
+    if (share) {
+        if constexpr (share_ok) {
+            if (dagger) {
+                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, true, true>), grid, blk, 0, st, a);
+                else hipLaunchKernelGGL((hop_share_kernel<B, true, false>), grid, blk, 0, st, a);
+            } else {
+                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, false, true>), grid, blk, 0, st, a);
+                else hipLaunchKernelGGL((hop_share_kernel<B, false, false>), grid, blk, 0, st, a);
+            }
+        }
+    } else if (rt == 4) {
         if constexpr (tiles) QKX_LAUNCH_TILES(4);
     } else if (rt == 3) {
         if constexpr (tiles) QKX_LAUNCH_TILES(3);
@@ -1024,6 +1190,8 @@ void set_hop_skip(const int *flag) { g_skip = flag; }
 void set_rhs_tile(int rt) { g_rhs_tile = rt; }
 
 void set_site_order(int blocked) { g_site_order = blocked; }
+
+void set_batch_link_sharing(int on) { g_share_links = on; }
 
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32)
 {

@@ -39,6 +39,8 @@ double *g_bpart = nullptr;
 int g_bpart_cap = 0;
 double *g_bres = nullptr, *g_bres_host = nullptr;
 double *g_bred = nullptr;   // [0, NR) p.Ap, [NR, 2 NR) |r|^2, pre-summed when the partials are many
+double *g_bstage = nullptr; // first-stage sums of many partials: kStageRows rows of NR
+constexpr int kStageRows = 256;
 constexpr int kDirectPartials = 2048;   // workgroups' partials read directly by each consumer
 
 int ensure_bstate(int nblocks)
@@ -49,6 +51,7 @@ int ensure_bstate(int nblocks)
         QKX_HIP(hipMalloc(&g_bres, NR * sizeof(double)));
         QKX_HIP(hipHostMalloc((void **)&g_bres_host, NR * sizeof(double), hipHostMallocDefault));
         QKX_HIP(hipMalloc(&g_bred, 2 * NR * sizeof(double)));
+        QKX_HIP(hipMalloc(&g_bstage, kStageRows * NR * sizeof(double)));
     }
     size_t need = (size_t)nblocks * NR;
     if ((int)need > g_bpart_cap) {
@@ -80,15 +83,20 @@ __device__ void block_sums_rhs(const double *part, int nblocks, int nr, double *
     __syncthreads();
 }
 
-// per-workgroup, per-rhs partial of v (rr = this thread's right-hand side)
-__device__ void block_partial_rhs(double v, int rr, bool valid, int nr, double *dst)
+// per-workgroup, per-rhs partial of v, summed in thread order (fixed, whatever
+// order the waves run in).  The thread's unit is blockIdx.x * blockDim.x +
+// threadIdx.x and its right-hand side that unit mod nr.
+__device__ void block_partial_rhs(double v, bool valid, int nr, double *dst)
 {
-    __shared__ double acc[NR];
-    if (threadIdx.x < NR) acc[threadIdx.x] = 0.0;
+    __shared__ double vals[kT];
+    vals[threadIdx.x] = valid ? v : 0.0;
     __syncthreads();
-    if (valid) atomicAdd(&acc[rr], v);
-    __syncthreads();
-    if ((int)threadIdx.x < nr) dst[(size_t)blockIdx.x * nr + threadIdx.x] = acc[threadIdx.x];
+    if ((int)threadIdx.x < nr) {
+        const int u0 = (int)(((size_t)blockIdx.x * blockDim.x) % (size_t)nr);
+        double s = 0;
+        for (int t = ((int)threadIdx.x - u0 + nr) % nr; t < (int)blockDim.x; t += nr) s += vals[t];
+        dst[(size_t)blockIdx.x * nr + threadIdx.x] = s;
+    }
 }
 
 __device__ __forceinline__ bool frozen(const BSlot *c, int r) { return !c->active[r] || c->r2[r] <= c->stop2[r]; }
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(kT) void cgb_update_kernel(SX x, SR r, S p, S Ap, i
         }
         r.store(u, rv);
     }
-    block_partial_rhs(acc, rr, valid, nr, r2_part);
+    block_partial_rhs(acc, valid, nr, r2_part);
 }
 
 // mode 0: iteration; mode 1: restart after a reliable update (beta from r2_old)
@@ -211,14 +219,7 @@ __global__ __launch_bounds__(kT) void normb_partial_kernel(S r, int n, int nr, d
 #pragma unroll
         for (int k = 0; k < 12; k++) acc += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
     }
-    block_partial_rhs(acc, u % nr, valid, nr, part);
-}
-
-__global__ __launch_bounds__(kT) void normb_finish_kernel(const double *part, int nblocks, int nr, double *res)
-{
-    __shared__ double s[NR];
-    block_sums_rhs(part, nblocks, nr, s);
-    if ((int)threadIdx.x < nr) res[threadIdx.x] = s[threadIdx.x];
+    block_partial_rhs(acc, valid, nr, part);
 }
 
 // out[r] = sum_b part[b*nr + r], fixed order, one workgroup of 1024
@@ -239,6 +240,28 @@ __global__ __launch_bounds__(1024) void sum_rhs_kernel(const double *part, int n
     }
 }
 
+// First stage of a many-row sum: workgroup g sums rows [g chunk, (g+1) chunk)
+// of part into out[g*nr + r], in a fixed order (strided per thread, then the
+// thread groups in order).
+__global__ __launch_bounds__(kT) void sum_rhs_chunks_kernel(const double *part, int nblocks, int nr, int chunk,
+                                                            double *out)
+{
+    __shared__ double lds[kT];
+    const int groups = blockDim.x / nr;
+    const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+    const int b0 = blockIdx.x * chunk, b1 = min(nblocks, b0 + chunk);
+    double v = 0;
+    if (g < groups)
+        for (int b = b0 + g; b < b1; b += groups) v += part[(size_t)b * nr + r];
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+        double t = 0;
+        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
+        out[(size_t)blockIdx.x * nr + threadIdx.x] = t;
+    }
+}
+
 template <class SL>
 __global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
 {
@@ -255,12 +278,29 @@ __global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
 
 int nblk(int n) { return (n + kT - 1) / kT; }
 
+// dst[r] = sum_b part[b*nr + r] in a fixed order: a single workgroup for a
+// few hundred rows; beyond that a first stage of up to kStageRows workgroups
+// over contiguous chunks (one workgroup reading every row is bound by its
+// load latency: 64 us for 16k rows of 12).
+void sum_rows(const double *part, int nblocks, int nr, double *dst)
+{
+    if (nblocks > kStageRows) {
+        const int chunk = (nblocks + kStageRows - 1) / kStageRows;
+        const int g = (nblocks + chunk - 1) / chunk;
+        hipLaunchKernelGGL(sum_rhs_chunks_kernel, dim3(g), dim3(kT), 0, qkx::g_stream, part, nblocks, nr, chunk,
+                           g_bstage);
+        part = g_bstage;
+        nblocks = g;
+    }
+    hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, nblocks, nr, dst);
+}
+
 // per-rhs partials (part, nblocks) as consumers should read them: pre-summed
 // into dst when there are many
 void reduce_if_many(const double *&part, int &nblocks, int nr, double *dst)
 {
     if (nblocks <= kDirectPartials) return;
-    hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, nblocks, nr, dst);
+    sum_rows(part, nblocks, nr, dst);
     part = dst;
     nblocks = 1;
 }
@@ -281,7 +321,7 @@ int norms(const qkx_spinor_s *f, std::vector<double> &out)
         return QKX_SUCCESS;
     });
     if (rc) return rc;
-    hipLaunchKernelGGL(normb_finish_kernel, dim3(1), dim3(kT), 0, qkx::g_stream, g_bpart + g_bpart_cap, nb, nr, g_bres);
+    sum_rows(g_bpart + g_bpart_cap, nb, nr, g_bres);
     QKX_HIP(hipGetLastError());
     QKX_HIP(hipMemcpyAsync(g_bres_host, g_bres, nr * sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream));
     QKX_HIP(hipStreamSynchronize(qkx::g_stream));
@@ -470,7 +510,9 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     QKX_CHECK_ARG(o->tol > 0 && o->maxIter > 0, "qkx_solve_batch: bad tol/maxIter");
     const int nr = b->nrhs;
     const int *L = b->L;
-    if (int rc = ensure_bstate(b->V * nr / kT + 64)) return rc;
+    // hop workgroups hold at least 128 (site, RHS) units (whole sites when
+    // links are shared): partial capacity for that many workgroups
+    if (int rc = ensure_bstate(b->V * nr / 128 + 64)) return rc;
     auto t0 = std::chrono::steady_clock::now();
     const bool mixed = o->inner_prec != QKX_PREC_DOUBLE;
     const int lp = o->inner_prec;

@@ -187,3 +187,39 @@ def test_solve_batch_propagator_point_sources(q, cfg1):
     for sc in range(12):
         res = rel(orc.tm(xs[sc], Ub, L1, K1, M1, 1), src[sc])
         assert st[sc].converged and res <= 1e-10, f"spin-colour {sc}: {res:.2e}"
+
+
+@pytest.mark.parametrize("L,nrhs", [(LS, 12), (LS, 8), ((8, 6, 4, 8), 9), ((8, 8, 8, 16), 12), ((8, 8, 8, 16), 16)])
+def test_batch_link_sharing_identical(q, L, nrhs):
+    """FP16 batches of 8-16 right-hand sides fetch and rebuild each site's
+    links once per workgroup, in LDS: the same bits as every (site, RHS)
+    thread doing it itself, and as the same operator on single fields.  The
+    small lattices leave the last workgroup partly beyond the sites; the
+    8^3x16 ones take the XCD-remapped grid."""
+    print(f"link sharing L={L} nrhs={nrhs}", flush=True)
+    U = q.gen_gauge(L, 0.4, 23)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    uh = q.Gauge(L, q.HALF, q.RECON_12, -1)
+    uh.convert_from(u)
+    p = q.params(L, KS, MS, 1, -1)
+    src = _srcs(q, L, nrhs, 300)
+    be, oe = q.Spinor(L, q.HALF, q.EVEN, host=src, nrhs=nrhs), q.Spinor(L, q.HALF, q.EVEN, nrhs=nrhs)
+    bf, of = q.Spinor(L, q.HALF, host=src, nrhs=nrhs), q.Spinor(L, q.HALF, nrhs=nrhs)
+    got = {}
+    try:
+        for share in (0, 1):
+            q.set_batch_link_sharing(share)
+            oe.tm_xxDagTm_xx(be, uh, p, q.EE)
+            r1 = oe.download()
+            of.tm(bf, uh, p)
+            r2 = of.download()
+            of.tmDag(bf, uh, p)
+            got[share] = (r1, r2, of.download())
+    finally:
+        q.set_batch_link_sharing(1)
+    for k, (a, b) in enumerate(zip(got[0], got[1])):
+        assert np.array_equal(a, b), k
+    for r in (0, nrhs - 1):
+        xs, ys = q.Spinor(L, q.HALF, q.EVEN, host=src[r]), q.Spinor(L, q.HALF, q.EVEN)
+        ys.tm_xxDagTm_xx(xs, uh, p, q.EE)
+        assert np.array_equal(ys.download(), got[1][0][r]), r
