@@ -532,16 +532,27 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         }
     }
 
-    BatchOp Ah{}, Al{};
+    // FP32 links for the parts whose result is stored at FP16/FP32 anyway:
+    // the normal-equation right-hand side and the residual r = Op^dag s of
+    // each reliable update (s itself, the true residual, stays FP64)
+    const qkx_gauge_s *Um = U;
+    if (mixed) {
+        qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
+        if ((rc = qkx::gauge_cached(&Uc->mid, U, QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
+        Um = Uc->mid;
+    }
+
+    BatchOp Ah{}, Al{}, Am{};
     Ah.U = U;
     Al.U = Ui;
-    Ah.kappa = Al.kappa = p->kappa;
-    Ah.nr = Al.nr = nr;
-    Ah.eo = Al.eo = o->eo != 0;
+    Am.U = Um;
+    Ah.kappa = Al.kappa = Am.kappa = p->kappa;
+    Ah.nr = Al.nr = Am.nr = nr;
+    Ah.eo = Al.eo = Am.eo = o->eo != 0;
     for (int r = 0; r < NR; r++) {
         const double m = (mu && r < nr) ? mu[r] : p->mu;
-        Ah.a[r] = Al.a[r] = 2.0 * p->kappa * m * p->twistSign;
-        Ah.nrm[r] = Al.nrm[r] = 1.0 + 4.0 * p->kappa * p->kappa * m * m;
+        Ah.a[r] = Al.a[r] = Am.a[r] = 2.0 * p->kappa * m * p->twistSign;
+        Ah.nrm[r] = Al.nrm[r] = Am.nrm[r] = 1.0 + 4.0 * p->kappa * p->kappa * m * m;
     }
     // temporaries; those a variant does not use are not allocated (a 12-rhs
     // FP64 full field at 48^3x96 is 24.5 GB)
@@ -575,13 +586,26 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         le = he;
     }
     qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub), *xs = T(QKX_PREC_DOUBLE, sub),
-                 *s = T(QKX_PREC_DOUBLE, sub), *rh = T(QKX_PREC_DOUBLE, sub, mixed),
+                 *s = T(QKX_PREC_DOUBLE, sub),
                  *bo = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo), *to = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo),
                  *r = T(rp, sub), *pp = T(lp, sub), *Ap = T(lp, sub), *xl = T(xp, sub, mixed),
                  *tf = eo ? T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL) : hf;
+    const int S32 = QKX_PREC_SINGLE;
+    qkx_spinor_s *mo = T(S32, QKX_SUBSET_ODD, mixed && eo), *me = T(S32, QKX_SUBSET_EVEN, mixed && eo),
+                 *sm = T(S32, sub, mixed), *rm = T(S32, sub, mixed);
     if (nomem) return QKX_ERR_NOMEM;
     Ah.tmp_full = hf; Ah.tmp_odd = ho; Ah.tmp_even = he;
     Al.tmp_full = lf; Al.tmp_odd = lo; Al.tmp_even = le;
+    Am.tmp_full = nullptr; Am.tmp_odd = mo; Am.tmp_even = me;   // Op^dag needs no full temporary
+    // r_out = Op^dag s in FP32 (mixed) or FP64, s an FP64 field; r_out any precision
+    auto dag_residual = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
+        if (!mixed) return Ah.apply_dag(r_out, src64);
+        int e = qkx_spinor_copy(sm, src64);
+        if (e) return e;
+        if (r_out->prec == QKX_PREC_SINGLE) return Am.apply_dag(r_out, sm);
+        if ((e = Am.apply_dag(rm, sm))) return e;
+        return qkx_spinor_copy(r_out, rm);
+    };
 
     std::vector<double> b2, bn2, s2;
     if ((rc = norms(b, b2))) return rc;
@@ -602,7 +626,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     } else {
         if ((rc = qkx_spinor_copy(bh, b))) return rc;
     }
-    if ((rc = Ah.apply_dag(bn, bh))) return rc;
+    if ((rc = dag_residual(bn, bh))) return rc;
     if ((rc = qkx_spinor_zero(xs)) || (xl && (rc = qkx_spinor_zero(xl)))) return rc;
     if ((rc = qkx_spinor_copy(r, bn))) return rc;
     if ((rc = norms(bn, bn2))) return rc;
@@ -667,8 +691,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         if ((rc = write_bslot(E, h))) return rc;
         if (mixed) {
             nrel++;
-            if ((rc = Ah.apply_dag(rh, s))) return rc;
-            if ((rc = qkx_spinor_copy(r, rh))) return rc;
+            if ((rc = dag_residual(r, s))) return rc;
         }
         if ((rc = brestart(E, pp, r, 0))) return rc;
         if ((rc = read_bslot(E, h))) return rc;
