@@ -36,6 +36,22 @@ struct CgFuse {
 };
 int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
            double cR, double gR, double cX, double gX, int g5out, const CgFuse &f, int *nblocks_out);
+// The CG update fused into a batched hop (FP16 batches of 8-16 RHS on FP16
+// links, shared-link kernel): x += alpha p, r -= alpha (hop result), alpha_r
+// = r2cur[r] / pap[r] or 0 where !active[r] or r2cur[r] <= stop2[r]; per-
+// workgroup per-RHS |r|^2 partials into r2_part.  Device pointers.
+struct CgBatchFuse {
+    qkx_spinor_s *x, *r, *p;
+    const double *r2cur, *stop2;
+    const int *active;
+    const double *pap;
+    double *r2_part;
+};
+int cg_fuse_on();   // qkx_set_cg_fuse: 1 when the CG update is fused into the last hop
+bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_spinor_s *x,
+                            const qkx_spinor_s *r);
+int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
+                 const double coef[][4], int g5out, const CgBatchFuse &f, int *nblocks_out);
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);

@@ -598,6 +598,23 @@ template <class SX, class SR, class SP> struct CgFin {
     }
 };
 
+// What a batched hop (hop_share_kernel) does with its result in a CG
+// iteration: alpha_r = |r_r|^2 / (p.Ap)_r (0 for a frozen right-hand side:
+// not active, or |r|^2 at its stopping threshold), x += alpha p and
+// r -= alpha Ap at this (site, RHS), per-workgroup per-RHS |r|^2 partials;
+// Ap itself is never stored.  pap is one row of per-RHS sums.
+template <class SX, class SR, class SP> struct BCgFin {
+    SX x;
+    SR r;
+    SP p;
+    const double *r2cur, *stop2;
+    const int *active;
+    const double *pap;
+    double *r2_part;
+};
+template <class F> struct IsBCg : std::false_type {};
+template <class SX, class SR, class SP> struct IsBCg<BCgFin<SX, SR, SP>> : std::true_type {};
+
 // Minimum waves per SIMD requested from the register allocator for the 16-bit
 // hops (1: no request).  At 24^3x48 a half-lattice launch is 1296 workgroups;
 // residency of 6 waves per SIMD holds all of them at once.
@@ -757,14 +774,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
 // In hop_kernel each thread fetches and reconstructs its site's links itself,
 // and a wave repeats that work for the ~5 sites it covers 12 times over.
 constexpr int kShareSites = 32;   // sites per workgroup at most (nr >= 8, <= 256 threads)
+template <class F> __device__ __forceinline__ double *fin_r2_part(const F &f)
+{
+    if constexpr (IsBCg<F>::value) return f.r2_part;
+    else return nullptr;
+}
 
-template <int B, bool DAG, bool AUX>
-__global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH, SpinorH, Gauge16<false>> a)
+template <int B, bool DAG, bool AUX, class Fin = StoreFin>
+__global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH, SpinorH, Gauge16<false>> a, Fin fin)
 {
     using T = float;
+    constexpr bool CG = IsBCg<Fin>::value;
     if (a.skip && *a.skip) return;   // uniform over the grid
     __shared__ int lsite[kShareSites][8];          // link site of each (site, direction); -1: no site
     __shared__ uint32_t lrow[kShareSites][8][9];   // the link's 9 (re, im) half pairs
+    __shared__ double alpha_s[QKX_MAX_RHS];        // CG step of each right-hand side
+    if constexpr (CG) {
+        const int t = threadIdx.x;
+        if (t < a.nrhs)
+            alpha_s[t] = (!fin.active[t] || fin.r2cur[t] <= fin.stop2[t]) ? 0.0 : fin.r2cur[t] / fin.pap[t];
+    }
     const Geom &g = a.g;
     const int nr = a.nrhs;
     int p, u;
@@ -854,17 +883,42 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
         } else {
             epilogue<B, T>(ep, R, nullptr, res);
         }
-        a.out.store(a.out_off[p] + cb * nr + rr, res);
-        if (a.norm_partial) {
+        if constexpr (CG) {
+            // x += alpha p, r -= alpha Ap for this (site, RHS)
+            const int so = a.out_off[p] + cb * nr + rr;
+            using TX = typename decltype(fin.x)::Real;
+            using TR = typename decltype(fin.r)::Real;
+            Cx<TX> xv[12];
+            Cx<typename decltype(fin.p)::Real> pv[12];
+            fin.x.load(so, xv);
+            fin.p.load(so, pv);
+            const TX ax = TX(alpha_s[rr]);
 #pragma unroll
-            for (int k = 0; k < 12; k++)
-                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+            for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+            fin.x.store(so, xv);
+            Cx<TR> rv[12];
+            fin.r.load(so, rv);
+            const TR ar = TR(alpha_s[rr]);
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                rv[k] = {rv[k].re - ar * TR(res[k].re), rv[k].im - ar * TR(res[k].im)};
+                nrm2 += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+            }
+            fin.r.store(so, rv);
+        } else {
+            a.out.store(a.out_off[p] + cb * nr + rr, res);
+            if (a.norm_partial) {
+#pragma unroll
+                for (int k = 0; k < 12; k++)
+                    nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+            }
         }
     }
-    if (a.norm_partial) {
+    if (CG || a.norm_partial) {
         // one partial per right-hand side, summed in thread order
         const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
-        block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
+        double *dst = CG ? fin_r2_part(fin) : a.norm_partial;
+        block_partial_rhs(nrm2, nr, u0, dst + blockIdx.x * nr);
     }
 }
 
@@ -1090,8 +1144,9 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     // FP16 batches of 8 or more RHS on FP16 links: each workgroup fetches and
     // reconstructs its sites' links once into LDS (hop_share_kernel); its
     // size is a whole number of sites
+    constexpr bool bcg = IsBCg<Fin>::value;   // batched CG update fused (shared links only)
     constexpr bool share_ok = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> &&
-                              std::is_same_v<Fin, StoreFin> && QKX_HOP16_DOT2;
+                              (std::is_same_v<Fin, StoreFin> || bcg) && QKX_HOP16_DOT2;
     const bool share = share_ok && g_share_links && rt == 1 && a.nrhs >= 8 && g_block / a.nrhs <= kShareSites;
     const int bs = share ? a.nrhs * (g_block / a.nrhs) : g_block;   // threads per workgroup
     const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
@@ -1135,14 +1190,22 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     constexpr bool store = std::is_same_v<Fin, StoreFin>;
     constexpr bool tiles = store && B == QKX_BASIS_UKQCD;
     if (!tiles) rt = 1;
-    if (share) {
+    if constexpr (bcg) {
+        // the fused batched update exists in the shared-link kernel only, on an aux field
+        if (!share || !aux) {
+            qkx::set_error("hop: the batched CG update needs shared links (FP16, 8-16 RHS) and an aux field");
+            return QKX_ERR_ARG;
+        }
+        if (dagger) hipLaunchKernelGGL((hop_share_kernel<B, true, true, Fin>), grid, blk, 0, st, a, fin);
+        else hipLaunchKernelGGL((hop_share_kernel<B, false, true, Fin>), grid, blk, 0, st, a, fin);
+    } else if (share) {
         if constexpr (share_ok) {
             if (dagger) {
-                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, true, true>), grid, blk, 0, st, a);
-                else hipLaunchKernelGGL((hop_share_kernel<B, true, false>), grid, blk, 0, st, a);
+                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, true, true>), grid, blk, 0, st, a, fin);
+                else hipLaunchKernelGGL((hop_share_kernel<B, true, false>), grid, blk, 0, st, a, fin);
             } else {
-                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, false, true>), grid, blk, 0, st, a);
-                else hipLaunchKernelGGL((hop_share_kernel<B, false, false>), grid, blk, 0, st, a);
+                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, false, true>), grid, blk, 0, st, a, fin);
+                else hipLaunchKernelGGL((hop_share_kernel<B, false, false>), grid, blk, 0, st, a, fin);
             }
         }
     } else if (rt == 4) {
@@ -1151,12 +1214,14 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
         if constexpr (tiles) QKX_LAUNCH_TILES(3);
     } else if (rt == 2) {
         if constexpr (tiles) QKX_LAUNCH_TILES(2);
-    } else if (depth == 2) {
-        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 2); }
-        else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 2); }
-    } else {
-        if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 1); }
-        else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 1); }
+    } else if constexpr (!bcg) {
+        if (depth == 2) {
+            if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 2); }
+            else        { if (aux) QKX_LAUNCH_HOP(false, true, 2); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 2); }
+        } else {
+            if (dagger) { if (aux) QKX_LAUNCH_HOP(true, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(true, false, 1); }
+            else        { if (aux) QKX_LAUNCH_HOP(false, true, 1); else if constexpr (store) QKX_LAUNCH_HOP(false, false, 1); }
+        }
     }
 #undef QKX_LAUNCH_HOP
 #undef QKX_LAUNCH_TILE
@@ -1288,6 +1353,30 @@ int launch_hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gau
     }
     set_error("hop_cg: unsupported x/r precision combination");
     return QKX_ERR_ARG;
+}
+
+bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_spinor_s *x,
+                            const qkx_spinor_s *r)
+{
+    return QKX_HOP16_DOT2 && g_share_links && in->prec == QKX_PREC_HALF && U->prec == QKX_PREC_HALF &&
+           U->recon == QKX_RECON_12 && in->nrhs >= 8 && in->nrhs <= QKX_MAX_RHS && g_block / in->nrhs <= kShareSites &&
+           x->prec == QKX_PREC_SINGLE && r->prec == QKX_PREC_HALF;
+}
+
+int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
+                 const double coef[][4], int g5out, const CgBatchFuse &f, int *nblocks_out)
+{
+    if (!hop_cg_batch_available(in, U, f.x, f.r) || !aux || f.p->prec != QKX_PREC_HALF ||
+        aux->prec != QKX_PREC_HALF || f.p->nrhs != in->nrhs || f.x->nrhs != in->nrhs || f.r->nrhs != in->nrhs) {
+        set_error("hop_cg_batch: FP16 fields of 8-16 RHS on FP16 links, FP32 x, FP16 r and p");
+        return QKX_ERR_ARG;
+    }
+    Epi e[QKX_MAX_RHS];
+    for (int r = 0; r < in->nrhs; r++) e[r] = Epi{coef[r][0], coef[r][1], coef[r][2], coef[r][3], g5out};
+    BCgFin<SpinorF, SpinorH, SpinorH> fin{make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r),
+                                          make_spinor<SpinorH>(f.p), f.r2cur, f.stop2, f.active, f.pap, f.r2_part};
+    return launch_hop_t<SpinorH, GaugeH12, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, e, nullptr, nblocks_out,
+                                                            fin);
 }
 
 int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,

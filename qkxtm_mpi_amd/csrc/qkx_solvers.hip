@@ -539,6 +539,7 @@ double host_norm2(const qkx_spinor_s *f)
 
 namespace qkx {
 int cg_batch_size = 16;
+int cg_fuse_on() { return g_cg_fuse; }
 }
 
 extern "C" {

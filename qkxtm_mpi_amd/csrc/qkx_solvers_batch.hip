@@ -359,6 +359,27 @@ struct BatchOp {
         if (rc) return rc;
         return qkx::hop_batch(out, tmp_odd, in, U, QKX_BASIS_UKQCD, 0, 0, c2, g5out, part, np);
     }
+    // The second Schur application of the normal operator with the CG update
+    // fused into its last hop (Ap is not stored)
+    int schur_cg(const qkx_spinor_s *in, double sg, int g5out, const qkx::CgBatchFuse &f, int *np) const
+    {
+        double c1[NR][4], c2[NR][4];
+        for (int r = 0; r < NR; r++) {
+            const int q = r < nr ? r : 0;
+            const double ar = sg * a[q];
+            c1[r][0] = 1.0 / nrm[q];
+            c1[r][1] = -ar / nrm[q];
+            c1[r][2] = 0;
+            c1[r][3] = 0;
+            c2[r][0] = -kappa * kappa;
+            c2[r][1] = 0;
+            c2[r][2] = 1.0;
+            c2[r][3] = ar;
+        }
+        int rc = qkx::hop_batch(tmp_odd, in, nullptr, U, QKX_BASIS_UKQCD, 0, 1, c1, 0);
+        if (rc) return rc;
+        return qkx::hop_cg_batch(tmp_odd, in, U, 0, 0, c2, g5out, f, np);
+    }
     // full-lattice D (dag = 0) or D^dag (dag = 1), aux = in
     int dirac(qkx_spinor_s *out, const qkx_spinor_s *in, int dag, double *part = nullptr, int *np = nullptr) const
     {
@@ -440,9 +461,42 @@ int enqueue_bblas(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
     return QKX_SUCCESS;
 }
 
+// One iteration with the update fused into the last hop (even-odd FP16
+// batches of 8-16 RHS): p.Ap from the partials of the first Schur
+// application, summed to one row before the fused hop reads it.
+int enqueue_biter_fused(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p)
+{
+    const int n = entries(p), nr = p->nrhs;
+    qkx::set_hop_skip(&E.cur()->flag);
+    int npap = 0;
+    int rc = A.schur(A.tmp_even, p, +1, 1, g_bpart, &npap);
+    if (rc) {
+        qkx::set_hop_skip(nullptr);
+        return rc;
+    }
+    sum_rows(g_bpart, npap, nr, g_bred);
+    double *r2_part = g_bpart + g_bpart_cap;
+    BSlot *cur = E.cur();
+    const qkx::CgBatchFuse f{x, r, p, cur->r2, cur->stop2, cur->active, g_bred, r2_part};
+    int nr2 = 0;
+    rc = A.schur_cg(A.tmp_even, -1, 1, f, &nr2);
+    qkx::set_hop_skip(nullptr);
+    if (rc) return rc;
+    const double *r2_src = r2_part;
+    reduce_if_many(r2_src, nr2, nr, g_bred + NR);
+    hipLaunchKernelGGL((cgb_dir_kernel<SpinorH, SpinorH>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream,
+                       make_spinor<SpinorH>(p), make_spinor<SpinorH>(r), n, nr, E.cur(), E.next(), r2_src, nr2,
+                       (const double *)g_bred, 1, E.delta2, 0);
+    QKX_HIP(hipGetLastError());
+    E.j++;
+    return QKX_SUCCESS;
+}
+
 int enqueue_biter(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
                   qkx_spinor_s *Ap)
 {
+    if (A.eo && qkx::cg_fuse_on() && p->prec == QKX_PREC_HALF && qkx::hop_cg_batch_available(p, A.U, x, r))
+        return enqueue_biter_fused(E, A, x, r, p);
     qkx::set_hop_skip(&E.cur()->flag);
     int npap = 0;
     int rc = A.apply(Ap, p, g_bpart, &npap);
