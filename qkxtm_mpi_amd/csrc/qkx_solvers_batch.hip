@@ -338,44 +338,38 @@ struct BatchOp {
     bool eo;
     qkx_spinor_s *tmp_full, *tmp_odd, *tmp_even;
 
-    // Schur operator (tm_xx, lib/lattice_util.cpp:2001-2022) with twist sign sg
+    // Coefficients of the two hops of the Schur operator (tm_xx,
+    // lib/lattice_util.cpp:2001-2022) with twist sign sg: the odd
+    // intermediate A^-1 H_oe in, then A in - kappa^2 H_eo (that)
+    void schur_coefs(double sg, double c1[NR][4], double c2[NR][4]) const
+    {
+        for (int r = 0; r < NR; r++) {
+            const int q = r < nr ? r : 0;
+            const double ar = sg * a[q];
+            c1[r][0] = 1.0 / nrm[q];
+            c1[r][1] = -ar / nrm[q];
+            c1[r][2] = 0;
+            c1[r][3] = 0;
+            c2[r][0] = -kappa * kappa;
+            c2[r][1] = 0;
+            c2[r][2] = 1.0;
+            c2[r][3] = ar;
+        }
+    }
     int schur(qkx_spinor_s *out, const qkx_spinor_s *in, double sg, int g5out, double *part = nullptr,
               int *np = nullptr) const
     {
         double c1[NR][4], c2[NR][4];
-        for (int r = 0; r < NR; r++) {
-            const int q = r < nr ? r : 0;
-            const double ar = sg * a[q];
-            c1[r][0] = 1.0 / nrm[q];
-            c1[r][1] = -ar / nrm[q];
-            c1[r][2] = 0;
-            c1[r][3] = 0;
-            c2[r][0] = -kappa * kappa;
-            c2[r][1] = 0;
-            c2[r][2] = 1.0;
-            c2[r][3] = ar;
-        }
+        schur_coefs(sg, c1, c2);
         int rc = qkx::hop_batch(tmp_odd, in, nullptr, U, QKX_BASIS_UKQCD, 0, 1, c1, 0);
         if (rc) return rc;
         return qkx::hop_batch(out, tmp_odd, in, U, QKX_BASIS_UKQCD, 0, 0, c2, g5out, part, np);
     }
-    // The second Schur application of the normal operator with the CG update
-    // fused into its last hop (Ap is not stored)
+    // The same with the CG update fused into the last hop (Ap is not stored)
     int schur_cg(const qkx_spinor_s *in, double sg, int g5out, const qkx::CgBatchFuse &f, int *np) const
     {
         double c1[NR][4], c2[NR][4];
-        for (int r = 0; r < NR; r++) {
-            const int q = r < nr ? r : 0;
-            const double ar = sg * a[q];
-            c1[r][0] = 1.0 / nrm[q];
-            c1[r][1] = -ar / nrm[q];
-            c1[r][2] = 0;
-            c1[r][3] = 0;
-            c2[r][0] = -kappa * kappa;
-            c2[r][1] = 0;
-            c2[r][2] = 1.0;
-            c2[r][3] = ar;
-        }
+        schur_coefs(sg, c1, c2);
         int rc = qkx::hop_batch(tmp_odd, in, nullptr, U, QKX_BASIS_UKQCD, 0, 1, c1, 0);
         if (rc) return rc;
         return qkx::hop_cg_batch(tmp_odd, in, U, 0, 0, c2, g5out, f, np);
