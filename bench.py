@@ -268,6 +268,9 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
         worst = max(worst, max(s.true_residual for s in sts))
         x_last = x if n == 12 else None
         del b
+    # device memory in use after the batched solves: the solver's field pool
+    # keeps its temporaries, so this is the peak of the batched solve
+    mem_used, mem_total = q.mem_info()
     b1 = point_batch(q, L, x0, 0, 1)
     x1 = q.Spinor(L)
     t_single, st1 = timed(q, lambda: q.solve(x1, b1, u, p, eo=True, inner_prec=P, delta=delta, tol=TOL,
@@ -323,6 +326,7 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
         "config": {"workload": f"{'x'.join(map(str, L))} 12 spin-colour point sources, kappa={kap} mu={mu}, "
                                f"batches of {chunk}", "lattice": list(L), "chunk": chunk},
         "batched_iterations": iters, "max_true_residual": worst,
+        "device_memory_after_batch_GB": mem_used / 1e9, "device_memory_total_GB": mem_total / 1e9,
         "single_rhs_mixed_s": t_single, "single_rhs_mixed_x12_s": 12 * t_single,
         "single_rhs_iterations": st1.iterations,
         "fp64_cg_one_rhs_s": t_fp64, "fp64_cg_x12_s": 12 * t_fp64, "fp64_cg_iterations": st64.iterations,

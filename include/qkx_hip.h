@@ -104,6 +104,9 @@ int  qkx_finalize(void);
 const char *qkx_last_error(void);
 int  qkx_device_count(int *n);
 int  qkx_sync(void);
+/* Device memory of the current device, in bytes: in use (by every allocation
+ * on it, the library's field pool included) and total. */
+int  qkx_mem_info(double *used_bytes, double *total_bytes);
 /* HIP events on the library stream, for timing (ms between start and stop). */
 int  qkx_timer_start(void);
 int  qkx_timer_stop(double *ms);

@@ -74,7 +74,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (all return int unless listed in _RESTYPES)
 SIGNATURES = {
-    "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [],
+    "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [], "qkx_mem_info": [_DP, _DP],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
     "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_set_batch_link_sharing": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
@@ -199,6 +199,13 @@ def device_count():
 
 def sync():
     _check(lib().qkx_sync(), "qkx_sync")
+
+
+def mem_info():
+    """(bytes in use, total bytes) of the current device."""
+    used, total = ctypes.c_double(), ctypes.c_double()
+    _check(lib().qkx_mem_info(ctypes.byref(used), ctypes.byref(total)), "qkx_mem_info")
+    return used.value, total.value
 
 
 def params(L, kappa, mu, twistSign=1, boundaryT=-1, tol=1e-10, maxIter=10000, reliableDelta=0.1):

@@ -186,6 +186,16 @@ int qkx_device_count(int *n)
     return QKX_SUCCESS;
 }
 
+int qkx_mem_info(double *used_bytes, double *total_bytes)
+{
+    QKX_CHECK_ARG(used_bytes && total_bytes, "qkx_mem_info: null argument");
+    size_t fr = 0, tot = 0;
+    QKX_HIP(hipMemGetInfo(&fr, &tot));
+    *used_bytes = (double)(tot - fr);
+    *total_bytes = (double)tot;
+    return QKX_SUCCESS;
+}
+
 int qkx_sync(void)
 {
     int rc = qkx::ensure_init();
