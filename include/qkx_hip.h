@@ -97,8 +97,9 @@ typedef struct qkx_gauge_s  qkx_gauge;
 typedef struct qkx_spinor_s qkx_spinor;
 
 /* ---- library --------------------------------------------------------------- */
-/* Selects the device and creates the library stream.  Idempotent.  No
- * reference counterpart (the reference has no device). */
+/* The entries of this section (device, stream, timing, instrumentation, launch
+ * tuning, memory) have no reference counterpart: the reference has no device.
+ * Selects the device and creates the library stream.  Idempotent. */
 int  qkx_init(int device);
 int  qkx_finalize(void);
 const char *qkx_last_error(void);
