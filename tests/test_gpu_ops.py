@@ -166,6 +166,17 @@ def test_upload_download_roundtrip_and_subsets(q):
     assert np.array_equal(got[~odd], a[~odd]) and not np.any(got[odd])
 
 
+def test_mem_info_counts_fields(q):
+    """qkx_mem_info: bytes in use grow by at least a new field's size."""
+    used0, total = q.mem_info()
+    assert 0 < used0 <= total
+    L = (32, 32, 16, 32)      # a size no other test allocates
+    f = q.Spinor(L, nrhs=5)   # 5 x 524288 sites x 192 B = 503 MB
+    used1, _ = q.mem_info()
+    assert used1 - used0 >= 0.9 * 5 * 524288 * 192
+    del f
+
+
 def test_argument_errors(q):
     L = (4, 4, 4, 4)
     with pytest.raises(q.QkxError):
