@@ -28,6 +28,11 @@ class Vector;
 
 enum PARITY { EO, OE, OO, EE };   /* reference include/lattice_util.h:23 */
 
+/* LatticeGeometry, reference include/lattice_util.h:25-76, lib/lattice_util.cpp:44-378,
+ * for one process holding the whole lattice.  pointPlus/pointMinus are the
+ * neighbour tables of lib/lattice_util.cpp:86-99 (periodic; on one process the
+ * ghost-zone redirection of :111-202 never applies).  Fields of one lattice size
+ * share one set of tables, built by the first and freed with the last. */
 class LatticeGeometry {
   public:
     long int V4d;
@@ -40,10 +45,19 @@ class LatticeGeometry {
     int procPos[NDIM];
     int rank;
     int Nprocs;
+
+    int procPlus[NDIM];      /* next / previous process in each direction: 0 */
+    int procMinus[NDIM];
+    int *pointPlus[NDIM];    /* lexicographic index of x + mu, x - mu (periodic) */
+    int *pointMinus[NDIM];
+
     bool initialized;
 
     LatticeGeometry(LatticeInfo *latInfo);
-    virtual ~LatticeGeometry() {}
+    virtual ~LatticeGeometry();
+
+    /* gamma_1 .. gamma_4, gamma_5 in the UKQCD basis, lib/lattice_util.cpp:273-282 */
+    Complex gammaM[5][4][4];
 };
 
 class Vector : public LatticeGeometry {

@@ -12,6 +12,9 @@
 #include "qkxtm/blas_qkxTM.h"
 #include "qkx_hip.h"
 
+#include <cstdlib>
+#include <vector>
+
 using namespace qkxTM;
 
 namespace {
@@ -46,9 +49,79 @@ int parity_code(PARITY p)
     return -99;
 }
 
+// Neighbour tables shared by the fields of one lattice size (the reference
+// builds 8 V ints for every field: 21 MB per Vector at 24^3x48).
+struct NbTables {
+    int L[NDIM];
+    int *plus[NDIM], *minus[NDIM];
+    long refs;
+};
+std::vector<NbTables> g_tables;
+
+bool same_dims(const int a[NDIM], const int b[NDIM])
+{
+    for (int i = 0; i < NDIM; i++)
+        if (a[i] != b[i]) return false;
+    return true;
+}
+
+void tables_acquire(const int L[NDIM], int *plus[NDIM], int *minus[NDIM])
+{
+    for (size_t k = 0; k < g_tables.size(); k++)
+        if (same_dims(g_tables[k].L, L)) {
+            g_tables[k].refs++;
+            for (int m = 0; m < NDIM; m++) {
+                plus[m] = g_tables[k].plus[m];
+                minus[m] = g_tables[k].minus[m];
+            }
+            return;
+        }
+    NbTables t;
+    const long V = (long)L[0] * L[1] * L[2] * L[3];
+    for (int m = 0; m < NDIM; m++) {
+        t.L[m] = L[m];
+        t.plus[m] = (int *)malloc(V * sizeof(int));
+        t.minus[m] = (int *)malloc(V * sizeof(int));
+        if (!t.plus[m] || !t.minus[m]) errorQKXTM("Error out of memory");
+    }
+    t.refs = 1;
+    // stride of direction m in the lexicographic index x + L0 (y + L1 (z + L2 t))
+    long stride[NDIM];
+    stride[0] = 1;
+    for (int m = 1; m < NDIM; m++) stride[m] = stride[m - 1] * L[m - 1];
+    for (long iv = 0; iv < V; iv++)
+        for (int m = 0; m < NDIM; m++) {
+            const int c = (int)((iv / stride[m]) % L[m]);
+            const long up = (c + 1 == L[m]) ? -(long)(L[m] - 1) * stride[m] : stride[m];
+            const long dn = (c == 0) ? (long)(L[m] - 1) * stride[m] : -stride[m];
+            t.plus[m][iv] = (int)(iv + up);
+            t.minus[m][iv] = (int)(iv + dn);
+        }
+    g_tables.push_back(t);
+    for (int m = 0; m < NDIM; m++) {
+        plus[m] = t.plus[m];
+        minus[m] = t.minus[m];
+    }
+}
+
+void tables_release(const int L[NDIM])
+{
+    for (size_t k = 0; k < g_tables.size(); k++)
+        if (same_dims(g_tables[k].L, L)) {
+            if (--g_tables[k].refs == 0) {
+                for (int m = 0; m < NDIM; m++) {
+                    free(g_tables[k].plus[m]);
+                    free(g_tables[k].minus[m]);
+                }
+                g_tables.erase(g_tables.begin() + k);
+            }
+            return;
+        }
+}
+
 } // namespace
 
-// ---- LatticeGeometry: lib/lattice_util.cpp:44-66 (the single-process part) --
+// ---- LatticeGeometry: lib/lattice_util.cpp:44-99, 273-282 (one process) ------
 LatticeGeometry::LatticeGeometry(LatticeInfo *latInfo)
 {
     rank = latInfo->rank < 0 ? 0 : latInfo->rank;
@@ -68,8 +141,38 @@ LatticeGeometry::LatticeGeometry(LatticeInfo *latInfo)
     lV4d = V4d;
     lV3d = V3d;
     Nprocs = 1;
+    for (int i = 0; i < NDIM; i++) procPlus[i] = procMinus[i] = 0;
+    tables_acquire(L, pointPlus, pointMinus);
+    // UKQCD basis: gammaM[0..3] = gamma_1..gamma_4, gammaM[4] = gamma_5
+    // = gamma_1 gamma_2 gamma_3 gamma_4 (the basis the hopping kernels use)
+    for (int g = 0; g < 5; g++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) gammaM[g][a][b] = Complex(0, 0);
+    const Complex I(0, 1), one(1, 0);
+    gammaM[0][0][3] = I;
+    gammaM[0][1][2] = I;
+    gammaM[0][2][1] = -I;
+    gammaM[0][3][0] = -I;
+    gammaM[1][0][3] = one;
+    gammaM[1][1][2] = -one;
+    gammaM[1][2][1] = -one;
+    gammaM[1][3][0] = one;
+    gammaM[2][0][2] = I;
+    gammaM[2][1][3] = -I;
+    gammaM[2][2][0] = -I;
+    gammaM[2][3][1] = I;
+    gammaM[3][0][0] = one;
+    gammaM[3][1][1] = one;
+    gammaM[3][2][2] = -one;
+    gammaM[3][3][3] = -one;
+    gammaM[4][0][2] = one;
+    gammaM[4][1][3] = one;
+    gammaM[4][2][0] = one;
+    gammaM[4][3][1] = one;
     initialized = true;
 }
+
+LatticeGeometry::~LatticeGeometry() { tables_release(L); }
 
 // ---- Vector: lib/lattice_util.cpp:382-477 ----------------------------------
 Vector::Vector(LatticeInfo *latInfo) : LatticeGeometry(latInfo), dev_(0)
