@@ -51,6 +51,17 @@ class LatticeGeometry {
     int *pointPlus[NDIM];    /* lexicographic index of x + mu, x - mu (periodic) */
     int *pointMinus[NDIM];
 
+    /* ghost-zone bookkeeping, lib/lattice_util.cpp:102-207, 392-397, at its
+     * one-process values: no ghost zone (start 0, NtotalBoundary 0, pointers
+     * null; the reference's Vector, Gauge and Propagator each declare the two
+     * pointer arrays, here they sit in the common base) */
+    int placeBoundaryStartPlus[NDIM];
+    int placeBoundaryStartMinus[NDIM];
+    int lSurfaceBoundary[NDIM];
+    int NtotalBoundary;
+    void *ptr_boundaryPlusStart[NDIM];
+    void *ptr_boundaryMinusStart[NDIM];
+
     bool initialized;
 
     LatticeGeometry(LatticeInfo *latInfo);

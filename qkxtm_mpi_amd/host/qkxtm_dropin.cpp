@@ -141,7 +141,13 @@ LatticeGeometry::LatticeGeometry(LatticeInfo *latInfo)
     lV4d = V4d;
     lV3d = V3d;
     Nprocs = 1;
-    for (int i = 0; i < NDIM; i++) procPlus[i] = procMinus[i] = 0;
+    for (int i = 0; i < NDIM; i++) {
+        procPlus[i] = procMinus[i] = 0;
+        placeBoundaryStartPlus[i] = placeBoundaryStartMinus[i] = 0;
+        lSurfaceBoundary[i] = (int)(V4d / L[i]);
+        ptr_boundaryPlusStart[i] = ptr_boundaryMinusStart[i] = 0;
+    }
+    NtotalBoundary = 0;
     tables_acquire(L, pointPlus, pointMinus);
     // UKQCD basis: gammaM[0..3] = gamma_1..gamma_4, gammaM[4] = gamma_5
     // = gamma_1 gamma_2 gamma_3 gamma_4 (the basis the hopping kernels use)
