@@ -23,6 +23,7 @@ typedef std::complex<double> Complex;
 #define NSPINS 4
 #define NCOLORS 3
 #define NDF 2
+#define MOM_MAX 1000
 #define NSPINOR (NSPINS * NCOLORS * NDF)
 #define NLINKS (NDIM * NCOLORS * NCOLORS * NDF)
 #define NPROPS (NSPINS * NSPINS * NCOLORS * NCOLORS * NDF)
@@ -31,6 +32,11 @@ typedef std::complex<double> Complex;
 #define MG(v, color1, color2) ((v) * NCOLORS * NCOLORS + (color1) * NCOLORS + color2)
 #define MU(color1, color2) ((color1) * NCOLORS + color2)
 #define LEXIC(it, iz, iy, ix, L) ((it) * L[0] * L[1] * L[2] + (iz) * L[0] * L[1] + (iy) * L[0] + (ix))
+/* lexicographic indices within the 3-d faces (ghost-zone order of the reference) */
+#define LEXIC_TZY(it, iz, iy, L) ((it) * L[1] * L[2] + (iz) * L[1] + (iy))
+#define LEXIC_TZX(it, iz, ix, L) ((it) * L[0] * L[2] + (iz) * L[0] + (ix))
+#define LEXIC_TYX(it, iy, ix, L) ((it) * L[0] * L[1] + (iy) * L[0] + (ix))
+#define LEXIC_ZYX(iz, iy, ix, L) ((iz) * L[0] * L[1] + (iy) * L[0] + (ix))
 
 /* The reference aborts the MPI job; one GPU process exits. */
 #define errorQKXTM(...)               \
