@@ -166,6 +166,14 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src);
 /* Device -> host AoS (reference layout), FP64 recon-18 fields; the links as
  * stored (the boundaryT = -1 sign is applied by the kernels, not stored). */
 int qkx_gauge_download(const qkx_gauge *g, double *U_aos);
+/* The largest |row2 - conj(row0 x row1)| over all links of an FP64 18-real
+ * field: how far a 12-real copy (row 2 rebuilt) would move any link.  Below
+ * 1e-14 the solvers (qkx_solve, qkx_solve_batch) run their internal FP64
+ * operator on such a copy (25% fewer bytes per hop); the reported true
+ * residual always uses the field itself.  Links that carry the boundary sign
+ * or are not SU(3) give O(1).  Measured once per change of the links.  No
+ * reference counterpart. */
+int qkx_gauge_su3_deviation(const qkx_gauge *g, double *dev);
 
 /* Vector::Vector lib/lattice_util.cpp:382-413 */
 int qkx_spinor_create(qkx_spinor **s, const int L[4], int prec, int subset);

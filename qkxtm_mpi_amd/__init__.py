@@ -76,7 +76,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [], "qkx_mem_info": [_DP, _DP],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
-    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_set_batch_link_sharing": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
+    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_gauge_su3_deviation": [_P, _DP], "qkx_set_batch_link_sharing": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
     "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
@@ -246,6 +246,12 @@ class Gauge:
         U = np.empty((volume(self.L), 4, 3, 3), dtype=np.complex128)
         _check(lib().qkx_gauge_download(self.h, _ptr(U)), "qkx_gauge_download")
         return U
+
+    def su3_deviation(self):
+        """max |row2 - conj(row0 x row1)| over the links (FP64 18-real fields)."""
+        d = ctypes.c_double()
+        _check(lib().qkx_gauge_su3_deviation(self.h, ctypes.byref(d)), "qkx_gauge_su3_deviation")
+        return d.value
 
     def read_lime(self, path):
         """Links from an ILDG/LIME file (qkxTM_MPI_getGaugeLime, lib/read_conf_lime.cpp:111)."""

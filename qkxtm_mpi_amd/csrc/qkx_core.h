@@ -47,6 +47,12 @@ struct CgBatchFuse {
     const double *pap;
     double *r2_part;
 };
+// max over links of |row2 - conj(row0 x row1)| of an FP64 18-real field,
+// measured once per generation of its links
+int gauge_recon_deviation(const qkx_gauge_s *U, double *dev);
+// the field the solvers use for their internal FP64 operator: U itself, or
+// its cached FP64 12-real copy while U's links are SU(3) to 1e-14
+int gauge_fp64_operator(const qkx_gauge_s *U, const qkx_gauge_s **op);
 int cg_fuse_on();   // qkx_set_cg_fuse: 1 when the CG update is fused into the last hop
 bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_spinor_s *x,
                             const qkx_spinor_s *r);

@@ -227,6 +227,35 @@ void gauge_modified(qkx_gauge_s *g) { g->gen++; }
 
 void *stage_buffer(size_t bytes) { return stage(bytes); }
 
+// Per-workgroup max over (site, mu) of |row2 - conj(row0 x row1)|, the
+// reconstruction the 12-real fields use; a NaN anywhere gives NaN.
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+__global__ __launch_bounds__(256) void recon_dev_kernel(GaugeD18 G, int V, double *part)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0;
+    if (t < 4 * V) {
+        const int mu = t / V, s = t - mu * V;
+        Cx<double> u[9], r[9];
+        G.load(mu, s, u);
+#pragma unroll
+        for (int k = 0; k < 6; k++) r[k] = u[k];
+        recon_row2(r);
+#pragma unroll
+        for (int k = 6; k < 9; k++) m = max_nan(max_nan(m, fabs(u[k].re - r[k].re)), fabs(u[k].im - r[k].im));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
+    __shared__ double lds[4];
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = lds[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = max_nan(v, lds[w]);
+        part[blockIdx.x] = v;
+    }
+}
+
 int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon)
 {
     int rc;
@@ -240,6 +269,47 @@ int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon)
         return QKX_SUCCESS;
     }
     return qkx_gauge_convert(*cache, U);   // records U->gen
+}
+
+int gauge_recon_deviation(const qkx_gauge_s *U, double *dev)
+{
+    if (U->prec != QKX_PREC_DOUBLE || U->recon != QKX_RECON_18) {
+        set_error("gauge_recon_deviation: FP64 18-real fields only");
+        return QKX_ERR_ARG;
+    }
+    qkx_gauge_s *Um = const_cast<qkx_gauge_s *>(U);
+    if (Um->recon_gen != U->gen) {
+        const int n = 4 * U->V, nb = (n + 255) / 256;
+        double *part = nullptr;
+        QKX_HIP(hipMalloc(&part, nb * sizeof(double)));
+        hipLaunchKernelGGL(recon_dev_kernel, dim3(nb), dim3(256), 0, g_stream, make_gauge<GaugeD18>(U), U->V, part);
+        std::vector<double> h(nb);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, g_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+        (void)hipFree(part);
+        QKX_HIP(e);
+        double m = 0;
+        for (double v : h) m = v > m || v != v ? v : m;   // a NaN link propagates
+        Um->recon_dev = m;
+        Um->recon_gen = U->gen;
+    }
+    *dev = U->recon_dev;
+    return QKX_SUCCESS;
+}
+
+int gauge_fp64_operator(const qkx_gauge_s *U, const qkx_gauge_s **op)
+{
+    *op = U;
+    if (U->prec != QKX_PREC_DOUBLE || U->recon != QKX_RECON_18) return QKX_SUCCESS;
+    double dev = 0;
+    int rc = gauge_recon_deviation(U, &dev);
+    if (rc) return rc;
+    if (!(dev <= 1e-14)) return QKX_SUCCESS;
+    qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
+    if ((rc = gauge_cached(&Uc->outer, U, QKX_PREC_DOUBLE, QKX_RECON_12))) return rc;
+    *op = Uc->outer;
+    return QKX_SUCCESS;
 }
 
 // zero the sites of one parity of a full field
@@ -421,8 +491,11 @@ int qkx_gauge_create(qkx_gauge **out, const int L[4], int prec, int recon, int b
     g->bc = boundaryT;
     g->inner = nullptr;
     g->mid = nullptr;
+    g->outer = nullptr;
     g->gen = 0;
     g->src_gen = 0;
+    g->recon_dev = -1.0;
+    g->recon_gen = ~0u;   // not measured
     size_t per = (prec == QKX_PREC_DOUBLE ? 16 : prec == QKX_PREC_SINGLE ? 8 : 4) * (recon == 18 ? 9 : 6);
     g->bytes = per * 4 * (size_t)g->V;
     if (hipMalloc(&g->data, g->bytes) != hipSuccess) {
@@ -440,6 +513,7 @@ int qkx_gauge_destroy(qkx_gauge *g)
     (void)hipStreamSynchronize(qkx::g_stream);
     if (g->inner) qkx_gauge_destroy(g->inner);
     if (g->mid) qkx_gauge_destroy(g->mid);
+    if (g->outer) qkx_gauge_destroy(g->outer);
     if (g->data) (void)hipFree(g->data);
     delete g;
     return QKX_SUCCESS;
@@ -486,6 +560,12 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src)
     qkx::gauge_modified(dst);
     dst->src_gen = src->gen;
     return QKX_SUCCESS;
+}
+
+int qkx_gauge_su3_deviation(const qkx_gauge *g, double *dev)
+{
+    QKX_CHECK_ARG(g && dev, "qkx_gauge_su3_deviation: null argument");
+    return qkx::gauge_recon_deviation(g, dev);
 }
 
 int qkx_gauge_download(const qkx_gauge *g, double *U)

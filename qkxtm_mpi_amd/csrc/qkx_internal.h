@@ -548,6 +548,13 @@ struct qkx_gauge_s {
     qkx_gauge_s *inner;
     // cached FP32 12-real copy for the reliable updates of the mixed solvers (owned)
     qkx_gauge_s *mid;
+    // cached FP64 12-real copy for the solvers' internal FP64 operator, used
+    // only while the links are SU(3) to 1e-14 (owned)
+    qkx_gauge_s *outer;
+    // max over links of |row2 - conj(row0 x row1)| (what a 12-real copy would
+    // change), and the generation it was measured at (~0u: never)
+    double recon_dev;
+    unsigned recon_gen;
     // generation of the links: bumped by every write (upload, conversion into
     // this field, smearing, projection); a cached copy records the generation
     // of the field it was converted from and is refreshed when they differ

@@ -27,6 +27,7 @@
 #include <chrono>
 #include <memory>
 #include <cmath>
+#include <string>
 #include <vector>
 
 namespace {
@@ -670,8 +671,26 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     const int sub = o->eo ? QKX_SUBSET_EVEN : QKX_SUBSET_FULL;
     int rc;
 
+    // FP64 operator of the checks, the source and the reconstruction (and of
+    // a pure FP64 solve): U, or its cached 12-real copy while U is SU(3) to
+    // 1e-14; the reported true residual is always formed with U itself
+    const qkx_gauge_s *Uop = U;
+    if ((rc = qkx::gauge_fp64_operator(U, &Uop))) return rc;
+    // the low-precision copies store 12 reals (row 2 rebuilt): links that are
+    // not SU(3) -- e.g. carrying the boundary sign -- would give an inner
+    // operator other than U's
+    if (mixed && !U_inner_arg) {
+        double dev = 0;
+        if ((rc = qkx::gauge_recon_deviation(U, &dev))) return rc;
+        if (!(dev <= 1e-6)) {
+            qkx::set_error("qkx_solve: mixed precision stores the links as 12 reals (row 2 rebuilt) and these links are "
+                           "not SU(3) (deviation " + std::to_string(dev) + "); upload them without the boundary "
+                           "sign with boundaryT = -1, or solve in FP64");
+            return QKX_ERR_ARG;
+        }
+    }
     // inner gauge: caller's, or a cached conversion of U
-    const qkx_gauge_s *Ui = U;
+    const qkx_gauge_s *Ui = Uop;
     if (mixed) {
         if (U_inner_arg) {
             Ui = U_inner_arg;
@@ -689,7 +708,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
 
     qkx::Temp hf(L, QKX_PREC_DOUBLE, QKX_SUBSET_FULL), ho(L, QKX_PREC_DOUBLE, QKX_SUBSET_ODD),
         he(L, QKX_PREC_DOUBLE, QKX_SUBSET_EVEN);
-    NormalOp Ah{U, kappa, a, nrm, o->eo != 0, hf.f, ho.f, he.f};
+    NormalOp Ah{Uop, kappa, a, nrm, o->eo != 0, hf.f, ho.f, he.f};
     qkx::Temp lf(L, lp, QKX_SUBSET_FULL), lo_(L, lp, QKX_SUBSET_ODD), le(L, lp, QKX_SUBSET_EVEN);
     NormalOp Al{Ui, kappa, a, nrm, o->eo != 0, lf.f, lo_.f, le.f};
 
@@ -747,7 +766,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         // bh_e = b_e + kappa H_eo A^-1 b_o
         if ((rc = qkx_spinor_copy(bh.f, b)) || (rc = qkx_spinor_copy(bo.f, b))) return rc;
         if ((rc = qkx::site(to.f, bo.f, nullptr, QKX_BASIS_UKQCD, -1, 1.0 / nrm, -a / nrm, 0, 0, 0))) return rc;
-        if ((rc = qkx::hop(bh.f, to.f, bh.f, U, QKX_BASIS_UKQCD, 0, 0, kappa, 0, 1.0, 0, 0))) return rc;
+        if ((rc = qkx::hop(bh.f, to.f, bh.f, Uop, QKX_BASIS_UKQCD, 0, 0, kappa, 0, 1.0, 0, 0))) return rc;
     } else {
         if ((rc = qkx_spinor_copy(bh.f, b))) return rc;
     }
@@ -841,7 +860,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     // assemble the full solution
     if (o->eo) {
         // x_o = A^-1 (b_o + kappa H_oe x_e)
-        if ((rc = qkx::hop(to.f, xs.f, bo.f, U, QKX_BASIS_UKQCD, 0, 1, kappa / nrm, -a * kappa / nrm, 1.0 / nrm,
+        if ((rc = qkx::hop(to.f, xs.f, bo.f, Uop, QKX_BASIS_UKQCD, 0, 1, kappa / nrm, -a * kappa / nrm, 1.0 / nrm,
                            -a / nrm, 0)))
             return rc;
         if ((rc = qkx_spinor_copy(x, xs.f)) || (rc = qkx_spinor_copy(x, to.f))) return rc;

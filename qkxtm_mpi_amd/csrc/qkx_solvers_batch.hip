@@ -16,6 +16,7 @@
 #include "qkx_core.h"
 #include <chrono>
 #include <cmath>
+#include <string>
 #include <vector>
 
 namespace {
@@ -567,7 +568,25 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     const int sub = o->eo ? QKX_SUBSET_EVEN : QKX_SUBSET_FULL;
     int rc;
 
-    const qkx_gauge_s *Ui = U;
+    // FP64 operator of the checks, the source and the reconstruction (and of
+    // a pure FP64 solve): U, or its cached 12-real copy while U is SU(3) to
+    // 1e-14; the reported true residuals are always formed with U itself
+    const qkx_gauge_s *Uop = U;
+    if ((rc = qkx::gauge_fp64_operator(U, &Uop))) return rc;
+    // the low-precision copies store 12 reals (row 2 rebuilt): links that are
+    // not SU(3) -- e.g. carrying the boundary sign -- would give an inner
+    // operator other than U's
+    if (mixed && !U_inner_arg) {
+        double dev = 0;
+        if ((rc = qkx::gauge_recon_deviation(U, &dev))) return rc;
+        if (!(dev <= 1e-6)) {
+            qkx::set_error("qkx_solve_batch: mixed precision stores the links as 12 reals (row 2 rebuilt) and these links are "
+                           "not SU(3) (deviation " + std::to_string(dev) + "); upload them without the boundary "
+                           "sign with boundaryT = -1, or solve in FP64");
+            return QKX_ERR_ARG;
+        }
+    }
+    const qkx_gauge_s *Ui = Uop;
     if (mixed) {
         if (U_inner_arg) {
             Ui = U_inner_arg;
@@ -591,7 +610,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     }
 
     BatchOp Ah{}, Al{}, Am{};
-    Ah.U = U;
+    Ah.U = Uop;
     Al.U = Ui;
     Am.U = Um;
     Ah.kappa = Al.kappa = Am.kappa = p->kappa;
@@ -670,7 +689,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         }
         if ((rc = qkx_spinor_copy(bh, b)) || (rc = qkx_spinor_copy(bo, b))) return rc;
         if ((rc = qkx::site_batch(to, bo, nullptr, QKX_BASIS_UKQCD, -1, c, 0))) return rc;
-        if ((rc = qkx::hop(bh, to, bh, U, QKX_BASIS_UKQCD, 0, 0, p->kappa, 0, 1.0, 0, 0))) return rc;
+        if ((rc = qkx::hop(bh, to, bh, Uop, QKX_BASIS_UKQCD, 0, 0, p->kappa, 0, 1.0, 0, 0))) return rc;
     } else {
         if ((rc = qkx_spinor_copy(bh, b))) return rc;
     }
@@ -757,7 +776,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
             c[q][2] = 1.0 / nz;
             c[q][3] = -ar / nz;
         }
-        if ((rc = qkx::hop_batch(to, xs, bo, U, QKX_BASIS_UKQCD, 0, 1, c, 0))) return rc;
+        if ((rc = qkx::hop_batch(to, xs, bo, Uop, QKX_BASIS_UKQCD, 0, 1, c, 0))) return rc;
         if ((rc = qkx_spinor_copy(x, xs)) || (rc = qkx_spinor_copy(x, to))) return rc;
     } else {
         if ((rc = qkx_spinor_copy(x, xs))) return rc;
@@ -765,6 +784,7 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     // true residuals of the full system, FP64
     {
         BatchOp F = Ah;
+        F.U = U;
         F.eo = false;
         if ((rc = F.dirac(tf, x, 0))) return rc;
         if ((rc = qkx::axpby_dev(tf, b, -1.0, 1.0, nullptr, 0))) return rc;

@@ -179,3 +179,41 @@ def test_graph_replay_identical_to_eager(q):
     for key, v in out.items():
         assert np.array_equal(v[0], ref[0]) and v[1:3] == ref[1:3], key
         assert np.array_equal(v[3], ref[3]) and v[4] == ref[4], key
+
+
+def test_fp64_operator_copy_only_for_su3_links(q):
+    """The solvers' internal FP64 operator runs on a 12-real copy only while
+    every link is SU(3) to 1e-14 (row 2 = conj(row0 x row1)).  Links that
+    carry the boundary sign fail that test, APE-smeared links (projected to
+    about 1e-10) fail it too.  The same system solved with signed links
+    (18-real path) and with unsigned links and boundaryT = -1 (12-real path)
+    gives the same solution.  Mixed precision stores 12 reals and refuses
+    links that are not SU(3)."""
+    L, kappa, mu = (8, 8, 8, 16), 0.1632, 0.004
+    U = q.gen_gauge(L, 0.3, 3)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    assert u.su3_deviation() <= 1e-14
+    us = q.Gauge(L, boundaryT=+1, host=orc.apply_bc(U, L, -1))
+    assert us.su3_deviation() > 1.0
+    ape = q.Gauge(L, boundaryT=-1)
+    ape.APE_smearing(u, 5, 0.5)
+    assert 1e-14 < ape.su3_deviation() < 1e-6
+    b = q.gen_z2(L, 4)
+    sols = []
+    for g, bt in ((u, -1), (us, +1)):
+        p = q.params(L, kappa, mu, 1, bt)
+        x, bs = q.Spinor(L), q.Spinor(L, host=b)
+        print(f"FP64 solve boundaryT={bt} deviation={g.su3_deviation():.2e}", flush=True)
+        st = q.solve(x, bs, g, p, eo=True, inner_prec=q.DOUBLE, tol=1e-10)
+        assert st.converged and st.true_residual <= 1e-10
+        sols.append(x.download())
+    assert rel(sols[0], sols[1]) <= 1e-8
+    # mixed precision stores 12 reals: signed links are refused with a reason,
+    # smeared links (SU(3) to ~1e-10) are accepted
+    with pytest.raises(q.QkxError, match="not SU\\(3\\)"):
+        q.solve(q.Spinor(L), q.Spinor(L, host=b), us, q.params(L, kappa, mu, 1, +1), eo=True, inner_prec=q.HALF,
+                delta=0.01, tol=1e-10)
+    x = q.Spinor(L)
+    st = q.solve(x, q.Spinor(L, host=b), ape, q.params(L, kappa, mu, 1, -1), eo=True, inner_prec=q.HALF,
+                 delta=0.01, tol=1e-10)
+    assert st.converged and st.true_residual <= 1e-10
