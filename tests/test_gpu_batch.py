@@ -251,3 +251,27 @@ def test_batched_fused_update_matches_unfused(q):
     assert abs(its[0] - its[1]) <= 2
     for r in range(12):
         assert rel(sols[1][r], sols[0][r]) <= 1e-8, r
+
+
+def test_multi_mu_batch_of_8_on_the_fused_path(q):
+    """Eight twisted masses on one gauge field (8 RHS: shared links and the
+    fused CG update): every member meets its true residual and agrees with
+    the single-RHS solve of its own mass."""
+    L, kappa = (8, 8, 8, 16), 0.1632
+    mus = [0.004, 0.006, 0.01, 0.015, 0.02, 0.03, 0.05, 0.08]
+    U = q.gen_gauge(L, 0.3, 9)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    src = q.gen_z2(L, 6)
+    print("multi-mu batch of 8 on the fused path", flush=True)
+    b = q.Spinor(L, host=np.stack([src] * 8), nrhs=8)
+    x = q.Spinor(L, nrhs=8)
+    st = q.solve_batch(x, b, u, q.params(L, kappa, mus[0], 1, -1), mu=mus, eo=True, inner_prec=q.HALF,
+                       delta=0.01, tol=1e-10)
+    assert all(s.converged and s.true_residual <= 1e-10 for s in st)
+    got = x.download()
+    for r in (0, 3, 7):
+        xs, bs = q.Spinor(L), q.Spinor(L, host=src)
+        s1 = q.solve(xs, bs, u, q.params(L, kappa, mus[r], 1, -1), eo=True, inner_prec=q.HALF, delta=0.01,
+                     tol=1e-10)
+        assert s1.converged
+        assert rel(got[r], xs.download()) <= 1e-8, r
