@@ -130,10 +130,12 @@ int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
  * index order.  Results per site are identical; reductions sum in another
  * order. */
 int  qkx_set_site_order(int blocked);
-/* 1 (default): hops on batches of 8-16 FP16 right-hand sides with FP16 links
- * fetch and reconstruct each site's 8 links once per workgroup, into LDS,
- * for all the site's right-hand sides; 0: every (site, RHS) thread does it
- * itself.  The results are identical. */
+/* Hops on batches of 3-16 FP16 right-hand sides with FP16 links can fetch
+ * and reconstruct each site's 8 links once per workgroup, into LDS, for all
+ * the site's right-hand sides (and fuse the batched CG update into the last
+ * hop).  1 (default): when the launch holds at least 4 waves per SIMD of
+ * (site, RHS) units; 2: always; 0: never (every thread fetches its links).
+ * The hop results are identical. */
 int  qkx_set_batch_link_sharing(int on);
 /* CG iterations enqueued between host reads of the solver state (default 16;
  * 1 gives no launches past the stopping iteration). */

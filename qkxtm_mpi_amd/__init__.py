@@ -667,9 +667,10 @@ def set_rhs_tile(rt=0):
     _check(lib().qkx_set_rhs_tile(int(rt)), "qkx_set_rhs_tile")
 
 
-def set_batch_link_sharing(on=True):
-    """Share each site's links between the RHS threads of FP16 batches (default on)."""
-    _check(lib().qkx_set_batch_link_sharing(int(on)), "qkx_set_batch_link_sharing")
+def set_batch_link_sharing(mode=1):
+    """Links shared between the RHS threads of FP16 batches: 0 never, 1 when the
+    launch is large enough (default), 2 always."""
+    _check(lib().qkx_set_batch_link_sharing(int(mode)), "qkx_set_batch_link_sharing")
 
 
 def set_site_order(blocked=True):

@@ -281,7 +281,7 @@ int qkx_set_cg_batch(int iterations)
 
 int qkx_set_batch_link_sharing(int on)
 {
-    QKX_CHECK_ARG(on == 0 || on == 1, "qkx_set_batch_link_sharing: 0 or 1");
+    QKX_CHECK_ARG(on >= 0 && on <= 2, "qkx_set_batch_link_sharing: 0, 1 or 2");
     qkx::set_batch_link_sharing(on);
     return QKX_SUCCESS;
 }

@@ -773,21 +773,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
 // direction), into LDS; every (site, RHS) thread then reads them from there.
 // In hop_kernel each thread fetches and reconstructs its site's links itself,
 // and a wave repeats that work for the ~5 sites it covers 12 times over.
-constexpr int kShareSites = 32;   // sites per workgroup at most (nr >= 8, <= 256 threads)
+// Sites per workgroup at most: 32 for 8-16 RHS (12 KB of LDS), 85 for 3-7 RHS
+// (29 KB: still 5 workgroups of 4 waves per CU, the register-limited
+// occupancy).  Two RHS would need 128 sites and 43 KB, which costs occupancy.
+constexpr int kShareSites = 32;
+constexpr int kShareSitesFew = 85;
+constexpr int kShareMinRhs = 3;
 template <class F> __device__ __forceinline__ double *fin_r2_part(const F &f)
 {
     if constexpr (IsBCg<F>::value) return f.r2_part;
     else return nullptr;
 }
 
-template <int B, bool DAG, bool AUX, class Fin = StoreFin>
+template <int B, bool DAG, bool AUX, class Fin = StoreFin, int SITES = kShareSites>
 __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH, SpinorH, Gauge16<false>> a, Fin fin)
 {
     using T = float;
     constexpr bool CG = IsBCg<Fin>::value;
     if (a.skip && *a.skip) return;   // uniform over the grid
-    __shared__ int lsite[kShareSites][8];          // link site of each (site, direction); -1: no site
-    __shared__ uint32_t lrow[kShareSites][8][9];   // the link's 9 (re, im) half pairs
+    __shared__ int lsite[SITES][8];          // link site of each (site, direction); -1: no site
+    __shared__ uint32_t lrow[SITES][8][9];   // the link's 9 (re, im) half pairs
     __shared__ double alpha_s[QKX_MAX_RHS];        // CG step of each right-hand side
     if constexpr (CG) {
         const int t = threadIdx.x;
@@ -1083,7 +1088,15 @@ int g_block = 256;
 int g_depth64 = 2;   // pipeline depth of the FP64 hop (2 measured faster than 1 at 24^3x48)
 int g_depth32 = 2;   // pipeline depth of the FP32/FP16/BF16 hops
 int g_rhs_tile = 0;  // right-hand sides per thread in batched hops (0 or 1: one)
-int g_share_links = 1;  // FP16 batches of >= 8 RHS: links shared per workgroup through LDS
+int g_share_links = 1;  // FP16 batches of >= 3 RHS share links through LDS: 0 off, 1 automatic, 2 always
+// Automatic sharing needs at least 4 waves per SIMD of (site, RHS) units (MI355X:
+// 1024 SIMDs): in smaller launches the extra barrier phase costs more than it saves.
+constexpr long kShareMinUnits = 4L * 1024 * 64;
+bool share_links_for(int nrhs, long units)
+{
+    if (!g_share_links || nrhs < kShareMinRhs || g_block / nrhs > kShareSitesFew) return false;
+    return g_share_links == 2 || units >= kShareMinUnits;
+}
 int g_site_order = 0;   // 1: blocked walk (HopTrav), 0 (default): checkerboard index order
 const int *g_skip = nullptr;
 
@@ -1147,9 +1160,10 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     constexpr bool bcg = IsBCg<Fin>::value;   // batched CG update fused (shared links only)
     constexpr bool share_ok = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> &&
                               (std::is_same_v<Fin, StoreFin> || bcg) && QKX_HOP16_DOT2;
-    const bool share = share_ok && g_share_links && rt == 1 && a.nrhs >= 8 && g_block / a.nrhs <= kShareSites;
-    const int bs = share ? a.nrhs * (g_block / a.nrhs) : g_block;   // threads per workgroup
     const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
+    const bool share = share_ok && rt == 1 && share_links_for(a.nrhs, (long)sites_out * a.nrhs);
+    const bool few = a.nrhs < 8;   // 3-7 RHS: the larger site capacity
+    const int bs = share ? a.nrhs * (g_block / a.nrhs) : g_block;   // threads per workgroup
     int nb = (sites_out * upsite + bs - 1) / bs;
     a.nblk8 = 0;
     if (g_xcd_remap && nb >= 64) {
@@ -1196,17 +1210,25 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
             qkx::set_error("hop: the batched CG update needs shared links (FP16, 8-16 RHS) and an aux field");
             return QKX_ERR_ARG;
         }
-        if (dagger) hipLaunchKernelGGL((hop_share_kernel<B, true, true, Fin>), grid, blk, 0, st, a, fin);
-        else hipLaunchKernelGGL((hop_share_kernel<B, false, true, Fin>), grid, blk, 0, st, a, fin);
+        if (few) {
+            if (dagger) hipLaunchKernelGGL((hop_share_kernel<B, true, true, Fin, kShareSitesFew>), grid, blk, 0, st, a, fin);
+            else hipLaunchKernelGGL((hop_share_kernel<B, false, true, Fin, kShareSitesFew>), grid, blk, 0, st, a, fin);
+        } else {
+            if (dagger) hipLaunchKernelGGL((hop_share_kernel<B, true, true, Fin>), grid, blk, 0, st, a, fin);
+            else hipLaunchKernelGGL((hop_share_kernel<B, false, true, Fin>), grid, blk, 0, st, a, fin);
+        }
     } else if (share) {
         if constexpr (share_ok) {
-            if (dagger) {
-                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, true, true>), grid, blk, 0, st, a, fin);
-                else hipLaunchKernelGGL((hop_share_kernel<B, true, false>), grid, blk, 0, st, a, fin);
+#define QKX_LAUNCH_SHARE(DG, AX, NS) \
+    hipLaunchKernelGGL((hop_share_kernel<B, DG, AX, StoreFin, NS>), grid, blk, 0, st, a, fin)
+            if (few) {
+                if (dagger) { if (aux) QKX_LAUNCH_SHARE(true, true, kShareSitesFew); else QKX_LAUNCH_SHARE(true, false, kShareSitesFew); }
+                else        { if (aux) QKX_LAUNCH_SHARE(false, true, kShareSitesFew); else QKX_LAUNCH_SHARE(false, false, kShareSitesFew); }
             } else {
-                if (aux) hipLaunchKernelGGL((hop_share_kernel<B, false, true>), grid, blk, 0, st, a, fin);
-                else hipLaunchKernelGGL((hop_share_kernel<B, false, false>), grid, blk, 0, st, a, fin);
+                if (dagger) { if (aux) QKX_LAUNCH_SHARE(true, true, kShareSites); else QKX_LAUNCH_SHARE(true, false, kShareSites); }
+                else        { if (aux) QKX_LAUNCH_SHARE(false, true, kShareSites); else QKX_LAUNCH_SHARE(false, false, kShareSites); }
             }
+#undef QKX_LAUNCH_SHARE
         }
     } else if (rt == 4) {
         if constexpr (tiles) QKX_LAUNCH_TILES(4);
@@ -1358,8 +1380,9 @@ int launch_hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gau
 bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_spinor_s *x,
                             const qkx_spinor_s *r)
 {
-    return QKX_HOP16_DOT2 && g_share_links && in->prec == QKX_PREC_HALF && U->prec == QKX_PREC_HALF &&
-           U->recon == QKX_RECON_12 && in->nrhs >= 8 && in->nrhs <= QKX_MAX_RHS && g_block / in->nrhs <= kShareSites &&
+    // the fused update is an even-odd hop: Vh output sites
+    return QKX_HOP16_DOT2 && in->prec == QKX_PREC_HALF && U->prec == QKX_PREC_HALF && U->recon == QKX_RECON_12 &&
+           in->nrhs <= QKX_MAX_RHS && share_links_for(in->nrhs, (long)in->Vh * in->nrhs) &&
            x->prec == QKX_PREC_SINGLE && r->prec == QKX_PREC_HALF;
 }
 

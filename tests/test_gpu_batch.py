@@ -189,9 +189,10 @@ def test_solve_batch_propagator_point_sources(q, cfg1):
         assert st[sc].converged and res <= 1e-10, f"spin-colour {sc}: {res:.2e}"
 
 
-@pytest.mark.parametrize("L,nrhs", [(LS, 12), (LS, 8), ((8, 6, 4, 8), 9), ((8, 8, 8, 16), 12), ((8, 8, 8, 16), 16)])
+@pytest.mark.parametrize("L,nrhs", [(LS, 12), (LS, 8), ((8, 6, 4, 8), 9), ((8, 8, 8, 16), 12), ((8, 8, 8, 16), 16),
+                                    (LS, 3), ((8, 6, 4, 8), 5), ((8, 8, 8, 16), 7)])
 def test_batch_link_sharing_identical(q, L, nrhs):
-    """FP16 batches of 8-16 right-hand sides fetch and rebuild each site's
+    """FP16 batches of 3-16 right-hand sides fetch and rebuild each site's
     links once per workgroup, in LDS: the same bits as every (site, RHS)
     thread doing it itself, and as the same operator on single fields.  The
     small lattices leave the last workgroup partly beyond the sites; the
@@ -207,7 +208,7 @@ def test_batch_link_sharing_identical(q, L, nrhs):
     bf, of = q.Spinor(L, q.HALF, host=src, nrhs=nrhs), q.Spinor(L, q.HALF, nrhs=nrhs)
     got = {}
     try:
-        for share in (0, 1):
+        for share in (0, 2):
             q.set_batch_link_sharing(share)
             oe.tm_xxDagTm_xx(be, uh, p, q.EE)
             r1 = oe.download()
@@ -217,12 +218,12 @@ def test_batch_link_sharing_identical(q, L, nrhs):
             got[share] = (r1, r2, of.download())
     finally:
         q.set_batch_link_sharing(1)
-    for k, (a, b) in enumerate(zip(got[0], got[1])):
+    for k, (a, b) in enumerate(zip(got[0], got[2])):
         assert np.array_equal(a, b), k
     for r in (0, nrhs - 1):
         xs, ys = q.Spinor(L, q.HALF, q.EVEN, host=src[r]), q.Spinor(L, q.HALF, q.EVEN)
         ys.tm_xxDagTm_xx(xs, uh, p, q.EE)
-        assert np.array_equal(ys.download(), got[1][0][r]), r
+        assert np.array_equal(ys.download(), got[2][0][r]), r
 
 
 def test_batched_fused_update_matches_unfused(q):
@@ -239,6 +240,7 @@ def test_batched_fused_update_matches_unfused(q):
     b = q.point_sources(L, x0)
     sols, its = [], []
     try:
+        q.set_batch_link_sharing(2)   # this lattice is below the automatic threshold
         for fuse in (0, 1):
             q.set_cg_fuse(fuse)
             x = q.Spinor(L, nrhs=12)
@@ -248,28 +250,34 @@ def test_batched_fused_update_matches_unfused(q):
             its.append(st[0].iterations)
     finally:
         q.set_cg_fuse(1)
+        q.set_batch_link_sharing(1)
     assert abs(its[0] - its[1]) <= 2
     for r in range(12):
         assert rel(sols[1][r], sols[0][r]) <= 1e-8, r
 
 
-def test_multi_mu_batch_of_8_on_the_fused_path(q):
-    """Eight twisted masses on one gauge field (8 RHS: shared links and the
-    fused CG update): every member meets its true residual and agrees with
+@pytest.mark.parametrize("mus", [[0.004, 0.006, 0.01, 0.015, 0.02, 0.03, 0.05, 0.08], [0.004, 0.01, 0.02]])
+def test_multi_mu_batch_on_the_fused_path(q, mus):
+    """Eight (or three) twisted masses on one gauge field: shared links and
+    the fused CG update; every member meets its true residual and agrees with
     the single-RHS solve of its own mass."""
     L, kappa = (8, 8, 8, 16), 0.1632
-    mus = [0.004, 0.006, 0.01, 0.015, 0.02, 0.03, 0.05, 0.08]
+    n = len(mus)
     U = q.gen_gauge(L, 0.3, 9)
     u = q.Gauge(L, boundaryT=-1, host=U)
     src = q.gen_z2(L, 6)
-    print("multi-mu batch of 8 on the fused path", flush=True)
-    b = q.Spinor(L, host=np.stack([src] * 8), nrhs=8)
-    x = q.Spinor(L, nrhs=8)
-    st = q.solve_batch(x, b, u, q.params(L, kappa, mus[0], 1, -1), mu=mus, eo=True, inner_prec=q.HALF,
-                       delta=0.01, tol=1e-10)
+    print(f"multi-mu batch of {n} on the fused path", flush=True)
+    b = q.Spinor(L, host=np.stack([src] * n), nrhs=n)
+    x = q.Spinor(L, nrhs=n)
+    try:
+        q.set_batch_link_sharing(2)   # this lattice is below the automatic threshold
+        st = q.solve_batch(x, b, u, q.params(L, kappa, mus[0], 1, -1), mu=mus, eo=True, inner_prec=q.HALF,
+                           delta=0.01, tol=1e-10)
+    finally:
+        q.set_batch_link_sharing(1)
     assert all(s.converged and s.true_residual <= 1e-10 for s in st)
     got = x.download()
-    for r in (0, 3, 7):
+    for r in (0, n // 2, n - 1):
         xs, bs = q.Spinor(L), q.Spinor(L, host=src)
         s1 = q.solve(xs, bs, u, q.params(L, kappa, mus[r], 1, -1), eo=True, inner_prec=q.HALF, delta=0.01,
                      tol=1e-10)
