@@ -36,7 +36,7 @@ struct CgFuse {
 };
 int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
            double cR, double gR, double cX, double gX, int g5out, const CgFuse &f, int *nblocks_out);
-// The CG update fused into a batched hop (FP16 batches of 8-16 RHS on FP16
+// The CG update fused into a batched hop (FP16 batches of 3-16 RHS on FP16
 // links, shared-link kernel): x += alpha p, r -= alpha (hop result), alpha_r
 // = r2cur[r] / pap[r] or 0 where !active[r] or r2cur[r] <= stop2[r]; per-
 // workgroup per-RHS |r|^2 partials into r2_part.  Device pointers.

@@ -1207,7 +1207,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     if constexpr (bcg) {
         // the fused batched update exists in the shared-link kernel only, on an aux field
         if (!share || !aux) {
-            qkx::set_error("hop: the batched CG update needs shared links (FP16, 8-16 RHS) and an aux field");
+            qkx::set_error("hop: the batched CG update needs shared links (FP16, 3-16 RHS, a large enough launch) and an aux field");
             return QKX_ERR_ARG;
         }
         if (few) {
@@ -1391,7 +1391,7 @@ int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gaug
 {
     if (!hop_cg_batch_available(in, U, f.x, f.r) || !aux || f.p->prec != QKX_PREC_HALF ||
         aux->prec != QKX_PREC_HALF || f.p->nrhs != in->nrhs || f.x->nrhs != in->nrhs || f.r->nrhs != in->nrhs) {
-        set_error("hop_cg_batch: FP16 fields of 8-16 RHS on FP16 links, FP32 x, FP16 r and p");
+        set_error("hop_cg_batch: FP16 fields of 3-16 RHS on FP16 links (shared links in use), FP32 x, FP16 r and p");
         return QKX_ERR_ARG;
     }
     Epi e[QKX_MAX_RHS];

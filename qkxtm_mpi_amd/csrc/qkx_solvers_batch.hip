@@ -457,7 +457,7 @@ int enqueue_bblas(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
 }
 
 // One iteration with the update fused into the last hop (even-odd FP16
-// batches of 8-16 RHS): p.Ap from the partials of the first Schur
+// batches of 3-16 RHS that share links): p.Ap from the partials of the first Schur
 // application, summed to one row before the fused hop reads it.
 int enqueue_biter_fused(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p)
 {
