@@ -283,3 +283,31 @@ def test_multi_mu_batch_on_the_fused_path(q, mus):
                      tol=1e-10)
         assert s1.converged
         assert rel(got[r], xs.download()) <= 1e-8, r
+
+
+@pytest.mark.parametrize("share", [2, 0])
+def test_solves_are_deterministic(q, share):
+    """Every reduction sums in a fixed order (no atomics), so the same solve
+    run twice gives the same bits: the single-RHS mixed solver, and the
+    batched solver on the shared-link fused path (share 2) and on the
+    per-thread path (share 0)."""
+    L = (8, 8, 8, 16)
+    U = q.gen_gauge(L, 0.3, 11)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    p = q.params(L, 0.1632, 0.004, 1, -1)
+    b1 = q.Spinor(L, host=q.gen_z2(L, 8))
+    bb = q.point_sources(L, q.gen_site(L, 2))
+    print(f"determinism, link sharing mode {share}", flush=True)
+    runs = []
+    try:
+        q.set_batch_link_sharing(share)
+        for _ in range(2):
+            x1, xb = q.Spinor(L), q.Spinor(L, nrhs=12)
+            s1 = q.solve(x1, b1, u, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=1e-10)
+            sb = q.solve_batch(xb, bb, u, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=1e-10)
+            runs.append((x1.download(), xb.download(), s1.iterations, [s.iterations for s in sb]))
+    finally:
+        q.set_batch_link_sharing(1)
+    (a1, ab, ia, iba), (c1, cb, ic, ibc) = runs
+    assert ia == ic and iba == ibc
+    assert np.array_equal(a1, c1) and np.array_equal(ab, cb)
