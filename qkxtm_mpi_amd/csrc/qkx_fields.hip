@@ -5,6 +5,7 @@
 // (include/lattice_util.h:90), Gauge::M Complex[V][4][3][3] (:196).
 #include "qkx_internal.h"
 #include "qkx_core.h"
+#include <string>
 #include <vector>
 
 namespace {
@@ -138,16 +139,21 @@ __global__ void gauge_from_aos(void *dst, const double2 *src, Geom g)
 }
 
 template <int PREC, int RECON>
-__global__ void gauge_from_d18(void *dst, const double2 *src, int V)
+// flip_t: the t-links of the last time slice are stored negated (a field
+// whose links carry the antiperiodic sign, copied with boundaryT = -1 instead)
+__global__ void gauge_from_d18(void *dst, const double2 *src, Geom g, int flip_t)
 {
+    const int V = g.V;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
+    const bool last_t = flip_t && blocked_to_lex(s, g) / (g.L[0] * g.L[1] * g.L[2]) == g.L[3] - 1;
     for (int mu = 0; mu < 4; mu++) {
         Cx<double> U[9];
+        const double sg = (mu == 3 && last_t) ? -1.0 : 1.0;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             double2 d = src[(size_t)(mu * 9 + k) * V + s];
-            U[k] = {d.x, d.y};
+            U[k] = {sg * d.x, sg * d.y};
         }
         gauge_store<PREC, RECON>(dst, V, mu, s, U);
     }
@@ -228,47 +234,111 @@ void gauge_modified(qkx_gauge_s *g) { g->gen++; }
 void *stage_buffer(size_t bytes) { return stage(bytes); }
 
 // Per-workgroup max over (site, mu) of |row2 - conj(row0 x row1)|, the
-// reconstruction the 12-real fields use; a NaN anywhere gives NaN.
+// reconstruction the 12-real fields use (part[2b]), and the same with the
+// t-links of the last time slice negated (part[2b + 1]); a NaN gives NaN.
 __device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
-__global__ __launch_bounds__(256) void recon_dev_kernel(GaugeD18 G, int V, double *part)
+__global__ __launch_bounds__(256) void recon_dev_kernel(GaugeD18 G, Geom g, double *part)
 {
+    const int V = g.V;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0;
+    double m = 0, mf = 0;
     if (t < 4 * V) {
         const int mu = t / V, s = t - mu * V;
+        const bool last_t = mu == 3 && blocked_to_lex(s, g) / (g.L[0] * g.L[1] * g.L[2]) == g.L[3] - 1;
         Cx<double> u[9], r[9];
         G.load(mu, s, u);
 #pragma unroll
         for (int k = 0; k < 6; k++) r[k] = u[k];
-        recon_row2(r);
+        recon_row2(r);   // unchanged when rows 0 and 1 are both negated
 #pragma unroll
-        for (int k = 6; k < 9; k++) m = max_nan(max_nan(m, fabs(u[k].re - r[k].re)), fabs(u[k].im - r[k].im));
+        for (int k = 6; k < 9; k++) {
+            const double dre = fabs(u[k].re - r[k].re), dim = fabs(u[k].im - r[k].im);
+            const double fre = last_t ? fabs(u[k].re + r[k].re) : dre, fim = last_t ? fabs(u[k].im + r[k].im) : dim;
+            m = max_nan(max_nan(m, dre), dim);
+            mf = max_nan(max_nan(mf, fre), fim);
+        }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
-    __shared__ double lds[4];
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    for (int off = 32; off > 0; off >>= 1) {
+        m = max_nan(m, __shfl_xor(m, off, 64));
+        mf = max_nan(mf, __shfl_xor(mf, off, 64));
+    }
+    __shared__ double lds[2][4];
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = m;
+        lds[1][threadIdx.x >> 6] = mf;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double v = lds[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = max_nan(v, lds[w]);
-        part[blockIdx.x] = v;
+        double v = lds[0][0], vf = lds[1][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            v = max_nan(v, lds[0][w]);
+            vf = max_nan(vf, lds[1][w]);
+        }
+        part[2 * blockIdx.x] = v;
+        part[2 * blockIdx.x + 1] = vf;
     }
+}
+
+// dst <- src (FP64 18-real), with the last time slice's t-links negated when
+// flip_t (dst then applies boundaryT = -1 itself)
+int gauge_convert_flip(qkx_gauge_s *dst, const qkx_gauge_s *src, int flip_t)
+{
+    dst->bc = flip_t ? -1 : src->bc;
+    dim3 grid((dst->V + 255) / 256), blk(256);
+    const double2 *s2 = (const double2 *)src->data;
+    const Geom g = make_geom(src->L, src->bc);
+    if (dst->prec == QKX_PREC_DOUBLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 18>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    else if (dst->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 12>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    else if (dst->prec == QKX_PREC_SINGLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 18>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    else if (dst->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 12>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    else if (dst->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_HALF, 12>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    else hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_BF16, 12>), grid, blk, 0, g_stream, dst->data, s2, g, flip_t);
+    QKX_HIP(hipGetLastError());
+    gauge_modified(dst);
+    dst->src_gen = src->gen;
+    return QKX_SUCCESS;
+}
+
+// How a 12-real copy at precision prec can carry U's links: 0 as stored (SU(3)
+// to 1e-14 for FP64, 1e-6 below), 1 with the last time slice's t-links
+// negated and boundaryT = -1 (links that carry the antiperiodic sign, with
+// boundaryT = +1), -1 not at all.
+int recon12_mode(const qkx_gauge_s *U, int prec, int *mode)
+{
+    double dev = 0;
+    int rc = gauge_recon_deviation(U, &dev);
+    if (rc) return rc;
+    const double tol = prec == QKX_PREC_DOUBLE ? 1e-14 : 1e-6;
+    if (dev <= tol) *mode = 0;
+    else if (U->bc == 1 && U->recon_dev_flip <= tol) *mode = 1;
+    else *mode = -1;
+    return QKX_SUCCESS;
 }
 
 int gauge_cached(qkx_gauge_s **cache, const qkx_gauge_s *U, int prec, int recon)
 {
-    int rc;
-    if (*cache && ((*cache)->prec != prec || (*cache)->recon != recon || (*cache)->bc != U->bc)) {
+    int rc, mode = 0;
+    if (recon == QKX_RECON_12 && U->prec == QKX_PREC_DOUBLE && U->recon == QKX_RECON_18) {
+        if ((rc = recon12_mode(U, prec, &mode))) return rc;
+        if (mode < 0) {
+            set_error("12-real copy: the links are not SU(3) (row 2 != conj(row0 x row1) by " +
+                      std::to_string(U->recon_dev) + ", also with the boundary sign removed); upload them "
+                      "without the boundary sign with boundaryT = -1, or use 18-real storage / FP64");
+            return QKX_ERR_ARG;
+        }
+    }
+    const int bc = mode == 1 ? -1 : U->bc;
+    if (*cache && ((*cache)->prec != prec || (*cache)->recon != recon || (*cache)->bc != bc)) {
         qkx_gauge_destroy(*cache);
         *cache = nullptr;
     }
     if (!*cache) {
-        if ((rc = qkx_gauge_create(cache, U->L, prec, recon, U->bc))) return rc;
+        if ((rc = qkx_gauge_create(cache, U->L, prec, recon, bc))) return rc;
     } else if ((*cache)->src_gen == U->gen) {
         return QKX_SUCCESS;
     }
-    return qkx_gauge_convert(*cache, U);   // records U->gen
+    return gauge_convert_flip(*cache, U, mode);   // records U->gen
 }
 
 int gauge_recon_deviation(const qkx_gauge_s *U, double *dev)
@@ -281,17 +351,22 @@ int gauge_recon_deviation(const qkx_gauge_s *U, double *dev)
     if (Um->recon_gen != U->gen) {
         const int n = 4 * U->V, nb = (n + 255) / 256;
         double *part = nullptr;
-        QKX_HIP(hipMalloc(&part, nb * sizeof(double)));
-        hipLaunchKernelGGL(recon_dev_kernel, dim3(nb), dim3(256), 0, g_stream, make_gauge<GaugeD18>(U), U->V, part);
-        std::vector<double> h(nb);
+        QKX_HIP(hipMalloc(&part, 2 * nb * sizeof(double)));
+        hipLaunchKernelGGL(recon_dev_kernel, dim3(nb), dim3(256), 0, g_stream, make_gauge<GaugeD18>(U),
+                           make_geom(U->L, U->bc), part);
+        std::vector<double> h(2 * nb);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, g_stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, g_stream);
         if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
         (void)hipFree(part);
         QKX_HIP(e);
-        double m = 0;
-        for (double v : h) m = v > m || v != v ? v : m;   // a NaN link propagates
+        double m = 0, mf = 0;
+        for (int b = 0; b < nb; b++) {   // a NaN link propagates
+            m = h[2 * b] > m || h[2 * b] != h[2 * b] ? h[2 * b] : m;
+            mf = h[2 * b + 1] > mf || h[2 * b + 1] != h[2 * b + 1] ? h[2 * b + 1] : mf;
+        }
         Um->recon_dev = m;
+        Um->recon_dev_flip = mf;
         Um->recon_gen = U->gen;
     }
     *dev = U->recon_dev;
@@ -302,10 +377,10 @@ int gauge_fp64_operator(const qkx_gauge_s *U, const qkx_gauge_s **op)
 {
     *op = U;
     if (U->prec != QKX_PREC_DOUBLE || U->recon != QKX_RECON_18) return QKX_SUCCESS;
-    double dev = 0;
-    int rc = gauge_recon_deviation(U, &dev);
+    int mode = 0;
+    int rc = recon12_mode(U, QKX_PREC_DOUBLE, &mode);
     if (rc) return rc;
-    if (!(dev <= 1e-14)) return QKX_SUCCESS;
+    if (mode < 0) return QKX_SUCCESS;
     qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
     if ((rc = gauge_cached(&Uc->outer, U, QKX_PREC_DOUBLE, QKX_RECON_12))) return rc;
     *op = Uc->outer;
@@ -495,6 +570,7 @@ int qkx_gauge_create(qkx_gauge **out, const int L[4], int prec, int recon, int b
     g->gen = 0;
     g->src_gen = 0;
     g->recon_dev = -1.0;
+    g->recon_dev_flip = -1.0;
     g->recon_gen = ~0u;   // not measured
     size_t per = (prec == QKX_PREC_DOUBLE ? 16 : prec == QKX_PREC_SINGLE ? 8 : 4) * (recon == 18 ? 9 : 6);
     g->bytes = per * 4 * (size_t)g->V;
@@ -546,20 +622,7 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src)
     QKX_CHECK_ARG(dst && src, "qkx_gauge_convert: null argument");
     QKX_CHECK_ARG(src->prec == QKX_PREC_DOUBLE && src->recon == 18, "qkx_gauge_convert: source must be FP64 recon 18");
     QKX_CHECK_ARG(dst->V == src->V, "qkx_gauge_convert: lattices differ");
-    dst->bc = src->bc;
-    dim3 grid((dst->V + 255) / 256), blk(256);
-    const double2 *s2 = (const double2 *)src->data;
-    int V = dst->V;
-    if (dst->prec == QKX_PREC_DOUBLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 18>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    else if (dst->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_DOUBLE, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    else if (dst->prec == QKX_PREC_SINGLE && dst->recon == 18) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 18>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    else if (dst->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_SINGLE, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    else if (dst->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    else hipLaunchKernelGGL((gauge_from_d18<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, dst->data, s2, V);
-    QKX_HIP(hipGetLastError());
-    qkx::gauge_modified(dst);
-    dst->src_gen = src->gen;
-    return QKX_SUCCESS;
+    return qkx::gauge_convert_flip(dst, src, 0);
 }
 
 int qkx_gauge_su3_deviation(const qkx_gauge *g, double *dev)

@@ -554,6 +554,7 @@ struct qkx_gauge_s {
     // max over links of |row2 - conj(row0 x row1)| (what a 12-real copy would
     // change), and the generation it was measured at (~0u: never)
     double recon_dev;
+    double recon_dev_flip;   // the same with the last time slice's t-links negated
     unsigned recon_gen;
     // generation of the links: bumped by every write (upload, conversion into
     // this field, smearing, projection); a cached copy records the generation

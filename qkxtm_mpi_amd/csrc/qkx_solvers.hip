@@ -676,19 +676,6 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     // 1e-14; the reported true residual is always formed with U itself
     const qkx_gauge_s *Uop = U;
     if ((rc = qkx::gauge_fp64_operator(U, &Uop))) return rc;
-    // the low-precision copies store 12 reals (row 2 rebuilt): links that are
-    // not SU(3) -- e.g. carrying the boundary sign -- would give an inner
-    // operator other than U's
-    if (mixed && !U_inner_arg) {
-        double dev = 0;
-        if ((rc = qkx::gauge_recon_deviation(U, &dev))) return rc;
-        if (!(dev <= 1e-6)) {
-            qkx::set_error("qkx_solve: mixed precision stores the links as 12 reals (row 2 rebuilt) and these links are "
-                           "not SU(3) (deviation " + std::to_string(dev) + "); upload them without the boundary "
-                           "sign with boundaryT = -1, or solve in FP64");
-            return QKX_ERR_ARG;
-        }
-    }
     // inner gauge: caller's, or a cached conversion of U
     const qkx_gauge_s *Ui = Uop;
     if (mixed) {

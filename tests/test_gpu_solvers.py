@@ -182,13 +182,15 @@ def test_graph_replay_identical_to_eager(q):
 
 
 def test_fp64_operator_copy_only_for_su3_links(q):
-    """The solvers' internal FP64 operator runs on a 12-real copy only while
-    every link is SU(3) to 1e-14 (row 2 = conj(row0 x row1)).  Links that
-    carry the boundary sign fail that test, APE-smeared links (projected to
-    about 1e-10) fail it too.  The same system solved with signed links
-    (18-real path) and with unsigned links and boundaryT = -1 (12-real path)
-    gives the same solution.  Mixed precision stores 12 reals and refuses
-    links that are not SU(3)."""
+    """The solvers' 12-real copies (the FP64 check operator, the FP32/FP16
+    inner fields) rebuild row 2 as conj(row0 x row1), so they carry a field
+    only while its links are SU(3): to 1e-14 for the FP64 copy, 1e-6 below.
+    Links that carry the antiperiodic sign (boundaryT = +1) are copied with
+    the last time slice's t-links negated and boundaryT = -1, the same
+    operator; APE-smeared links (SU(3) to about 1e-10) keep the 18-real FP64
+    operator; links that are not SU(3) at all are refused for mixed precision
+    and solved in FP64 on their 18 reals.  Every path reaches the same
+    solution."""
     L, kappa, mu = (8, 8, 8, 16), 0.1632, 0.004
     U = q.gen_gauge(L, 0.3, 3)
     u = q.Gauge(L, boundaryT=-1, host=U)
@@ -199,21 +201,23 @@ def test_fp64_operator_copy_only_for_su3_links(q):
     ape.APE_smearing(u, 5, 0.5)
     assert 1e-14 < ape.su3_deviation() < 1e-6
     b = q.gen_z2(L, 4)
-    sols = []
-    for g, bt in ((u, -1), (us, +1)):
-        p = q.params(L, kappa, mu, 1, bt)
-        x, bs = q.Spinor(L), q.Spinor(L, host=b)
-        print(f"FP64 solve boundaryT={bt} deviation={g.su3_deviation():.2e}", flush=True)
-        st = q.solve(x, bs, g, p, eo=True, inner_prec=q.DOUBLE, tol=1e-10)
+
+    def solve(g, bt, prec):
+        x = q.Spinor(L)
+        print(f"solve boundaryT={bt} prec={prec} deviation={g.su3_deviation():.2e}", flush=True)
+        st = q.solve(x, q.Spinor(L, host=b), g, q.params(L, kappa, mu, 1, bt), eo=True, inner_prec=prec,
+                     delta=0.01, tol=1e-10)
         assert st.converged and st.true_residual <= 1e-10
-        sols.append(x.download())
-    assert rel(sols[0], sols[1]) <= 1e-8
-    # mixed precision stores 12 reals: signed links are refused with a reason,
-    # smeared links (SU(3) to ~1e-10) are accepted
+        return x.download()
+
+    ref = solve(u, -1, q.DOUBLE)
+    for g, bt, prec in ((us, +1, q.DOUBLE), (u, -1, q.HALF), (us, +1, q.HALF), (us, +1, q.SINGLE)):
+        assert rel(solve(g, bt, prec), ref) <= 1e-8, (bt, prec)
+    solve(ape, -1, q.HALF)
+    # links that are not SU(3): refused for mixed precision, solved in FP64
+    wide = q.Gauge(L, boundaryT=-1, host=1.01 * U)
+    assert wide.su3_deviation() > 1e-3
     with pytest.raises(q.QkxError, match="not SU\\(3\\)"):
-        q.solve(q.Spinor(L), q.Spinor(L, host=b), us, q.params(L, kappa, mu, 1, +1), eo=True, inner_prec=q.HALF,
-                delta=0.01, tol=1e-10)
-    x = q.Spinor(L)
-    st = q.solve(x, q.Spinor(L, host=b), ape, q.params(L, kappa, mu, 1, -1), eo=True, inner_prec=q.HALF,
-                 delta=0.01, tol=1e-10)
-    assert st.converged and st.true_residual <= 1e-10
+        q.solve(q.Spinor(L), q.Spinor(L, host=b), wide, q.params(L, kappa, mu, 1, -1), eo=True,
+                inner_prec=q.HALF, delta=0.01, tol=1e-10)
+    solve(wide, -1, q.DOUBLE)
