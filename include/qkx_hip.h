@@ -173,8 +173,9 @@ int qkx_gauge_download(const qkx_gauge *g, double *U_aos);
  * 1e-14 the solvers (qkx_solve, qkx_solve_batch) run their internal FP64
  * operator on such a copy (25% fewer bytes per hop); the reported true
  * residual always uses the field itself.  Links that carry the boundary sign
- * or are not SU(3) give O(1).  Measured once per change of the links.  No
- * reference counterpart. */
+ * or are not SU(3) give O(1) (sign-carrying links are copied with the sign
+ * removed and boundaryT = -1 instead).  Measured once per change of the
+ * links.  No reference counterpart. */
 int qkx_gauge_su3_deviation(const qkx_gauge *g, double *dev);
 
 /* Vector::Vector lib/lattice_util.cpp:382-413 */
