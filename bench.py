@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--batch-kernels-only", action="store_true",
                     help="only apply the FP16 even-odd normal operator to a batch of 12 right-hand sides at "
                          "24^3x48, three times (hardware-counter runs of the batched hop)")
+    ap.add_argument("--batch-share", type=int, default=1, choices=[0, 1, 2],
+                    help="links shared per workgroup in batched hops: 0 never, 1 automatic (default), 2 always")
     return ap.parse_args()
 
 
@@ -523,6 +525,7 @@ def main():
         run_smear(q, args)
         return
     if args.batch_kernels_only:
+        q.set_batch_link_sharing(args.batch_share)
         batch_kernels(q)
         return
 
