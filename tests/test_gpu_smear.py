@@ -106,6 +106,31 @@ def test_gauss_batch_slice_and_noop(q):
         keep.Gaussian_smearing(keep, u, 2, 4.0)
 
 
+def test_gauss_batch_of_12_whole_lattice_and_slice(q):
+    """The 12-field batch of a propagator's sink (whole lattice) and of its
+    point sources (one time slice): every member against the oracle, and the
+    slice form equal to the whole-lattice one on fields that live on it."""
+    L = (4, 6, 4, 8)
+    U = q.gen_gauge(L, 0.3, 81)
+    Ub = orc.apply_bc(U, L, -1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    V = int(np.prod(L))
+    print("gauss batch of 12", flush=True)
+    srcs = np.stack([q.gen_gaussian(L, 90 + r) for r in range(12)])
+    ob = q.Spinor(L, nrhs=12)
+    ob.Gaussian_smearing(q.Spinor(L, host=srcs, nrhs=12), u, 7, 2.5)
+    got = ob.download()
+    for r in range(12):
+        assert rel(got[r], orc.gauss_smear(srcs[r], Ub, L, 7, 2.5)) <= 1e-13, r
+    t0 = 5
+    on_slice = srcs.copy()
+    on_slice[:, np.arange(V) // (V // L[3]) != t0] = 0
+    full, sl = q.Spinor(L, nrhs=12), q.Spinor(L, nrhs=12)
+    full.Gaussian_smearing(q.Spinor(L, host=on_slice, nrhs=12), u, 7, 2.5)
+    sl.Gaussian_smearing(q.Spinor(L, host=on_slice, nrhs=12), u, 7, 2.5, t=t0)
+    assert np.array_equal(full.download(), sl.download())
+
+
 def test_smeared_source_solve_smeared_sink(q):
     """The drivers' chain (bin/test_invert.cpp:77-117): APE-smear the links,
     Gaussian-smear a point source with them, solve D x = b on the unsmeared
