@@ -13,7 +13,7 @@ Inputs are not stored: they are regenerated from (L, eps, seeds) by the
 library's counter-hash generator; a hash of them is kept to detect drift.
 
 usage (in the build container, after __graft_entry__.build()):
-    python tests/golden/make_golden.py [ops] [smear] [prop]     (default: all)
+    python tests/golden/make_golden.py [ops] [cfg2] [smear] [prop]     (default: all)
 """
 import os
 import struct
@@ -43,6 +43,9 @@ CASES = [
 ]
 # config-1 of the benchmark set: 8^3x16, eps 0.4, kappa 0.156, mu 0.01, Z2 source
 CG_CASE = ("cfg1", (8, 8, 8, 16), 0.4, 0.156, 0.01, +1, -1, 1, 2, 3)
+# config-2 (the headline solve): 24^3x48, eps 0.3, kappa 0.1632, mu 0.004, Z2
+# source, gauge seed 1 and source seed 2 as in bench.py; 64 sampled sites
+CG_CASE2 = ("cfg2", (24, 24, 24, 48), 0.3, 0.1632, 0.004, +1, -1, 1, 2, 3)
 
 
 def write_input(path, L, kappa, mu, ts, bt, tol, maxit, U, psi, phi):
@@ -100,8 +103,8 @@ def run_ops_case(case, tmpdir):
     print(f"ops_{tag}.npz: {len(names)} operator outputs")
 
 
-def run_cg_case(tmpdir):
-    tag, L, eps, kappa, mu, ts, bt, sg, sp, sf = CG_CASE
+def run_cg_case(tmpdir, case=CG_CASE, nsites=8):
+    tag, L, eps, kappa, mu, ts, bt, sg, sp, sf = case
     U = q.gen_gauge(L, eps, sg)
     src = q.gen_z2(L, sp)
     phi = np.zeros_like(src)
@@ -111,10 +114,11 @@ def run_cg_case(tmpdir):
     write_input(fin, L, kappa, mu, ts, bt, 1e-14, 20000, U, src, phi)
     subprocess.run([HARNESS, "ops", fin, fout], check=True, stdout=subprocess.DEVNULL)
     recs = read_records(fout)
-    R, sites = probes(L)
+    os.remove(fout)
+    R, sites = probes(L, nsites)
     data = {"L": np.array(L), "params": np.array([eps, kappa, mu, ts, bt, sg, sp, sf]),
             "input_hash": np.array(input_hash(U, src)), "sites": sites}
-    for name in ("tm", "tmDag", "tmDagtm", "tm_xxDagTm_xx_EE"):
+    for name in ("tm", "tmDag", "tmDagtm", "tm_xx_EE", "tm_xxDagTm_xx_EE", "tm_chiral", "tm_xxDagTm_xx_OO"):
         s = summarize(recs[name].view(np.complex128), R, sites)
         data["norm2_" + name] = np.array(s["norm2"])
         data["proj_" + name] = s["proj"]
@@ -130,8 +134,8 @@ def run_cg_case(tmpdir):
     data["site_cg_x"] = s["sites"]
     res = np.linalg.norm(Dx - src.ravel()) / np.linalg.norm(src.ravel())
     data["cg_true_residual"] = np.array(res)
-    np.savez_compressed(os.path.join(OUT, "cg_cfg1.npz"), **data)
-    print(f"cg_cfg1.npz: reference CG true residual {res:.3e}")
+    np.savez_compressed(os.path.join(OUT, f"cg_{tag}.npz"), **data)
+    print(f"cg_{tag}.npz: reference CG true residual {res:.3e}")
 
 
 # smearing: (tag, L, eps, boundaryT, gauge seed, psi seed, nAPE, alphaAPE, nGauss, alphaGauss)
@@ -232,12 +236,15 @@ def run_prop_case(tmpdir):
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: make -C oracle/ref (needs the reference sources)")
-    only = set(sys.argv[1:])   # sections: ops, smear, prop (default: all)
+    only = set(sys.argv[1:])   # sections: ops, cfg2, smear, prop (default: all)
     with tempfile.TemporaryDirectory() as d:
         if not only or "ops" in only:
             for c in CASES:
                 run_ops_case(c, d)
             run_cg_case(d)
+        if not only or "cfg2" in only:
+            # the reference CG at 24^3x48 to tol 1e-14: several minutes on one core
+            run_cg_case(d, CG_CASE2, nsites=64)
         if not only or "smear" in only:
             for c in SMEAR_CASES:
                 run_smear_case(c, d)

@@ -68,3 +68,26 @@ def test_oracle_cg_matches_reference_cg():
     assert rel(s["sites"], g["site_cg_x"]) <= 1e-12
     res = rel(orc.tm(x, Ub, L, kappa, mu, int(ts)), src)
     assert res <= 1e-12 and float(g["cg_true_residual"]) <= 1e-12
+
+
+def test_oracle_matches_reference_at_24x24x24x48():
+    """Config 2 (the headline solve): the reference's own operators on the
+    24^3x48 Z2 source, and the reference CG solution's true residual (the
+    fixture that pins every solver of the library at this size)."""
+    g = _load(os.path.join(GOLD, "cg_cfg2.npz"))
+    L = tuple(int(v) for v in g["L"])
+    eps, kappa, mu, ts, bt, sg, sp, sf = g["params"]
+    U = q.gen_gauge(L, eps, int(sg))
+    src = q.gen_z2(L, int(sp))
+    assert str(g["input_hash"]) == input_hash(U, src), "generator output drifted"
+    Ub = orc.apply_bc(U, L, int(bt))
+    R, sites = probes(L, len(g["sites"]))
+    assert np.array_equal(sites, g["sites"])
+    for name in ("tm", "tmDag", "tm_xx_EE", "tm_xxDagTm_xx_EE", "tm_chiral", "tm_xxDagTm_xx_OO"):
+        y = oracle_op(name, Ub, src, src, L, kappa, mu, int(ts))
+        s = summarize(y, R, sites)
+        n2 = float(g["norm2_" + name])
+        assert abs(s["norm2"] - n2) <= 1e-13 * n2, name
+        assert np.max(np.abs(s["proj"] - g["proj_" + name])) <= 1e-13 * np.sqrt(n2 * 12 * np.prod(L)), name
+        assert rel(s["sites"], g["site_" + name]) <= 1e-14, name
+    assert float(g["cg_true_residual"]) <= 1e-13
