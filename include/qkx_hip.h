@@ -161,9 +161,16 @@ int  qkx_set_rhs_tile(int rt);
  * RECON_12 needs SU(3) links. */
 int qkx_gauge_create(qkx_gauge **g, const int L[4], int prec, int recon, int boundaryT);
 int qkx_gauge_destroy(qkx_gauge *g);
-/* Host AoS (reference layout) -> device. */
+/* Host AoS (reference layout) -> device.  A RECON_12 field rebuilds row 2 as
+ * conj(row0 x row1): links that carry the antiperiodic sign (a field created
+ * with boundaryT = +1) are stored with the last time slice's t-links negated
+ * and the field takes boundaryT = -1 (the same operator); links that are not
+ * SU(3) (to 1e-14 for FP64, 1e-6 below) are refused with QKX_ERR_ARG. */
 int qkx_gauge_upload(qkx_gauge *g, const double *U_aos);
-/* Device -> device precision/compression change (for mixed-precision solvers). */
+/* Device -> device precision/compression change (for mixed-precision
+ * solvers); src is FP64 recon 18.  A RECON_12 destination follows the rule
+ * of qkx_gauge_upload (sign-carrying links flipped, dst takes boundaryT = -1;
+ * non-SU(3) links refused). */
 int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src);
 /* Device -> host AoS (reference layout), FP64 recon-18 fields; the links as
  * stored (the boundaryT = -1 sign is applied by the kernels, not stored). */

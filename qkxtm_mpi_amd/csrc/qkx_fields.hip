@@ -121,20 +121,74 @@ __device__ __forceinline__ void gauge_store(void *base, int V, int mu, int s, co
     }
 }
 
+// flip_t: the t-links of the last time slice are stored negated (links that
+// carry the antiperiodic sign, stored in 12 reals with boundaryT = -1 instead)
 template <int PREC, int RECON>
-__global__ void gauge_from_aos(void *dst, const double2 *src, Geom g)
+__global__ void gauge_from_aos(void *dst, const double2 *src, Geom g, int flip_t)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.V) return;
     int lex = blocked_to_lex(s, g);
+    const bool last_t = flip_t && lex / (g.L[0] * g.L[1] * g.L[2]) == g.L[3] - 1;
     for (int mu = 0; mu < 4; mu++) {
         Cx<double> U[9];
+        const double sg = (mu == 3 && last_t) ? -1.0 : 1.0;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             double2 d = src[((size_t)lex * 4 + mu) * 9 + k];
-            U[k] = {d.x, d.y};
+            U[k] = {sg * d.x, sg * d.y};
         }
         gauge_store<PREC, RECON>(dst, g.V, mu, s, U);
+    }
+}
+
+// Per-workgroup max over the links of a host-layout field (U[lex][mu][9]) of
+// |row2 - conj(row0 x row1)| as stored (part[2b]) and with the last time
+// slice's t-links negated (part[2b + 1]); a NaN gives NaN.
+__device__ __forceinline__ double max_nan_aos(double a, double b) { return (b > a || b != b) ? b : a; }
+__global__ __launch_bounds__(256) void recon_dev_aos_kernel(const double2 *src, Geom g, double *part)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // (lex, mu), mu fastest
+    double m = 0, mf = 0;
+    if (t < 4 * g.V) {
+        const int lex = t >> 2, mu = t & 3;
+        const bool last_t = mu == 3 && lex / (g.L[0] * g.L[1] * g.L[2]) == g.L[3] - 1;
+        Cx<double> u[9], r[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double2 d = src[(size_t)t * 9 + k];
+            u[k] = {d.x, d.y};
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) r[k] = u[k];
+        recon_row2(r);   // unchanged when rows 0 and 1 are both negated
+#pragma unroll
+        for (int k = 6; k < 9; k++) {
+            const double dre = fabs(u[k].re - r[k].re), dim = fabs(u[k].im - r[k].im);
+            const double fre = last_t ? fabs(u[k].re + r[k].re) : dre, fim = last_t ? fabs(u[k].im + r[k].im) : dim;
+            m = max_nan_aos(max_nan_aos(m, dre), dim);
+            mf = max_nan_aos(max_nan_aos(mf, fre), fim);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        m = max_nan_aos(m, __shfl_xor(m, off, 64));
+        mf = max_nan_aos(mf, __shfl_xor(mf, off, 64));
+    }
+    __shared__ double lds[2][4];
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][threadIdx.x >> 6] = m;
+        lds[1][threadIdx.x >> 6] = mf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = lds[0][0], vf = lds[1][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            v = max_nan_aos(v, lds[0][w]);
+            vf = max_nan_aos(vf, lds[1][w]);
+        }
+        part[2 * blockIdx.x] = v;
+        part[2 * blockIdx.x + 1] = vf;
     }
 }
 
@@ -602,15 +656,49 @@ int qkx_gauge_upload(qkx_gauge *g, const double *U)
     void *st = stage(bytes);
     if (!st) { qkx::set_error("qkx_gauge_upload: staging allocation failed"); return QKX_ERR_NOMEM; }
     QKX_HIP(hipMemcpyAsync(st, U, bytes, hipMemcpyHostToDevice, qkx::g_stream));
+    const double2 *s2 = (const double2 *)st;
+    // 12-real storage rebuilds row 2: as for qkx_gauge_convert, links that
+    // carry the antiperiodic sign (boundaryT = +1) are stored with the last
+    // time slice's t-links negated and boundaryT = -1; links that are not
+    // SU(3) are refused
+    int flip = 0;
+    if (g->recon == QKX_RECON_12) {
+        const int n = 4 * g->V, nb = (n + 255) / 256;
+        double *part = nullptr;
+        QKX_HIP(hipMalloc(&part, 2 * nb * sizeof(double)));
+        hipLaunchKernelGGL(recon_dev_aos_kernel, dim3(nb), dim3(256), 0, qkx::g_stream, s2,
+                           qkx::make_geom(g->L, g->bc), part);
+        std::vector<double> h(2 * nb);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(qkx::g_stream);
+        (void)hipFree(part);
+        QKX_HIP(e);
+        double m = 0, mf = 0;
+        for (int b = 0; b < nb; b++) {   // a NaN link propagates
+            m = h[2 * b] > m || h[2 * b] != h[2 * b] ? h[2 * b] : m;
+            mf = h[2 * b + 1] > mf || h[2 * b + 1] != h[2 * b + 1] ? h[2 * b + 1] : mf;
+        }
+        const double tol = g->prec == QKX_PREC_DOUBLE ? 1e-14 : 1e-6;
+        if (m <= tol) {
+            flip = 0;
+        } else if (g->bc == 1 && mf <= tol) {
+            flip = 1;
+            g->bc = -1;
+        } else {
+            qkx::set_error("qkx_gauge_upload: 12-real field, but the links are not SU(3) (row 2 != conj(row0 x "
+                           "row1) by " + std::to_string(m) + ", also with the boundary sign removed)");
+            return QKX_ERR_ARG;
+        }
+    }
     Geom geo = qkx::make_geom(g->L, g->bc);
     dim3 grid((g->V + 255) / 256), blk(256);
-    const double2 *s2 = (const double2 *)st;
-    if (g->prec == QKX_PREC_DOUBLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
-    else if (g->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
-    else if (g->prec == QKX_PREC_SINGLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
-    else if (g->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
-    else if (g->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
-    else hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo);
+    if (g->prec == QKX_PREC_DOUBLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
+    else if (g->prec == QKX_PREC_DOUBLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_DOUBLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
+    else if (g->prec == QKX_PREC_SINGLE && g->recon == 18) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 18>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
+    else if (g->prec == QKX_PREC_SINGLE) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_SINGLE, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
+    else if (g->prec == QKX_PREC_HALF) hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_HALF, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
+    else hipLaunchKernelGGL((gauge_from_aos<QKX_PREC_BF16, 12>), grid, blk, 0, qkx::g_stream, g->data, s2, geo, flip);
     QKX_HIP(hipGetLastError());
     QKX_HIP(hipStreamSynchronize(qkx::g_stream));
     qkx::gauge_modified(g);
@@ -622,7 +710,22 @@ int qkx_gauge_convert(qkx_gauge *dst, const qkx_gauge *src)
     QKX_CHECK_ARG(dst && src, "qkx_gauge_convert: null argument");
     QKX_CHECK_ARG(src->prec == QKX_PREC_DOUBLE && src->recon == 18, "qkx_gauge_convert: source must be FP64 recon 18");
     QKX_CHECK_ARG(dst->V == src->V, "qkx_gauge_convert: lattices differ");
-    return qkx::gauge_convert_flip(dst, src, 0);
+    // a 12-real destination rebuilds row 2 as conj(row0 x row1): links that
+    // carry the antiperiodic sign are stored with the last time slice's
+    // t-links negated and boundaryT = -1 (the same operator); links that are
+    // not SU(3) are refused
+    int mode = 0;
+    if (dst->recon == QKX_RECON_12) {
+        int rc = qkx::recon12_mode(src, dst->prec, &mode);
+        if (rc) return rc;
+        if (mode < 0) {
+            qkx::set_error("qkx_gauge_convert: 12-real destination, but the links are not SU(3) (row 2 != "
+                           "conj(row0 x row1) by " + std::to_string(src->recon_dev) +
+                           ", also with the boundary sign removed)");
+            return QKX_ERR_ARG;
+        }
+    }
+    return qkx::gauge_convert_flip(dst, src, mode);
 }
 
 int qkx_gauge_su3_deviation(const qkx_gauge *g, double *dev)

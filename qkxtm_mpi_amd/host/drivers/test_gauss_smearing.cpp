@@ -5,11 +5,14 @@
 // 50, alpha 4), its norm, and the time slice t = 0 of the result written to a
 // file as "x y z mu c re im" (the reference writes the same slice to kale.dat).
 // Synthetic gauge (generator, eps 0.4, 8^3x16), passed through an ILDG/LIME
-// file when a second argument names one.
+// file when a second argument names one; a third argument overrides the
+// Gaussian step count.  The norm of vec_tmp after the smearing is printed too
+// (the reference leaves the last or the next-to-last iterate there).
 #include "qkxtm/qkxTM.h"
 #include "qkxtm/lattice_util.h"
 #include "qkx_hip.h"
 #include <cstdio>
+#include <cstdlib>
 
 using namespace qkxTM;
 
@@ -25,6 +28,7 @@ int main(int argc, char *argv[])
     latInfo.alphaAPE = 0.5;
     latInfo.NsmearGauss = 50;
     latInfo.alphaGauss = 4.0;
+    if (argc > 3) latInfo.NsmearGauss = atoi(argv[3]);
     int rank = 0;
 
     Gauge *gauge = new Gauge(&latInfo);
@@ -55,6 +59,7 @@ int main(int argc, char *argv[])
     vec_tmp->zero();
     vec_tmp->M[0][0][0] = Complex(1.0, 0.0);
     vec->Gaussian_smearing(*vec_tmp, *gauge_ape, &latInfo);
+    printfQKXTM("the norm of vec_tmp is %.17e\n", vec_tmp->norm2());
     delete vec_tmp;
     double vec_norm = vec->norm2();
     printfQKXTM("the norm of vector is %.17e\n", vec_norm);

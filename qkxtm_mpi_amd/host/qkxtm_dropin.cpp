@@ -470,15 +470,31 @@ int qkxTM_writeGaugeLime(char *fname, qkxTM::Gauge &u, char *message)
 }
 
 // Vector::Gaussian_smearing :1080-1146 (whole lattice, as the reference);
-// nothing happens for NsmearGauss = 0
+// nothing happens for NsmearGauss = 0.  The reference alternates its steps
+// between *this and vec_tmp, so vec_tmp ends with the smeared field (even
+// step count) or the iterate before the last step (odd count); both are
+// reproduced: n - 1 steps into one field, the last step into the other.
 void Vector::Gaussian_smearing(Vector &vec_tmp, Gauge &gauge, LatticeInfo *latInfo)
 {
-    if (latInfo->NsmearGauss == 0) return;
+    const int n = latInfo->NsmearGauss;
+    const double alpha = latInfo->alphaGauss;
+    if (n == 0) return;
     printfQKXTM("Perform Gaussian smearing\n");
     vec_tmp.upload();
-    check(qkx_gauss_smear(dev(), vec_tmp.dev(), gauge.dev(), latInfo->NsmearGauss, latInfo->alphaGauss, -1),
-          "Gaussian_smearing");
-    download();
+    if (n % 2 == 0) {
+        check(qkx_gauss_smear(dev(), vec_tmp.dev(), gauge.dev(), n, alpha, -1), "Gaussian_smearing");
+        download();
+        vec_tmp.copy(*this);
+    } else if (n == 1) {
+        check(qkx_gauss_smear(dev(), vec_tmp.dev(), gauge.dev(), 1, alpha, -1), "Gaussian_smearing");
+        download();
+    } else {
+        // dev() <- n - 1 steps, vec_tmp.dev() <- the last step
+        check(qkx_gauss_smear(dev(), vec_tmp.dev(), gauge.dev(), n - 1, alpha, -1), "Gaussian_smearing");
+        check(qkx_gauss_smear(vec_tmp.dev(), dev(), gauge.dev(), 1, alpha, -1), "Gaussian_smearing");
+        check(qkx_spinor_download(dev(), &(vec_tmp.M[0][0][0].real())), "Gaussian_smearing");
+        check(qkx_spinor_download(vec_tmp.dev(), &(M[0][0][0].real())), "Gaussian_smearing");
+    }
 }
 
 // Vector::rescale :2314-2321

@@ -147,7 +147,7 @@ def test_solve_batch_per_rhs_mu(q, cfg1, eo, prec):
     s = summarize(xs[0], R, sites)
     n = np.sqrt(float(g["norm2_cg_x"]))
     assert abs(np.sqrt(s["norm2"]) - n) <= 1e-9 * n
-    assert rel(s["sites"], g["site_cg_x"]) <= 1e-8
+    assert rel(s["sites"], g["site_cg_x"]) <= 1e-9
 
 
 def test_solve_batch_deterministic_and_zero_source(q, cfg1):
@@ -224,6 +224,37 @@ def test_batch_link_sharing_identical(q, L, nrhs):
         xs, ys = q.Spinor(L, q.HALF, q.EVEN, host=src[r]), q.Spinor(L, q.HALF, q.EVEN)
         ys.tm_xxDagTm_xx(xs, uh, p, q.EE)
         assert np.array_equal(ys.download(), got[2][0][r]), r
+
+
+def test_batch_with_sign_carrying_links(q):
+    """Links that carry the antiperiodic sign (boundaryT = +1, the drop-in
+    drivers' links) through qkx_solve_batch on the shared-link fused path:
+    its 12-real copies are stored with the last slice's t-links negated and
+    boundaryT = -1, which must give the solution of the unsigned links."""
+    L = (8, 8, 8, 16)
+    U = q.gen_gauge(L, 0.3, 7)
+    Ub = orc.apply_bc(U, L, -1)
+    x0 = q.gen_site(L, 3)
+    print(f"batch with sign-carrying links, 12 point sources at {x0}", flush=True)
+    b = q.point_sources(L, x0)
+    sols = []
+    try:
+        q.set_batch_link_sharing(2)   # this lattice is below the automatic threshold
+        for u, bt in ((q.Gauge(L, boundaryT=-1, host=U), -1), (q.Gauge(L, boundaryT=+1, host=Ub), +1)):
+            x = q.Spinor(L, nrhs=12)
+            st = q.solve_batch(x, b, u, q.params(L, 0.1632, 0.004, 1, bt), eo=True, inner_prec=q.HALF,
+                               delta=0.01, tol=1e-10)
+            assert all(s.converged and s.true_residual <= 1e-10 for s in st), bt
+            sols.append(x.download())
+    finally:
+        q.set_batch_link_sharing(1)
+    V = int(np.prod(L))
+    iv = ((x0[3] * L[2] + x0[2]) * L[1] + x0[1]) * L[0] + x0[0]
+    for r in (0, 7, 11):
+        pt = np.zeros((V, 4, 3), complex)
+        pt[iv, r // 3, r % 3] = 1.0
+        assert rel(orc.tm(sols[1][r], Ub, L, 0.1632, 0.004, 1), pt) <= 1e-10, r
+        assert rel(sols[1][r], sols[0][r]) <= 1e-8, r
 
 
 def test_batched_fused_update_matches_unfused(q):

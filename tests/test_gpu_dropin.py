@@ -77,6 +77,21 @@ def test_driver_test_gauss_smearing(q, tmp_path):
     got = (a[:, 5] + 1j * a[:, 6]).reshape(-1, 4, 3)
     assert rel(got, ref[:L[0] * L[1] * L[2]]) <= 1e-10
     assert np.array_equal(q.read_lime(lime, L), U)   # the file the driver wrote and read back
+    # the reference alternates between the vector and vec_tmp: after an even
+    # step count vec_tmp holds the result, after an odd one the iterate before
+    # the last step (lib/lattice_util.cpp:1097-1143)
+    assert abs(float(re.search(r"the norm of vec_tmp is (\S+)", r.stdout).group(1)) - orc.norm2(ref)) <= \
+        1e-10 * orc.norm2(ref)
+    for n in (1, 7):
+        r = subprocess.run([os.path.join(HOST, "test_gauss_smearing"), str(dump), str(lime), str(n)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        prev = orc.gauss_smear(pt, A, L, n - 1, 4.0) if n > 1 else pt
+        last = orc.gauss_smear(pt, A, L, n, 4.0)
+        got_tmp = float(re.search(r"the norm of vec_tmp is (\S+)", r.stdout).group(1))
+        got_vec = float(re.search(r"the norm of vector is (\S+)", r.stdout).group(1))
+        assert abs(got_tmp - orc.norm2(prev)) <= 1e-10 * orc.norm2(prev), n
+        assert abs(got_vec - orc.norm2(last)) <= 1e-10 * orc.norm2(last), n
 
 
 def test_driver_test_prop(q, tmp_path):

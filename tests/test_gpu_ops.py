@@ -137,6 +137,38 @@ def test_low_precision_tm(q, prec, recon, tol):
     assert rel(ye.download(), orc.tm_xx(psi, Ub, L, kappa, mu, 1, orc.UKQCD, orc.EE, 1)) <= 4 * tol
 
 
+@pytest.mark.parametrize("prec,tol", [("DOUBLE", 1e-13), ("SINGLE", 2e-6), ("HALF", 2e-3), ("BF16", 2e-2)])
+def test_sign_carrying_links_in_12_reals(q, prec, tol):
+    """Links that carry the antiperiodic sign (boundaryT = +1), converted or
+    uploaded into 12-real storage: row 2 rebuilt from the signed rows would
+    have the wrong sign on the last time slice, so the field is stored with
+    those t-links negated and takes boundaryT = -1 -- the same operator as the
+    18-real field.  Links that are not SU(3) are refused."""
+    L, kappa, mu = (8, 8, 4, 8), 0.1632, 0.004
+    P = getattr(q, prec)
+    U, psi, _ = _setup(q, L, 0.3, 25, -1)
+    Ub = orc.apply_bc(U, L, -1)
+    us = q.Gauge(L, boundaryT=+1, host=Ub)
+    p = q.params(L, kappa, mu, 1, -1)
+    ref = orc.tm(psi, Ub, L, kappa, mu, 1)
+    ref_e = orc.tm_xx(psi, Ub, L, kappa, mu, 1, orc.UKQCD, orc.EE, 1)
+    conv = q.Gauge(L, P, q.RECON_12, +1)
+    conv.convert_from(us)
+    up = q.Gauge(L, P, q.RECON_12, +1, host=Ub)
+    for name, u in (("convert", conv), ("upload", up)):
+        x, y = q.Spinor(L, P, host=psi), q.Spinor(L, P)
+        y.tm(x, u, p)
+        assert rel(y.download(), ref) <= tol, name
+        xe, ye = q.Spinor(L, P, q.EVEN, host=psi), q.Spinor(L, P, q.EVEN)
+        ye.tm_xxDagTm_xx(xe, u, p, q.EE)
+        assert rel(ye.download(), ref_e) <= 4 * tol, name
+    wide = q.Gauge(L, boundaryT=+1, host=1.01 * Ub)
+    with pytest.raises(q.QkxError, match="not SU\\(3\\)"):
+        q.Gauge(L, P, q.RECON_12, +1).convert_from(wide)
+    with pytest.raises(q.QkxError, match="not SU\\(3\\)"):
+        q.Gauge(L, P, q.RECON_12, -1, host=1.01 * U)
+
+
 def test_blas(q):
     L = (4, 4, 4, 8)
     a = q.gen_gaussian(L, 31)

@@ -37,7 +37,7 @@ def test_reference_cg_matches_oracle(q, cfg1):
     st = q.CG(p, u)(x, bd)
     print("iterations gpu", st.iterations, "oracle", k_ref, flush=True)
     assert st.converged and abs(st.iterations - k_ref) <= 2
-    assert rel(x.download(), x_ref) <= 1e-8
+    assert rel(x.download(), x_ref) <= 1e-9
 
 
 def test_reference_cg_maxiter_status(q, cfg1):
@@ -69,7 +69,7 @@ def test_solve_true_residual_and_reference_solution(q, cfg1, eo, prec):
     s = summarize(xs, R, sites)
     n = np.sqrt(float(g["norm2_cg_x"]))
     assert abs(np.sqrt(s["norm2"]) - n) <= 1e-9 * n
-    assert rel(s["sites"], g["site_cg_x"]) <= 1e-8
+    assert rel(s["sites"], g["site_cg_x"]) <= 1e-9
 
 
 def test_solve_nonfinite_source(q):
