@@ -240,6 +240,21 @@ int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
 int qkx_tm_xx(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
               const qkx_params *p, int basis, int parity, int normal);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): qkx_tm_xx with the same arguments and results, but its
+ * intermediate fields (the other parity, and for normal = 1 the first Schur
+ * application) live in a static device buffer of QKX_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds from qkx_init until the process ends, instead of the
+ * library's field pool.  The call allocates nothing; it needs qkx_init first
+ * and instrumentation off, and refuses (QKX_ERR_ARG) fields whose
+ * intermediates exceed the buffer: (normal ? 2 : 1) x V/2 x nrhs x the
+ * bytes per site of in's precision (FP64 192, FP32 96, FP16 52, BF16 48).
+ * Calls that overlap on one GPU share the buffer and race. */
+#define QKX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qkx_tm_xx_noalloc(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
+                      const qkx_params *p, int basis, int parity, int normal);
+
 /* ---- BLAS (include/blas_qkxTM.h:3-5, lib/blas_qkxTM.cpp) -------------------- */
 int qkx_redot(double *res, const qkx_spinor *x, const qkx_spinor *y); /* reCdotXdagY :4  */
 int qkx_axpy(qkx_spinor *x, double a, const qkx_spinor *y);          /* X_eq_X_p_aY :31 */

@@ -95,6 +95,7 @@ SIGNATURES = {
     "qkx_twist_add_dslash": [_P, _P, _P, ctypes.POINTER(Params), _I, _I, _I],
     "qkx_tm": [_P, _P, _P, ctypes.POINTER(Params), _I, _I],
     "qkx_tm_xx": [_P, _P, _P, ctypes.POINTER(Params), _I, _I, _I],
+    "qkx_tm_xx_noalloc": [_P, _P, _P, ctypes.POINTER(Params), _I, _I, _I],
     "qkx_redot": [_DP, _P, _P], "qkx_axpy": [_P, _D, _P], "qkx_xpay": [_P, _D, _P], "qkx_norm2": [_DP, _P],
     "qkx_cg": [_P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolverStats)],
     "qkx_solve": [_P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolveOpts), ctypes.POINTER(SolverStats)],

@@ -146,6 +146,7 @@ int qkx_init(int device)
     QKX_HIP(hipEventCreate(&g_t0));
     QKX_HIP(hipEventCreate(&g_t1));
     QKX_HIP(hipDeviceSynchronize());
+    if (int rc = noalloc_setup(device)) return rc;
     g_device = device;
     g_init = true;
     return QKX_SUCCESS;

@@ -70,6 +70,17 @@ int schur(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, doubl
           int basis, int pe, int g5out, qkx_spinor_s *tmp_other, double *norm_partial = nullptr,
           int *nblocks = nullptr);
 
+// qkx_tm_xx in two parts (qkx_ops.cpp): the argument checks (*pe = output
+// parity) and the application with caller-provided intermediates (t: the
+// other parity; y: parity pe, normal form only)
+int tm_xx_check(const qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_params *p,
+                int basis, int parity, int *pe);
+int tm_xx_apply(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_params *p, int basis,
+                int pe, int normal, qkx_spinor_s *t, qkx_spinor_s *y);
+// device address of the no-allocation scratch buffer on the library's device
+// (qkx_noalloc.hip): taken once per device by qkx_init
+int noalloc_setup(int device);
+
 // BLAS (qkx_blas.hip): device-resident results, no host synchronisation
 int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);
 // *result = sum of partial[0..n) in a fixed order (double-double), one launch

@@ -53,6 +53,40 @@ int schur(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, doubl
     return hop(out, tmp_other, in, U, basis, 0, pe, -kappa * kappa, 0.0, 1.0, a, g5out, norm_partial, nblocks);
 }
 
+// Argument checks of qkx_tm_xx; *pe receives the output parity (0 even, 1 odd).
+int tm_xx_check(const qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_params *p,
+                int basis, int parity, int *pe)
+{
+    QKX_CHECK_ARG(out && in && U && p, "qkx_tm_xx: null argument");
+    QKX_CHECK_ARG(out != in, "qkx_tm_xx: out must differ from in");
+    QKX_CHECK_ARG(parity == QKX_PARITY_EE || parity == QKX_PARITY_OO, "Error wrong parity choise");
+    if (int rc = check_basis(basis)) return rc;
+    QKX_CHECK_ARG(same_lattice(out, U->L) && same_lattice(in, U->L), "qkx_tm_xx: lattices differ");
+    *pe = parity == QKX_PARITY_EE ? 0 : 1;
+    QKX_CHECK_ARG(holds_parity(out, *pe) && holds_parity(in, *pe), "qkx_tm_xx: field subsets do not match parity");
+    return QKX_SUCCESS;
+}
+
+// qkx_tm_xx on parity pe with the caller's intermediate fields: t on the
+// other parity, y (normal only) on parity pe; both of in's precision and
+// right-hand-side count.
+int tm_xx_apply(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_params *p, int basis,
+                int pe, int normal, qkx_spinor_s *t, qkx_spinor_s *y)
+{
+    const double k = p->kappa, a = twist_a(p), n = twist_norm(p);
+    int rc;
+    if (!normal) {
+        rc = schur(out, in, U, k, a, n, basis, pe, 0, t);
+    } else {
+        // tm_xxDagTm_xx :2024-2035: g5 M(-s) g5 M(s)
+        rc = schur(y, in, U, k, a, n, basis, pe, 1, t);
+        if (rc) return rc;
+        rc = schur(out, y, U, k, -a, n, basis, pe, 1, t);
+    }
+    if (rc) return rc;
+    return zero_parity(out, 1 - pe);
+}
+
 } // namespace qkx
 
 extern "C" {
@@ -141,29 +175,15 @@ int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, const qkx_
 int qkx_tm_xx(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, const qkx_params *p, int basis, int parity,
               int normal)
 {
-    QKX_CHECK_ARG(out && in && U && p, "qkx_tm_xx: null argument");
-    QKX_CHECK_ARG(out != in, "qkx_tm_xx: out must differ from in");
-    QKX_CHECK_ARG(parity == QKX_PARITY_EE || parity == QKX_PARITY_OO, "Error wrong parity choise");
-    if (int rc = check_basis(basis)) return rc;
-    int pe = parity == QKX_PARITY_EE ? 0 : 1;
-    QKX_CHECK_ARG(holds_parity(out, pe) && holds_parity(in, pe), "qkx_tm_xx: field subsets do not match parity");
-    double k = p->kappa, a = twist_a(p), n = twist_norm(p);
-    int sub_other = pe == 0 ? QKX_SUBSET_ODD : QKX_SUBSET_EVEN;
+    int pe = 0;
+    if (int rc = qkx::tm_xx_check(out, in, U, p, basis, parity, &pe)) return rc;
+    const int sub_other = pe == 0 ? QKX_SUBSET_ODD : QKX_SUBSET_EVEN;
     qkx::Temp t(in->L, in->prec, sub_other, in->nrhs);
     if (!t.f) return QKX_ERR_NOMEM;
-    int rc;
-    if (!normal) {
-        rc = qkx::schur(out, in, U, k, a, n, basis, pe, 0, t.f);
-    } else {
-        // tm_xxDagTm_xx :2024-2035: g5 M(-s) g5 M(s)
-        qkx::Temp y(in->L, in->prec, pe == 0 ? QKX_SUBSET_EVEN : QKX_SUBSET_ODD, in->nrhs);
-        if (!y.f) return QKX_ERR_NOMEM;
-        rc = qkx::schur(y.f, in, U, k, a, n, basis, pe, 1, t.f);
-        if (rc) return rc;
-        rc = qkx::schur(out, y.f, U, k, -a, n, basis, pe, 1, t.f);
-    }
-    if (rc) return rc;
-    return qkx::zero_parity(out, 1 - pe);
+    if (!normal) return qkx::tm_xx_apply(out, in, U, p, basis, pe, 0, t.f, nullptr);
+    qkx::Temp y(in->L, in->prec, pe == 0 ? QKX_SUBSET_EVEN : QKX_SUBSET_ODD, in->nrhs);
+    if (!y.f) return QKX_ERR_NOMEM;
+    return qkx::tm_xx_apply(out, in, U, p, basis, pe, 1, t.f, y.f);
 }
 
 } // extern "C"
