@@ -370,6 +370,17 @@ int qkx_solve_gcr(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
                   const qkx_gauge *U_net, qkx_net *net, const qkx_params *p,
                   const qkx_gcr_opts *o, qkx_solver_stats *st);
 
+/* No reference counterpart: the solvers' global reduction, exposed so that
+ * its compensation can be checked (the reference sums with MPI_Allreduce,
+ * lib/blas_qkxTM.cpp:24).  *res = the sum of the n doubles at vals (host),
+ * added on the device the way the CG kernels add their per-workgroup
+ * partials: in double-double, in a fixed order, every consuming workgroup
+ * alike.  mode 0: eight workgroups each summing all n (the consumer side of
+ * the single-RHS CG; they must agree bit for bit), 1: the one-workgroup
+ * pre-sum used when partials are many, 2: the batched solver's row sum
+ * (two stages beyond 256 rows). */
+int qkx_reduce_sum(const double *vals, int n, int mode, double *res);
+
 /* Relative true residual |b - D x| / |b| in FP64, full lattice, UKQCD. */
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b,
                       const qkx_gauge *U, const qkx_params *p);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "qkx_solve_batch": [_P, _P, _P, _P, ctypes.POINTER(Params), _DP, ctypes.POINTER(SolveOpts),
                         ctypes.POINTER(SolverStats)],
     "qkx_true_residual": [_DP, _P, _P, _P, ctypes.POINTER(Params)],
+    "qkx_reduce_sum": [_P, _I, _I, _DP],
     "qkx_cg_record": [_P, _P, _P, ctypes.POINTER(Params), _I, ctypes.POINTER(_P), _I, _IP,
                       ctypes.POINTER(SolverStats)],
     "qkx_net_create": [ctypes.POINTER(_P), _I], "qkx_net_destroy": [_P], "qkx_net_nweights": [_P, _IP],

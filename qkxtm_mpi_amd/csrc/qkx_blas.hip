@@ -17,23 +17,6 @@ namespace {
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
-struct DD {
-    double hi, lo;
-};
-__device__ __forceinline__ DD two_sum(double a, double b)
-{
-    double s = a + b;
-    double bb = s - a;
-    double err = (a - (s - bb)) + (b - bb);
-    return {s, err};
-}
-__device__ __forceinline__ DD dd_add(DD a, DD b)
-{
-    DD s = two_sum(a.hi, b.hi);
-    double lo = s.lo + a.lo + b.lo;
-    return two_sum(s.hi, lo);
-}
-
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -82,7 +65,7 @@ __global__ __launch_bounds__(kRedThreads) void finish_kernel(const double *parti
 {
     __shared__ double lh[kRedThreads], ll[kRedThreads];
     DD acc{0.0, 0.0};
-    for (int b = threadIdx.x; b < n; b += blockDim.x) acc = dd_add(acc, DD{partial[b], 0.0});
+    for (int b = threadIdx.x; b < n; b += blockDim.x) acc = dd_add(acc, partial[b]);
     lh[threadIdx.x] = acc.hi;
     ll[threadIdx.x] = acc.lo;
     __syncthreads();
@@ -94,7 +77,7 @@ __global__ __launch_bounds__(kRedThreads) void finish_kernel(const double *parti
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *result = lh[0] + ll[0];
+    if (threadIdx.x == 0) *result = dd_value(DD{lh[0], ll[0]});
 }
 
 // x = ax*x + ay*y   (scalars from host or from device memory *sc)

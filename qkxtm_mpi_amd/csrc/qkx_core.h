@@ -81,6 +81,9 @@ int tm_xx_apply(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U,
 // (qkx_noalloc.hip): taken once per device by qkx_init
 int noalloc_setup(int device);
 
+// the batched solver's compensated per-RHS row sums (qkx_solvers_batch.hip)
+int sum_rows_dev(const double *part, int nblocks, int nr, double *dst);
+
 // BLAS (qkx_blas.hip): device-resident results, no host synchronisation
 int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);
 // *result = sum of partial[0..n) in a fixed order (double-double), one launch

@@ -579,24 +579,13 @@ template <class SX, class SR, class SP> struct CgFin {
     int npap;
     double *r2_part;          // out: per-workgroup |r|^2 after the update
     double alpha;
-    // p.Ap in the same fixed order in every workgroup: strided per-thread
-    // sums, an xor butterfly (the same bits in every lane), then the waves in
-    // order
+    // p.Ap, compensated (double-double), in the same fixed order in every
+    // workgroup: strided per-thread sums, an xor butterfly (the same bits in
+    // every lane), then the waves in order
     template <class A> __device__ __forceinline__ void begin(const A &)
     {
-        __shared__ double lds[4];
-        double v = 0;
-        for (int i = threadIdx.x; i < npap; i += blockDim.x) v += pap_part[i];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-        __syncthreads();
-        double pap = lds[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) pap += lds[w];
-        alpha = *r2cur / pap;
-        __syncthreads();
-    }
-};
+        __shared__ double lh[4], ll[4];
+        DD v{0.0, 0.0};
 
 // What a batched hop (hop_share_kernel) does with its result in a CG
 // iteration: alpha_r = |r_r|^2 / (p.Ap)_r (0 for a frozen right-hand side:

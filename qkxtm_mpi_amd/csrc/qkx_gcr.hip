@@ -88,12 +88,12 @@ __global__ void beta_kernel(const double *part, int nb, int nw, const double *wn
 {
     const int j = threadIdx.x;
     if (j >= nw) return;
-    double re = 0, im = 0;
+    DD re{0.0, 0.0}, im{0.0, 0.0};
     for (int b = 0; b < nb; b++) {
-        re += part[(size_t)b * 2 * MAXM + 2 * j];
-        im += part[(size_t)b * 2 * MAXM + 2 * j + 1];
+        re = dd_add(re, part[(size_t)b * 2 * MAXM + 2 * j]);
+        im = dd_add(im, part[(size_t)b * 2 * MAXM + 2 * j + 1]);
     }
-    beta[j] = make_double2(re / wn2[j], im / wn2[j]);
+    beta[j] = make_double2(dd_value(re) / wn2[j], dd_value(im) / wn2[j]);
 }
 
 // w -= sum beta_j W_j, z -= sum beta_j Z_j; partials of |w|^2 and <w, r>
@@ -136,12 +136,13 @@ __global__ __launch_bounds__(kT) void orth_kernel(double2 *w, double2 *z, VecSet
 __global__ void alpha_kernel(const double *part, int nb, double *wn2_i, double2 *alpha)
 {
     if (threadIdx.x != 0) return;
-    double n = 0, re = 0, im = 0;
+    DD dn{0.0, 0.0}, dre{0.0, 0.0}, dim{0.0, 0.0};
     for (int b = 0; b < nb; b++) {
-        n += part[(size_t)b * 3];
-        re += part[(size_t)b * 3 + 1];
-        im += part[(size_t)b * 3 + 2];
+        dn = dd_add(dn, part[(size_t)b * 3]);
+        dre = dd_add(dre, part[(size_t)b * 3 + 1]);
+        dim = dd_add(dim, part[(size_t)b * 3 + 2]);
     }
+    const double n = dd_value(dn), re = dd_value(dre), im = dd_value(dim);
     *wn2_i = n;
     *alpha = n > 0 ? make_double2(re / n, im / n) : make_double2(0.0, 0.0);
 }
@@ -176,9 +177,9 @@ __global__ __launch_bounds__(kT) void xr_kernel(double2 *x, double2 *r, const do
 __global__ void sum_kernel(const double *part, int nb, double *out)
 {
     if (threadIdx.x != 0) return;
-    double s = 0;
-    for (int b = 0; b < nb; b++) s += part[b];
-    *out = s;
+    DD s{0.0, 0.0};
+    for (int b = 0; b < nb; b++) s = dd_add(s, part[b]);
+    *out = dd_value(s);
 }
 
 struct State {

@@ -65,6 +65,43 @@ template <class T> __device__ __forceinline__ Cx<T> cjmac(Cx<T> acc, Cx<T> a, Cx
 }
 
 // ---------------------------------------------------------------------------
+// Compensated (double-double) sums for the global reductions of the solvers
+// and BLAS: a sum is carried as hi + lo, and every addition keeps its exact
+// rounding error (Knuth's TwoSum), so adding n per-workgroup partials keeps
+// about twice FP64's precision instead of drifting by up to n ulps.  dd_add
+// is symmetric in its operands (the two low parts are added together before
+// they meet the error term), so an xor butterfly leaves the same bits in
+// every lane and every workgroup that sums the same partials in the same
+// order takes the same decisions.
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    const DD s = two_sum(a.hi, b.hi);
+    return two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+__device__ __forceinline__ DD dd_add(DD a, double b)
+{
+    const DD s = two_sum(a.hi, b);
+    return two_sum(s.hi, s.lo + a.lo);
+}
+// xor-butterfly sum over the 64 lanes of a wavefront (same bits in every lane)
+__device__ __forceinline__ DD dd_wave_sum(DD v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dd_add(v, DD{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)});
+    return v;
+}
+__device__ __forceinline__ double dd_value(DD v) { return v.hi + v.lo; }
+
+// ---------------------------------------------------------------------------
 // geometry passed to kernels
 // Division of an index n < 2^30 by a fixed d as a multiply and a shift:
 // n / d = (n * m) >> s with s = 30 + ceil(log2 d), m = floor(2^s / d) + 1

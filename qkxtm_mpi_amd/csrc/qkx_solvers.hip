@@ -27,6 +27,7 @@
 #include <chrono>
 #include <memory>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -70,39 +71,45 @@ int ensure_state(int nblocks)
     return QKX_SUCCESS;
 }
 
-// Every thread of the workgroup gets sum(v[0..n)), added in a fixed order.
-// Sum of v[0..n) seen by every thread of the workgroup, in an order fixed by
-// n and the workgroup size only, so all workgroups of a launch agree bit for
-// bit.  The xor butterfly leaves the same bits in every lane (each step adds
-// the same two operands, in either order).
+// Sum of v[0..n) seen by every thread of the workgroup, compensated
+// (double-double), in an order fixed by n and the workgroup size only, so all
+// workgroups of a launch agree bit for bit.  The xor butterfly leaves the same
+// bits in every lane (dd_add is symmetric in its operands).
 __device__ double block_sum_array(const double *v, int n)
 {
-    __shared__ double lds[kThreads / 64];
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __shared__ double lh[kThreads / 64], ll[kThreads / 64];
+    DD s{0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s = dd_add(s, v[i]);
+    s = dd_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) {
+        lh[threadIdx.x >> 6] = s.hi;
+        ll[threadIdx.x >> 6] = s.lo;
+    }
     __syncthreads();
-    double r = lds[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r += lds[w];
-    __syncthreads();   // lds is reused by the next call
-    return r;
+    DD r{lh[0], ll[0]};
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dd_add(r, DD{lh[w], ll[w]});
+    __syncthreads();   // lh, ll are reused by the next call
+    return dd_value(r);
 }
 
-// out[0] = sum(part[0..n)) in a fixed order, by one workgroup
+// out[0] = sum(part[0..n)), compensated, in a fixed order, by one workgroup
 __global__ __launch_bounds__(1024) void sum_partials_kernel(const double *part, int n, double *out)
 {
-    __shared__ double lds[1024];
-    double v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
-    lds[threadIdx.x] = v;
+    __shared__ double lh[1024], ll[1024];
+    DD v{0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = dd_add(v, part[i]);
+    lh[threadIdx.x] = v.hi;
+    ll[threadIdx.x] = v.lo;
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
+        if ((int)threadIdx.x < st) {
+            const DD c = dd_add(DD{lh[threadIdx.x], ll[threadIdx.x]}, DD{lh[threadIdx.x + st], ll[threadIdx.x + st]});
+            lh[threadIdx.x] = c.hi;
+            ll[threadIdx.x] = c.lo;
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = lds[0];
+    if (threadIdx.x == 0) out[0] = dd_value(DD{lh[0], ll[0]});
 }
 
 // partials (part, n) as the consumers should read them: pre-summed into
@@ -214,6 +221,13 @@ __global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, int n, cons
     for (int k = 0; k < 12; k++)
         out[k] = {(typename S::Real)(rv[k].re + b * T(pv[k].re)), (typename S::Real)(rv[k].im + b * T(pv[k].im))};
     p.store(s, out);
+}
+
+// every workgroup sums v[0..n) with block_sum_array and writes its result
+__global__ __launch_bounds__(kThreads) void block_sum_probe_kernel(const double *v, int n, double *out)
+{
+    const double r = block_sum_array(v, n);
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
 // per-workgroup |r|^2
@@ -635,6 +649,44 @@ int qkx_cg_record(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const 
                       "qkx_cg_record: store fields must be single full fields on the lattice of b");
     if (nstored) *nstored = 0;
     return cg_impl(x, b, U, p, st, every, store, nstore, nstored);
+}
+
+int qkx_reduce_sum(const double *vals, int n, int mode, double *res)
+{
+    QKX_CHECK_ARG(vals && res && n > 0 && mode >= 0 && mode <= 2, "qkx_reduce_sum: bad arguments");
+    if (int rc = qkx::ensure_init()) return rc;
+    constexpr int kProbeGroups = 8;
+    double *d = nullptr;
+    QKX_HIP(hipMalloc(&d, sizeof(double) * ((size_t)n + kProbeGroups)));
+    double *out = d + n;
+    hipError_t e = hipMemcpyAsync(d, vals, sizeof(double) * n, hipMemcpyHostToDevice, qkx::g_stream);
+    int rc = QKX_SUCCESS;
+    if (e == hipSuccess) {
+        if (mode == 0) {
+            hipLaunchKernelGGL(block_sum_probe_kernel, dim3(kProbeGroups), dim3(kThreads), 0, qkx::g_stream,
+                               (const double *)d, n, out);
+        } else if (mode == 1) {
+            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, (const double *)d, n, out);
+        } else {
+            rc = qkx::sum_rows_dev(d, n, 1, out);
+        }
+        e = hipGetLastError();
+    }
+    double h[kProbeGroups] = {};
+    const int nout = mode == 0 ? kProbeGroups : 1;
+    if (e == hipSuccess && rc == QKX_SUCCESS)
+        e = hipMemcpyAsync(h, out, sizeof(double) * nout, hipMemcpyDeviceToHost, qkx::g_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(qkx::g_stream);
+    (void)hipFree(d);
+    QKX_HIP(e);
+    if (rc) return rc;
+    for (int g = 1; g < nout; g++)
+        if (std::memcmp(&h[g], &h[0], sizeof(double)) != 0) {
+            qkx::set_error("qkx_reduce_sum: workgroups disagree");
+            return QKX_ERR_HIP;
+        }
+    *res = h[0];
+    return QKX_SUCCESS;
 }
 
 int qkx_true_residual(double *res, const qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_params *p)

@@ -40,7 +40,7 @@ double *g_bpart = nullptr;
 int g_bpart_cap = 0;
 double *g_bres = nullptr, *g_bres_host = nullptr;
 double *g_bred = nullptr;   // [0, NR) p.Ap, [NR, 2 NR) |r|^2, pre-summed when the partials are many
-double *g_bstage = nullptr; // first-stage sums of many partials: kStageRows rows of NR
+double *g_bstage = nullptr; // first-stage sums of many partials: kStageRows rows of NR (hi, lo) pairs
 constexpr int kStageRows = 256;
 constexpr int kDirectPartials = 2048;   // workgroups' partials read directly by each consumer
 
@@ -52,7 +52,7 @@ int ensure_bstate(int nblocks)
         QKX_HIP(hipMalloc(&g_bres, NR * sizeof(double)));
         QKX_HIP(hipHostMalloc((void **)&g_bres_host, NR * sizeof(double), hipHostMallocDefault));
         QKX_HIP(hipMalloc(&g_bred, 2 * NR * sizeof(double)));
-        QKX_HIP(hipMalloc(&g_bstage, kStageRows * NR * sizeof(double)));
+        QKX_HIP(hipMalloc(&g_bstage, 2 * kStageRows * NR * sizeof(double)));   // (hi, lo) pairs
     }
     size_t need = (size_t)nblocks * NR;
     if ((int)need > g_bpart_cap) {
@@ -64,22 +64,23 @@ int ensure_bstate(int nblocks)
     return QKX_SUCCESS;
 }
 
-// sums[r] = sum_b part[b*nr + r] for r < nr, in a fixed order; every thread
-// of the workgroup sees the same values.
+// sums[r] = sum_b part[b*nr + r] for r < nr, compensated (double-double), in
+// a fixed order; every thread of the workgroup sees the same values.
 __device__ void block_sums_rhs(const double *part, int nblocks, int nr, double *sums)
 {
-    __shared__ double lds[kT];
+    __shared__ double lh[kT], ll[kT];
     const int groups = blockDim.x / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
-    double s = 0;
+    DD s{0.0, 0.0};
     if (g < groups)
-        for (int b = g; b < nblocks; b += groups) s += part[(size_t)b * nr + r];
-    lds[threadIdx.x] = s;
+        for (int b = g; b < nblocks; b += groups) s = dd_add(s, part[(size_t)b * nr + r]);
+    lh[threadIdx.x] = s.hi;
+    ll[threadIdx.x] = s.lo;
     __syncthreads();
     if ((int)threadIdx.x < nr) {
-        double t = 0;
-        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
-        sums[threadIdx.x] = t;
+        DD t{0.0, 0.0};
+        for (int gg = 0; gg < groups; gg++) t = dd_add(t, DD{lh[gg * nr + threadIdx.x], ll[gg * nr + threadIdx.x]});
+        sums[threadIdx.x] = dd_value(t);
     }
     __syncthreads();
 }
@@ -223,43 +224,52 @@ __global__ __launch_bounds__(kT) void normb_partial_kernel(S r, int n, int nr, d
     block_partial_rhs(acc, valid, nr, part);
 }
 
-// out[r] = sum_b part[b*nr + r], fixed order, one workgroup of 1024
-__global__ __launch_bounds__(1024) void sum_rhs_kernel(const double *part, int nblocks, int nr, double *out)
+// out[r] = sum_b part[b*nr + r], compensated, fixed order, one workgroup of
+// 1024.  dd_in: the rows are (hi, lo) pairs left by sum_rhs_chunks_kernel.
+__global__ __launch_bounds__(1024) void sum_rhs_kernel(const double *part, int nblocks, int nr, double *out,
+                                                       int dd_in)
 {
-    __shared__ double lds[1024];
+    __shared__ double lh[1024], ll[1024];
     const int groups = blockDim.x / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
-    double v = 0;
+    DD v{0.0, 0.0};
     if (g < groups)
-        for (int b = g; b < nblocks; b += groups) v += part[(size_t)b * nr + r];
-    lds[threadIdx.x] = v;
+        for (int b = g; b < nblocks; b += groups) {
+            const size_t i = (size_t)b * nr + r;
+            v = dd_in ? dd_add(v, DD{part[2 * i], part[2 * i + 1]}) : dd_add(v, part[i]);
+        }
+    lh[threadIdx.x] = v.hi;
+    ll[threadIdx.x] = v.lo;
     __syncthreads();
     if ((int)threadIdx.x < nr) {
-        double t = 0;
-        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
-        out[threadIdx.x] = t;
+        DD t{0.0, 0.0};
+        for (int gg = 0; gg < groups; gg++) t = dd_add(t, DD{lh[gg * nr + threadIdx.x], ll[gg * nr + threadIdx.x]});
+        out[threadIdx.x] = dd_value(t);
     }
 }
 
 // First stage of a many-row sum: workgroup g sums rows [g chunk, (g+1) chunk)
-// of part into out[g*nr + r], in a fixed order (strided per thread, then the
-// thread groups in order).
+// of part, compensated, into the (hi, lo) pair out[2 (g*nr + r)], in a fixed
+// order (strided per thread, then the thread groups in order).
 __global__ __launch_bounds__(kT) void sum_rhs_chunks_kernel(const double *part, int nblocks, int nr, int chunk,
                                                             double *out)
 {
-    __shared__ double lds[kT];
+    __shared__ double lh[kT], ll[kT];
     const int groups = blockDim.x / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
     const int b0 = blockIdx.x * chunk, b1 = min(nblocks, b0 + chunk);
-    double v = 0;
+    DD v{0.0, 0.0};
     if (g < groups)
-        for (int b = b0 + g; b < b1; b += groups) v += part[(size_t)b * nr + r];
-    lds[threadIdx.x] = v;
+        for (int b = b0 + g; b < b1; b += groups) v = dd_add(v, part[(size_t)b * nr + r]);
+    lh[threadIdx.x] = v.hi;
+    ll[threadIdx.x] = v.lo;
     __syncthreads();
     if ((int)threadIdx.x < nr) {
-        double t = 0;
-        for (int gg = 0; gg < groups; gg++) t += lds[gg * nr + threadIdx.x];
-        out[(size_t)blockIdx.x * nr + threadIdx.x] = t;
+        DD t{0.0, 0.0};
+        for (int gg = 0; gg < groups; gg++) t = dd_add(t, DD{lh[gg * nr + threadIdx.x], ll[gg * nr + threadIdx.x]});
+        const size_t i = (size_t)blockIdx.x * nr + threadIdx.x;
+        out[2 * i] = t.hi;
+        out[2 * i + 1] = t.lo;
     }
 }
 
@@ -290,10 +300,11 @@ void sum_rows(const double *part, int nblocks, int nr, double *dst)
         const int g = (nblocks + chunk - 1) / chunk;
         hipLaunchKernelGGL(sum_rhs_chunks_kernel, dim3(g), dim3(kT), 0, qkx::g_stream, part, nblocks, nr, chunk,
                            g_bstage);
-        part = g_bstage;
-        nblocks = g;
+        hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, (const double *)g_bstage, g, nr,
+                           dst, 1);
+        return;
     }
-    hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, nblocks, nr, dst);
+    hipLaunchKernelGGL(sum_rhs_kernel, dim3(1), dim3(1024), 0, qkx::g_stream, part, nblocks, nr, dst, 0);
 }
 
 // per-rhs partials (part, nblocks) as consumers should read them: pre-summed
@@ -544,6 +555,15 @@ int brestart(BEngine &E, qkx_spinor_s *p, qkx_spinor_s *r, int initial)
 
 namespace qkx {
 extern int cg_batch_size;
+// the batched solver's per-right-hand-side row sums (two stages beyond
+// kStageRows rows), for qkx_reduce_sum
+int sum_rows_dev(const double *part, int nblocks, int nr, double *dst)
+{
+    if (int rc = ensure_bstate(64)) return rc;
+    sum_rows(part, nblocks, nr, dst);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
 }
 
 extern "C" {
