@@ -48,13 +48,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="single",
-                    choices=["single", "learned", "propagator", "production", "smear"],
+                    choices=["single", "learned", "propagator", "production", "smear", "batch"],
                     help="single: the 24^3x48 single-RHS solve (default, the headline line); "
                          "learned: the 16^3x32 learned-preconditioner study (trains on the GPU, then compares "
                          "solvers on unseen gauge fields); propagator: 12 point sources at 32^3x64 in one "
                          "batch plus multi-mu batches at 16^3x32 and 24^3x48; production: the 12-source "
                          "propagator at 48^3x96; smear: source preparation at 48^3x96 (20 APE steps, 50 "
-                         "Gaussian steps on 12 point sources and on 12 sink fields)")
+                         "Gaussian steps on 12 point sources and on 12 sink fields); batch: the 12-RHS even-odd "
+                         "hop per inner precision at 24^3x48 and 32^3x64, per launch shape and autotuned")
     ap.add_argument("--chunk", type=int, default=12, help="right-hand sides per batch (propagator workloads)")
     ap.add_argument("--inner", default="fp16", choices=["fp16", "bf16", "fp32"],
                     help="inner precision of the batched propagator solves")
@@ -345,6 +346,32 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
             tseq += t1
         mu_rows.append({"lattice": list(Lm), "mu": mus.tolist(), "batch_s": tm_, "sequential_s": tseq,
                         "iterations": stm[0].iterations, "max_true_residual": max(s.true_residual for s in stm)})
+    uneven = None
+    if mu_batches:
+        # every lattice of mu_batches at the three masses, all instances in
+        # one lockstep solve (one launch per CG kernel), against the same
+        # instances solved one after the other
+        X, B, Us, Ps = [], [], [], []
+        for Lm in mu_batches:
+            um = q.Gauge(Lm, boundaryT=-1, host=q.gen_gauge(Lm, 0.3, 3))
+            src = q.gen_z2(Lm, 9)
+            for m in (0.004, 0.01, 0.02):
+                X.append(q.Spinor(Lm))
+                B.append(q.Spinor(Lm, host=src))
+                Us.append(um)
+                Ps.append(q.params(Lm, kap, m, 1, -1))
+        t_multi, stmu = timed(q, lambda: q.solve_multi(X, B, Us, Ps, inner_prec=P, delta=delta, tol=TOL,
+                                                        maxIter=50000), 3)
+        t_seq, its_seq = 0.0, []
+        for i in range(len(X)):
+            t1, s1 = timed(q, lambda: q.solve(X[i], B[i], Us[i], Ps[i], eo=True, inner_prec=P, delta=delta,
+                                               tol=TOL, maxIter=50000), 3)
+            t_seq += t1
+            its_seq.append(s1.iterations)
+        uneven = {"instances": [{"lattice": list(Lm), "mu": m} for Lm in mu_batches for m in (0.004, 0.01, 0.02)],
+                  "lockstep_s": t_multi, "sequential_s": t_seq, "speedup": t_seq / t_multi,
+                  "lockstep_iterations": [s.iterations for s in stmu], "sequential_iterations": its_seq,
+                  "max_true_residual": max(s.true_residual for s in stmu)}
     out = {
         "metric": f"time-to-1e-10 resid, 12-source propagator at {'x'.join(map(str, L))}, batched mixed-precision "
                   f"CG vs plain FP64 CG",
@@ -362,6 +389,7 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
         "dslash": {"fp64_tm_us": us64, "fp64_tm_GBs": by64 / n64 / (us64 * 1e-6) / 1e9,
                    f"{args.inner}_eo_hop_us": uslo, f"{args.inner}_eo_hop_GBs": bylo / nlo / (uslo * 1e-6) / 1e9},
         "multi_mu": mu_rows,
+        "uneven_instances": uneven,
         "assembly": {"rescale_us": us_rescale, "rotate_us": us_rotate,
                      "rescale_GBs": bya / (us_rescale * 1e-6) / 1e9, "rotate_GBs": bya / (us_rotate * 1e-6) / 1e9,
                      "note": "Vector::rescale(2 kappa) and rotateToPhysicalBasePlus on the 12-column propagator"},
@@ -531,6 +559,44 @@ def batch_kernels(q):
     print("batch kernels done", flush=True)
 
 
+def run_batch(q, args):
+    """The 12-RHS even-odd normal operator (4 hops) at 24^3x48 and 32^3x64 in
+    FP16, FP32 and BF16 storage on 12-real links: time per hop (HIP events,
+    20 calls after 3 warm-ups, median of 3) and compulsory bytes, with the
+    links fetched per thread, shared per workgroup, the fixed default shape,
+    and the autotuned shapes."""
+    rows = []
+    for Lb in ((24, 24, 24, 48), (32, 32, 32, 64)):
+        U = q.gen_gauge(Lb, EPS, 1)
+        u = q.Gauge(Lb, boundaryT=-1, host=U)
+        del U
+        p = q.params(Lb, KAPPA, MU, 1, -1)
+        src = np.stack([q.gen_gaussian(Lb, 100 + r) for r in range(12)])
+        for name, P in (("fp16", q.HALF), ("fp32", q.SINGLE), ("bf16", q.BF16)):
+            ul = q.Gauge(Lb, P, q.RECON_12, -1)
+            ul.convert_from(u)
+            b = q.Spinor(Lb, P, q.EVEN, host=src, nrhs=12)
+            o = q.Spinor(Lb, P, q.EVEN, nrhs=12)
+            fn = lambda: o.tm_xxDagTm_xx(b, ul, p, q.EE)   # noqa: E731
+            row = {"lattice": list(Lb), "storage": name}
+            for label, share, tune in (("per_thread_links", 0, 0), ("shared_links", 2, 0), ("default", 1, 0),
+                                       ("autotuned", 1, 1)):
+                q.set_batch_link_sharing(share)
+                q.set_autotune(tune)
+                n, by = bytes_per_call(q, fn)
+                us = loop_time(q, fn, n, calls=20, trials=3, warm=3)
+                row[label] = {"us_per_hop": us, "GB/s": by / n / (us * 1e-6) / 1e9,
+                              "frac": by / n / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
+            row["tuned_kinds"] = q.autotune_report().splitlines()
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+            del ul, b, o
+    q.set_batch_link_sharing(1)
+    q.set_autotune(1)
+    print(json.dumps({"metric": "12-RHS even-odd hop, time per launch and compulsory HBM bytes", "unit": "us",
+                      "higher_is_better": False, "n_gpus": 1, "rows": rows}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -549,6 +615,9 @@ def main():
         return
     if args.workload == "smear":
         run_smear(q, args)
+        return
+    if args.workload == "batch":
+        run_batch(q, args)
         return
     if args.batch_kernels_only:
         q.set_batch_link_sharing(args.batch_share)

@@ -137,6 +137,24 @@ int  qkx_set_site_order(int blocked);
  * (site, RHS) units; 2: always; 0: never (every thread fetches its links).
  * The hop results are identical. */
 int  qkx_set_batch_link_sharing(int on);
+/* The hop autotuner: on the first launch of a kind (spinor and link
+ * storage, right-hand sides, lattice sites, full or half lattice, dagger,
+ * xpay input, basis) it times the candidate launch shapes -- workgroup size
+ * (256 or 128 threads), links shared through LDS (batches of 3-16 RHS),
+ * pipeline depth -- on scratch fields of that kind and keeps the fastest for
+ * every later launch of the kind.  Each shape computes every output site
+ * with the same arithmetic: the same bits for FP16 and FP64 fields; FP32 and
+ * BF16 batches with shared links may round a contracted product-sum
+ * differently in their last bit.  The per-workgroup norm partials group
+ * sites differently.  mode 0: off (the fixed defaults), 1 (default):
+ * batched launches (nrhs > 1), 2: every launch.  Setting a mode clears the
+ * cache.  An explicit qkx_set_batch_link_sharing(0 or 2) wins over a tuned
+ * choice.  No tuning happens under graph capture or instrumentation. */
+int qkx_set_autotune(int mode);
+/* The tuned kinds, one line each ("... : block=B share=S depth=D T us
+ * (default T0 us)"), NUL-terminated into buf (len bytes); *needed receives
+ * the full length + 1. */
+int qkx_autotune_report(char *buf, int len, int *needed);
 /* CG iterations enqueued between host reads of the solver state (default 16;
  * 1 gives no launches past the stopping iteration). */
 int  qkx_set_cg_batch(int iterations);
@@ -320,6 +338,22 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
                     const qkx_gauge *U_inner, const qkx_params *p,
                     const double *mu, const qkx_solve_opts *o,
                     qkx_solver_stats *st);
+
+/* Solve several independent systems D_i x_i = b_i -- each on its own
+ * lattice (uneven volumes allowed), gauge field U[i] and parameters p[i]
+ * (kappa, mu, twistSign, boundaryT) -- in lockstep, even-odd preconditioned
+ * (o->eo must be 1) with the inner precision, reliable-update trigger,
+ * tolerance and iteration cap of o.  Every CG step is one launch of each
+ * kernel for all n <= 8 instances: the four hops of the normal operator, the
+ * update and the direction step, each workgroup working for one instance.
+ * Each instance keeps its own scalars and stops on its own true residual;
+ * reliable updates are taken per instance.  x[i], b[i]: single full FP64
+ * fields on U[i]'s lattice; U[i]: FP64 recon 18 (all SU(3) or none, so that
+ * their working copies share one storage); st (may be NULL) receives n
+ * entries.  No reference counterpart: the reference makes one solve per call
+ * (lib/solvers.cpp:16-81). */
+int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, const qkx_gauge *const *U,
+                    const qkx_params *p, const qkx_solve_opts *o, qkx_solver_stats *st);
 
 /* ---- learned preconditioner ----------------------------------------------- */
 /* No reference counterpart.  M = L2 L1, two gauge-equivariant

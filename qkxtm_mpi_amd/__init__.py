@@ -76,7 +76,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [], "qkx_mem_info": [_DP, _DP],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
-    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_gauge_su3_deviation": [_P, _DP], "qkx_set_batch_link_sharing": [_I], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
+    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_gauge_su3_deviation": [_P, _DP], "qkx_set_batch_link_sharing": [_I], "qkx_set_autotune": [_I], "qkx_autotune_report": [ctypes.c_char_p, _I, _IP], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
     "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
@@ -102,6 +102,8 @@ SIGNATURES = {
     "qkx_solve_batch": [_P, _P, _P, _P, ctypes.POINTER(Params), _DP, ctypes.POINTER(SolveOpts),
                         ctypes.POINTER(SolverStats)],
     "qkx_true_residual": [_DP, _P, _P, _P, ctypes.POINTER(Params)],
+    "qkx_solve_multi": [_I, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(Params),
+                        ctypes.POINTER(SolveOpts), ctypes.POINTER(SolverStats)],
     "qkx_reduce_sum": [_P, _I, _I, _DP],
     "qkx_cg_record": [_P, _P, _P, ctypes.POINTER(Params), _I, ctypes.POINTER(_P), _I, _IP,
                       ctypes.POINTER(SolverStats)],
@@ -521,6 +523,24 @@ def solve_batch(x: Spinor, b: Spinor, u: Gauge, p: Params, mu=None, eo=True, inn
     return list(st)
 
 
+def solve_multi(xs, bs, us, ps, inner_prec=DOUBLE, delta=0.1, tol=1e-10, maxIter=100000, allow_not_converged=False):
+    """Solve D_i x_i = b_i for independent instances -- each on its own
+    lattice, gauge field and parameters -- in lockstep, even-odd, every CG
+    launch serving all of them; returns a list of SolverStats."""
+    n = len(xs)
+    assert len(bs) == n and len(us) == n and len(ps) == n
+    o = SolveOpts(1, inner_prec, RECON_12, delta, 50, int(maxIter), tol)
+    st = (SolverStats * n)()
+    xa = (_P * n)(*[f.h.value for f in xs])
+    ba = (_P * n)(*[f.h.value for f in bs])
+    ua = (_P * n)(*[g.h.value for g in us])
+    pa = (Params * n)(*ps)
+    rc = lib().qkx_solve_multi(n, xa, ba, ua, pa, ctypes.byref(o), st)
+    if not (rc == NOT_CONVERGED and allow_not_converged):
+        _check(rc, "qkx_solve_multi")
+    return list(st)
+
+
 def cg_record(x: Spinor, b: Spinor, u: Gauge, p: Params, every: int, store):
     """Reference CG (as CG.__call__) that copies the iterate into store[i]
     after every `every` iterations; returns (SolverStats, number stored)."""
@@ -667,6 +687,20 @@ def set_cg_graph(on=True):
 
 def set_rhs_tile(rt=0):
     _check(lib().qkx_set_rhs_tile(int(rt)), "qkx_set_rhs_tile")
+
+
+def set_autotune(mode=1):
+    """Hop autotuner: 0 off, 1 batched launches (default), 2 every launch."""
+    _check(lib().qkx_set_autotune(int(mode)), "qkx_set_autotune")
+
+
+def autotune_report():
+    """The tuned launch kinds and their chosen shapes, one line each."""
+    n = ctypes.c_int()
+    _check(lib().qkx_autotune_report(None, 0, ctypes.byref(n)), "qkx_autotune_report")
+    buf = ctypes.create_string_buffer(max(1, n.value))
+    _check(lib().qkx_autotune_report(buf, n.value, ctypes.byref(n)), "qkx_autotune_report")
+    return buf.value.decode()
 
 
 def set_batch_link_sharing(mode=1):

@@ -3,6 +3,9 @@
 // pool of temporary fields.
 #include "qkx_internal.h"
 #include "qkx_core.h"
+#include <algorithm>
+#include <cstring>
+#include <string>
 #include <cstdio>
 #include <mutex>
 #include <vector>
@@ -99,6 +102,16 @@ qkx_spinor_s *pool_get_batch(const int L[4], int prec, int subset, int nrhs)
 }
 
 qkx_spinor_s *pool_get(const int L[4], int prec, int subset) { return pool_get_batch(L, prec, subset, 1); }
+
+void pool_release(qkx_spinor_s *f)
+{
+    for (size_t i = 0; i < g_pool.size(); i++)
+        if (g_pool[i].f == f) {
+            qkx_spinor_destroy(f);
+            g_pool.erase(g_pool.begin() + i);
+            return;
+        }
+}
 
 void pool_put(qkx_spinor_s *f)
 {
@@ -284,6 +297,25 @@ int qkx_set_batch_link_sharing(int on)
 {
     QKX_CHECK_ARG(on >= 0 && on <= 2, "qkx_set_batch_link_sharing: 0, 1 or 2");
     qkx::set_batch_link_sharing(on);
+    return QKX_SUCCESS;
+}
+
+int qkx_set_autotune(int mode)
+{
+    QKX_CHECK_ARG(mode >= 0 && mode <= 2, "qkx_set_autotune: 0, 1 or 2");
+    qkx::set_autotune(mode);
+    return QKX_SUCCESS;
+}
+
+int qkx_autotune_report(char *buf, int len, int *needed)
+{
+    const std::string r = qkx::tune_report();
+    if (needed) *needed = (int)r.size() + 1;
+    if (buf && len > 0) {
+        const size_t n = std::min((size_t)len - 1, r.size());
+        std::memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
     return QKX_SUCCESS;
 }
 

@@ -1,6 +1,7 @@
 // qkx_core.h -- internal entry points shared by the library's translation units.
 #pragma once
 #include <cstddef>
+#include <string>
 #include "qkx_hip.h"
 
 struct qkx_spinor_s;
@@ -18,6 +19,15 @@ int hop(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, cons
 int hop_batch(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U,
               int basis, int dagger, int par_out, const double coef[][4], int g5out,
               double *norm_partial = nullptr, int *nblocks_out = nullptr);
+// n <= 8 independent single-right-hand-side hops in one launch, instance i
+// on U[i]'s lattice: out[i] = epilogue_i(H in[i] (or H^dag), aux[i]) with
+// coef[i] = {cR, gR, cX, gX}; aux may be null (for all); norm_partial[i]
+// (optional) receives instance i's per-workgroup |out|^2 partials, count in
+// nblocks_out[i]; skip[i] (optional) makes instance i's workgroups return
+// when *skip[i] != 0.  Fields of one precision, links of one storage.
+int hop_multi(int n, qkx_spinor_s *const *out, const qkx_spinor_s *const *in, const qkx_spinor_s *const *aux,
+              const qkx_gauge_s *const *U, int dagger, int par_out, const double (*coef)[4], int g5out,
+              double *const *norm_partial = nullptr, int *nblocks_out = nullptr, const int *const *skip = nullptr);
 // site-local combination (qkx_dslash.hip)
 int site(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int basis, int keep_par,
          double cR, double gR, double cX, double gX, int g5out);
@@ -63,6 +73,12 @@ void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);
 void set_site_order(int blocked);
 void set_batch_link_sharing(int on);
+// hop autotuner (qkx_dslash.hip): 0 off, 1 batched launches, 2 every launch;
+// set_no_tune(1) keeps the calling thread's launches on cached or default
+// shapes (no scratch fields); tune_report lists the tuned launch kinds
+void set_autotune(int mode);
+void set_no_tune(int on);
+std::string tune_report();
 // even-odd Schur operator on parity pe (qkx_ops.cpp):
 //   out_pe = [A in - kappa^2 H A^-1 H in]_pe (then g5 if g5out); the optional
 //   norm_partial collects |out|^2 per workgroup of the second hop
@@ -111,6 +127,8 @@ double *result_slot(int k);
 qkx_spinor_s *pool_get(const int L[4], int prec, int subset);
 qkx_spinor_s *pool_get_batch(const int L[4], int prec, int subset, int nrhs);
 void pool_put(qkx_spinor_s *f);
+// destroys a pool field (it must not be in use elsewhere)
+void pool_release(qkx_spinor_s *f);
 
 // RAII temporary from the pool
 struct Temp {

@@ -15,7 +15,11 @@
 // include/hopping_term_chiral.h (chiral).  The table entries are the
 // reference's conventions and are necessarily identical to it.
 #include "qkx_internal.h"
+#include <algorithm>
+#include <map>
+#include <string>
 #include <type_traits>
+#include <vector>
 #include "qkx_core.h"
 
 namespace {
@@ -421,7 +425,7 @@ struct HopTrav {
     FastDiv dyb, dtb, dnyb;
 };
 
-template <class SIn, class SOut, class SAux, class G>
+template <class SIn, class SOut, class SAux, class G, int NEPI = QKX_MAX_RHS>
 struct HopArgs {
     Geom g;
     SIn in;
@@ -436,7 +440,7 @@ struct HopArgs {
     HopTrav tr;    // order in which the sites of a parity are walked
     const int *skip;   // when non-null and *skip != 0 the launch does nothing
     double *norm_partial;  // when non-null: per-workgroup (and per-rhs) sum of |out|^2
-    Epi epi[QKX_MAX_RHS];  // epilogue coefficients per right-hand side
+    Epi epi[NEPI];         // epilogue coefficients per right-hand side
 };
 
 // Sum v over the workgroup; thread 0 writes the result to dst.
@@ -473,29 +477,30 @@ __device__ __forceinline__ void block_partial_rhs(double v, int nr, int u0, doub
 // Hardware workgroup IDs are dealt round-robin to the 8 XCDs; remap them so
 // that each XCD walks one contiguous range of sites and its private L2 sees
 // the neighbour reuse of that range.
-__device__ __forceinline__ int logical_block(int nblk8)
+__device__ __forceinline__ int logical_block(int b, int nblk8)
 {
-    int b = blockIdx.x;
     if (nblk8 > 0) b = (b & 7) * nblk8 + (b >> 3);
     return b;
 }
 
 // Parity and unit index of this thread; a unit is one (site, group of
-// right-hand sides) of the output parity, `upsite` units per site.
-__device__ __forceinline__ void hop_unit(const Geom &g, int par_out, int nblk8, int upsite, int &p, int &u)
+// right-hand sides) of the output parity, `upsite` units per site.  bid is
+// the workgroup's index within its launch (or within its instance of a
+// multi-instance launch).
+__device__ __forceinline__ void hop_unit(const Geom &g, int par_out, int nblk8, int upsite, int &p, int &u, int bid)
 {
     if (par_out >= 0) {
         p = par_out;
-        u = logical_block(nblk8) * blockDim.x + threadIdx.x;
+        u = logical_block(bid, nblk8) * blockDim.x + threadIdx.x;
     } else if (nblk8 > 0) {
         // full lattice: XCD k owns one slab of checkerboard indices and
         // alternates between the two parities, so the links and spinors of a
         // slab (both parities) are fetched into one L2 only
-        const int b = blockIdx.x, i = b >> 3;
+        const int b = bid, i = b >> 3;
         p = i & 1;
         u = ((b & 7) * nblk8 + (i >> 1)) * blockDim.x + threadIdx.x;
     } else {
-        const int o = blockIdx.x * blockDim.x + threadIdx.x;
+        const int o = bid * blockDim.x + threadIdx.x;
         const int vhn = g.Vh * upsite;
         p = (o >= vhn) ? 1 : 0;
         u = o - p * vhn;
@@ -586,6 +591,19 @@ template <class SX, class SR, class SP> struct CgFin {
     {
         __shared__ double lh[4], ll[4];
         DD v{0.0, 0.0};
+        for (int i = threadIdx.x; i < npap; i += blockDim.x) v = dd_add(v, pap_part[i]);
+        v = dd_wave_sum(v);
+        if ((threadIdx.x & 63) == 0) {
+            lh[threadIdx.x >> 6] = v.hi;
+            ll[threadIdx.x >> 6] = v.lo;
+        }
+        __syncthreads();
+        DD pap{lh[0], ll[0]};
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) pap = dd_add(pap, DD{lh[w], ll[w]});
+        alpha = *r2cur / dd_value(pap);
+        __syncthreads();
+    }
+};
 
 // What a batched hop (hop_share_kernel) does with its result in a CG
 // iteration: alpha_r = |r_r|^2 / (p.Ap)_r (0 for a frozen right-hand side:
@@ -618,18 +636,20 @@ template <> struct HopWaves<SpinorD> { static constexpr int v = QKX_HOP64_WAVES;
 template <> struct HopWaves<SpinorH> { static constexpr int v = QKX_HOP16_WAVES; };
 template <> struct HopWaves<SpinorB> { static constexpr int v = QKX_HOP16_WAVES; };
 
-template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH, class Fin = StoreFin>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SIn>::v, 8))) void hop_kernel(
-    HopArgs<SIn, SOut, SAux, G> a, Fin fin)
+// The work of one hop workgroup: bid is its index within the launch (or
+// within its instance of a multi-instance launch), (p, u) its thread's parity
+// and unit (site x rhs) index within it (hop_unit), ep the epilogue of the
+// thread's right-hand side.  The caller selects ep from its own argument
+// block: a dynamic index into the argument structure here would make the
+// compiler copy the whole structure to scratch memory.
+template <int B, bool DAG, bool AUX, int DEPTH, class SIn, class SOut, class SAux, class G, int NEPI, class Fin>
+__device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI> &a, Fin &fin, int bid, int p, int u,
+                                         const Epi &ep)
 {
     using T = typename SOut::Real;
     constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
-    if (a.skip && *a.skip) return;   // uniform over the grid
-    fin.begin(a);
     const Geom &g = a.g;
     const int nr = a.nrhs;
-    int p, u;   // parity and unit (site x rhs) index within it
-    hop_unit(g, a.par_out, a.nblk8, nr, p, u);
     const int v = nr == 1 ? u : u / nr;   // position of the site in the walk
     const int rr = u - v * nr;
     double nrm2 = 0.0;
@@ -701,8 +721,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     }
 
     Cx<T> res[12];
-    Epi ep = a.epi[0];                 // uniform (scalar) load for a single rhs
-    if (nr > 1) ep = a.epi[rr];
     if constexpr (AUX) {
         Cx<T> xv[12];
         a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
@@ -743,16 +761,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     }
     }
     if constexpr (CG) {
-        block_partial(nrm2, fin.r2_part + blockIdx.x);
+        block_partial(nrm2, fin.r2_part + bid);
     } else if (a.norm_partial) {
         if (nr == 1) {
-            block_partial(nrm2, a.norm_partial + blockIdx.x);
+            block_partial(nrm2, a.norm_partial + bid);
         } else {
             // one partial per right-hand side, summed in thread order
             const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
-            block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
+            block_partial_rhs(nrm2, nr, u0, a.norm_partial + bid * nr);
         }
     }
+}
+
+template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH, class Fin = StoreFin>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SIn>::v, 8))) void hop_kernel(
+    HopArgs<SIn, SOut, SAux, G> a, Fin fin)
+{
+    if (a.skip && *a.skip) return;   // uniform over the grid
+    fin.begin(a);
+    int p, u;   // parity and unit (site x rhs) index within it
+    hop_unit(a.g, a.par_out, a.nblk8, a.nrhs, p, u, blockIdx.x);
+    Epi ep = a.epi[0];                 // uniform (scalar) load for a single rhs
+    if (a.nrhs > 1) ep = a.epi[u % a.nrhs];
+    hop_body<B, DAG, AUX, DEPTH>(a, fin, blockIdx.x, p, u, ep);
+}
+
+// Several independent hops -- each on its own lattice, gauge field and
+// fields, one right-hand side each -- in one launch: workgroups
+// [blk0[i], blk0[i + 1]) do instance i (each instance's range is a whole
+// number of 8-workgroup rounds, so the XCD remapping of an instance works as
+// in its own launch).  Used by the multi-instance solver.
+constexpr int kMaxInst = 8;
+template <class S, class G> struct MultiHop {
+    int n;
+    int blk0[kMaxInst + 1];
+    HopArgs<S, S, S, G, 1> inst[kMaxInst];
+};
+
+template <class S, class G, int B, bool DAG, bool AUX, int DEPTH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<S>::v, 8))) void hop_multi_kernel(
+    MultiHop<S, G> m)
+{
+    int i = 0;
+    while (i + 1 < m.n && (int)blockIdx.x >= m.blk0[i + 1]) i++;
+    // The instance's arguments are read in place from the kernel argument
+    // segment, where the launch put the bytes of m (the only argument, at
+    // offset 0): indexing the by-value parameter with i would make the
+    // compiler copy all of it to scratch memory first.
+    const auto *mk = (const MultiHop<S, G> *)__builtin_amdgcn_kernarg_segment_ptr();
+    const HopArgs<S, S, S, G, 1> &a = mk->inst[i];
+    if (a.skip && *a.skip) return;   // uniform over the instance
+    StoreFin fin;
+    const int bid = (int)blockIdx.x - m.blk0[i];
+    int p, u;
+    hop_unit(a.g, a.par_out, a.nblk8, 1, p, u, bid);
+    hop_body<B, DAG, AUX, DEPTH>(a, fin, bid, p, u, a.epi[0]);
 }
 
 // Batches of FP16 spinors on FP16 12-real links, 8 to 16 right-hand sides:
@@ -767,6 +830,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
 // occupancy).  Two RHS would need 128 sites and 43 KB, which costs occupancy.
 constexpr int kShareSites = 32;
 constexpr int kShareSitesFew = 85;
+constexpr int kShareSitesF32 = 64;   // FP32 links in LDS: 64 sites = 41 KB (3-7 RHS)
 constexpr int kShareMinRhs = 3;
 template <class F> __device__ __forceinline__ double *fin_r2_part(const F &f)
 {
@@ -791,7 +855,7 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
     const Geom &g = a.g;
     const int nr = a.nrhs;
     int p, u;
-    hop_unit(g, a.par_out, a.nblk8, nr, p, u);
+    hop_unit(g, a.par_out, a.nblk8, nr, p, u, blockIdx.x);
     const int v = u / nr;
     const int rr = u - v * nr;
     const int j = threadIdx.x / nr;   // site within the workgroup
@@ -916,6 +980,140 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
     }
 }
 
+// Batches of FP32 or BF16 spinors on 12-real links (FP32 or BF16), 3-16
+// right-hand sides: as hop_share_kernel, a workgroup holds whole sites, each
+// site's 8 links are fetched and their third rows rebuilt once per
+// workgroup, here into LDS as 9 FP32 complex numbers (72 B per link, the
+// boundary sign of the wrapping t hops already applied), and every (site,
+// RHS) thread multiplies its projected neighbour by them.  The per-thread
+// arithmetic is that of hop_kernel; the compiler may contract its products
+// and sums into FMAs differently in the two kernels, so results agree to
+// the last FP32 bit (one rounding of the storage format).
+template <class S, class G, int B, bool DAG, bool AUX, int SITES>
+__global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
+{
+    using T = float;
+    if (a.skip && *a.skip) return;   // uniform over the grid
+    __shared__ int lsite[SITES][8];               // link site of each (site, direction); -1: no site
+    __shared__ float lsign[SITES][8];             // boundary sign of the hop
+    __shared__ Cx<float> lU[SITES][8][9];         // the rebuilt links
+    const Geom &g = a.g;
+    const int nr = a.nrhs;
+    int p, u;
+    hop_unit(g, a.par_out, a.nblk8, nr, p, u, blockIdx.x);
+    const int v = u / nr;
+    const int rr = u - v * nr;
+    const int j = threadIdx.x / nr;
+    int cb = v;
+    HopSite hs{};
+    const bool valid = v < g.Vh;
+    if (valid) hs = hop_site_walk(g, a.tr, p, v, cb);
+    const int q = p ^ 1;
+    const int qoff = q * g.Vh, poff = p * g.Vh;
+    if (rr == 0) {
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            lsite[j][d] = !valid ? -1 : (d & 1) ? qoff + hs.nb[d] : poff + cb;
+            lsign[j][d] = (d == 6 && hs.tw_p) || (d == 7 && hs.tw_m) ? (float)g.bc : 1.0f;
+        }
+    }
+    __syncthreads();
+    const int nsites = blockDim.x / nr;
+    for (int t = threadIdx.x; t < nsites * 8; t += blockDim.x) {
+        const int js = t >> 3, d = t & 7;
+        const int sl = lsite[js][d];
+        if (sl >= 0) {
+            Cx<float> Ul[9];
+            a.U.unpack(a.U.fetch(d >> 1, sl), Ul);
+            const float sg = lsign[js][d];
+#pragma unroll
+            for (int k = 0; k < 9; k++) lU[js][d][k] = {sg * Ul[k].re, sg * Ul[k].im};
+        }
+    }
+    __syncthreads();
+    double nrm2 = 0.0;
+    if (valid) {
+        const int in_q = a.in_off[q];
+        Cx<T> R[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) R[k] = {T(0), T(0)};
+        constexpr int FW = DAG ? 1 : 0;
+        constexpr int BW = DAG ? 0 : 1;
+        const S &in = a.in;
+        // one hop with the link from LDS: project, multiply, reconstruct
+        auto hop = [&](auto MUC, auto KINDC, auto DAGLC, const typename S::Raw &raw, int d) {
+            constexpr int MU = decltype(MUC)::value, KIND = decltype(KINDC)::value;
+            constexpr bool DAGL = decltype(DAGLC)::value;
+            Cx<T> psi[12];
+            in.template unpack<spin_mask<B, MU, KIND>()>(raw, psi);
+            Cx<T> h[6];
+            project_k<B, MU, KIND, 0>(psi, h);
+            project_k<B, MU, KIND, 1>(psi, h);
+            Cx<T> Ul[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) Ul[k] = lU[j][d][k];
+            Cx<T> xi[6];
+            link_mul<DAGL, kFmaChain<S>>(Ul, h, xi);
+            collect_s<B, MU, KIND, 0>(R, xi);
+            collect_s<B, MU, KIND, 1>(R, xi);
+            collect_s<B, MU, KIND, 2>(R, xi);
+            collect_s<B, MU, KIND, 3>(R, xi);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        using IF = std::integral_constant<int, FW>;
+        using IB = std::integral_constant<int, BW>;
+        using BT = std::true_type;
+        using BF = std::false_type;
+        // neighbour spinors one hop ahead of the arithmetic
+        auto f0 = in.template fetch<spin_mask<B, 0, FW>()>(in_q + hs.nb[0] * nr + rr);
+        auto f1 = in.template fetch<spin_mask<B, 0, BW>()>(in_q + hs.nb[1] * nr + rr);
+        hop(I0{}, IF{}, BF{}, f0, 0);
+        QKX_SCHED_FENCE();
+        auto f2 = in.template fetch<spin_mask<B, 1, FW>()>(in_q + hs.nb[2] * nr + rr);
+        hop(I0{}, IB{}, BT{}, f1, 1);
+        QKX_SCHED_FENCE();
+        auto f3 = in.template fetch<spin_mask<B, 1, BW>()>(in_q + hs.nb[3] * nr + rr);
+        hop(I1{}, IF{}, BF{}, f2, 2);
+        QKX_SCHED_FENCE();
+        auto f4 = in.template fetch<spin_mask<B, 2, FW>()>(in_q + hs.nb[4] * nr + rr);
+        hop(I1{}, IB{}, BT{}, f3, 3);
+        QKX_SCHED_FENCE();
+        auto f5 = in.template fetch<spin_mask<B, 2, BW>()>(in_q + hs.nb[5] * nr + rr);
+        hop(I2{}, IF{}, BF{}, f4, 4);
+        QKX_SCHED_FENCE();
+        auto f6 = in.template fetch<spin_mask<B, 3, FW>()>(in_q + hs.nb[6] * nr + rr);
+        hop(I2{}, IB{}, BT{}, f5, 5);
+        QKX_SCHED_FENCE();
+        auto f7 = in.template fetch<spin_mask<B, 3, BW>()>(in_q + hs.nb[7] * nr + rr);
+        hop(I3{}, IF{}, BF{}, f6, 6);
+        QKX_SCHED_FENCE();
+        hop(I3{}, IB{}, BT{}, f7, 7);
+
+        Cx<T> res[12];
+        const Epi ep = a.epi[rr];
+        if constexpr (AUX) {
+            Cx<T> xv[12];
+            a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
+            epilogue<B>(ep, R, xv, res);
+        } else {
+            epilogue<B, T>(ep, R, nullptr, res);
+        }
+        a.out.store(a.out_off[p] + cb * nr + rr, res);
+        if (a.norm_partial) {
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+        }
+    }
+    if (a.norm_partial) {
+        const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+        block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
+    }
+}
+
 // One hop for RT right-hand sides of one site: the link is fetched and
 // unpacked once, then applied to the RT neighbour spinors (consecutive in
 // the batch layout).
@@ -966,7 +1164,7 @@ __global__ __launch_bounds__(256) void hop_tile_kernel(HopArgs<SIn, SOut, SAux, 
     const Geom &g = a.g;
     const int nr = a.nrhs, ng = nr / RT;
     int p, u;
-    hop_unit(g, a.par_out, a.nblk8, ng, p, u);
+    hop_unit(g, a.par_out, a.nblk8, ng, p, u, blockIdx.x);
     const int cb = u / ng;
     const int r0 = (u - cb * ng) * RT;
     double nrm2[RT];
@@ -1089,6 +1287,43 @@ bool share_links_for(int nrhs, long units)
 int g_site_order = 0;   // 1: blocked walk (HopTrav), 0 (default): checkerboard index order
 const int *g_skip = nullptr;
 
+// Launch shape of one hop: workgroup size, links shared through LDS by the
+// (site, RHS) threads of a workgroup, and the software-pipeline depth of the
+// per-thread kernel.  Every shape computes each output site with the same
+// arithmetic: the same bits for FP16 and FP64, and for FP32/BF16 with shared
+// links up to the compiler's FMA contraction (last bit); the per-workgroup
+// norm partials group the sites differently.
+struct HopShape {
+    int block;
+    int share;
+    int depth;
+};
+// The autotuner: on the first launch of a kind (spinor and link storage,
+// right-hand sides, output sites, full or half lattice, dagger, xpay input,
+// basis) it times the candidate shapes on scratch fields of that kind and
+// caches the fastest for every later launch of the kind.
+struct TuneKey {
+    int sp, gp, gr, nrhs, sites, full, dag, aux, basis;
+    bool operator<(const TuneKey &o) const
+    {
+        const int x[9] = {sp, gp, gr, nrhs, sites, full, dag, aux, basis};
+        const int y[9] = {o.sp, o.gp, o.gr, o.nrhs, o.sites, o.full, o.dag, o.aux, o.basis};
+        for (int i = 0; i < 9; i++)
+            if (x[i] != y[i]) return x[i] < y[i];
+        return false;
+    }
+};
+struct TuneEntry {
+    HopShape shape;
+    double us;            // its measured time per launch
+    double default_us;    // the time of the untuned default shape
+};
+std::map<TuneKey, TuneEntry> g_tuned;
+int g_autotune = 1;                 // 0 off, 1 batched launches (nrhs > 1), 2 every launch
+const HopShape *g_force = nullptr;  // set while the tuner times a candidate
+bool g_tuning = false;
+thread_local int g_no_tune = 0;     // callers that must not allocate (scratch fields) set this
+
 // Blocked walk for a launch moving site_bytes per output site: rows of y in
 // blocks of up to 8, and as many t slices per block (up to 8, dividing L3)
 // as keep one z plane of a block -- the traffic between a site's use as a
@@ -1119,7 +1354,74 @@ HopTrav hop_trav(const Geom &g, int par_out, double site_bytes)
 template <class S, class G, int B, class Fin = StoreFin>
 int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
                  const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, double *norm_partial,
-                 int *nblocks_out, const Fin &fin = Fin{})
+                 int *nblocks_out, const Fin &fin = Fin{});
+
+// Time the candidate shapes of one launch kind on scratch fields of that
+// kind (same precision, parity subset and right-hand-side count; links as
+// given) and record the fastest in g_tuned.  Nothing is recorded when the
+// stream is being captured into a graph, when instrumentation is on, or when
+// the scratch fields cannot be had; the launch then keeps its default shape.
+template <class S, class G, int B>
+void tune_hop(const TuneKey &key, const qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
+              const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, const HopShape &dflt, bool shareable)
+{
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(qkx::g_stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
+    if (qkx::prof_active()) return;
+    std::vector<HopShape> cands;
+    const bool sh_any = shareable && g_share_links != 0, sh_only = shareable && g_share_links == 2;
+    for (int block : {256, 128}) {
+        if (sh_any) cands.push_back({block, 1, dflt.depth});
+        if (!sh_only)
+            for (int depth : {2, 1}) cands.push_back({block, 0, depth});
+    }
+    qkx_spinor_s *so = qkx::pool_get_batch(out->L, out->prec, out->subset, out->nrhs);
+    qkx_spinor_s *si = qkx::pool_get_batch(in->L, in->prec, in->subset, in->nrhs);
+    qkx_spinor_s *sa = aux ? qkx::pool_get_batch(aux->L, aux->prec, aux->subset, aux->nrhs) : nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = so && si && (!aux || sa) && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    ok = ok && qkx_spinor_zero(si) == QKX_SUCCESS && (!sa || qkx_spinor_zero(sa) == QKX_SUCCESS);
+    g_tuning = true;
+    double best = 1e300, dflt_us = -1;
+    HopShape pick = dflt;
+    for (const HopShape &c : cands) {
+        if (!ok) break;
+        g_force = &c;
+        constexpr int kWarm = 2, kReps = 5;
+        int rc = 0;
+        for (int i = 0; i < kWarm && !rc; i++) rc = launch_hop_t<S, G, B>(so, si, sa, U, dagger, par_out, e, nullptr, nullptr);
+        (void)hipEventRecord(e0, qkx::g_stream);
+        for (int i = 0; i < kReps && !rc; i++) rc = launch_hop_t<S, G, B>(so, si, sa, U, dagger, par_out, e, nullptr, nullptr);
+        (void)hipEventRecord(e1, qkx::g_stream);
+        float ms = 0;
+        if (rc || hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            ok = false;
+            break;
+        }
+        const double us = 1e3 * ms / kReps;
+        if (c.block == dflt.block && c.share == dflt.share && (c.share || c.depth == dflt.depth)) dflt_us = us;
+        if (us < best) {
+            best = us;
+            pick = c;
+        }
+    }
+    g_force = nullptr;
+    g_tuning = false;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    // the scratch fields are freed, not kept in the pool: at the production
+    // volume they are gigabytes each
+    if (so) qkx::pool_release(so);
+    if (si) qkx::pool_release(si);
+    if (sa) qkx::pool_release(sa);
+    (void)hipGetLastError();
+    if (ok && best < 1e300) g_tuned[key] = TuneEntry{pick, best, dflt_us};
+}
+
+template <class S, class G, int B, class Fin>
+int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
+                 const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, double *norm_partial,
+                 int *nblocks_out, const Fin &fin)
 {
     HopArgs<S, S, S, G> a;
     a.g = qkx::make_geom(U->L, U->bc);
@@ -1147,12 +1449,41 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     // reconstructs its sites' links once into LDS (hop_share_kernel); its
     // size is a whole number of sites
     constexpr bool bcg = IsBCg<Fin>::value;   // batched CG update fused (shared links only)
-    constexpr bool share_ok = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> &&
-                              (std::is_same_v<Fin, StoreFin> || bcg) && QKX_HOP16_DOT2;
+    constexpr bool store = std::is_same_v<Fin, StoreFin>;
+    constexpr bool share16 = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> &&
+                             (store || bcg) && QKX_HOP16_DOT2;
+    // FP32 and BF16 batches on 12-real links share FP32 links (hop_share_f_kernel)
+    constexpr bool share32 = store && ((std::is_same_v<S, SpinorF> && std::is_same_v<G, GaugeF12>) ||
+                                       (std::is_same_v<S, SpinorB> && std::is_same_v<G, Gauge16<true>>));
+    constexpr bool share_ok = share16 || share32;
     const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
-    const bool share = share_ok && rt == 1 && share_links_for(a.nrhs, (long)sites_out * a.nrhs);
+    HopShape sh{g_block, share_ok && rt == 1 && share_links_for(a.nrhs, (long)sites_out * a.nrhs) ? 1 : 0,
+                sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32};
+    if constexpr (store) {
+        if (g_force) {
+            sh = *g_force;
+        } else if (rt == 1 && !g_tuning && !g_no_tune && (g_autotune == 2 || (g_autotune == 1 && a.nrhs > 1))) {
+            const TuneKey key{in->prec, U->prec, U->recon, a.nrhs, sites_out, par_out < 0 ? 1 : 0, dagger,
+                              aux ? 1 : 0, B};
+            auto it = g_tuned.find(key);
+            if (it == g_tuned.end()) {
+                tune_hop<S, G, B>(key, out, in, aux, U, dagger, par_out, e, sh, share_ok && a.nrhs >= kShareMinRhs);
+                it = g_tuned.find(key);
+            }
+            if (it != g_tuned.end()) {
+                const int dshare = sh.share;
+                sh = it->second.shape;
+                if (g_share_links != 1) sh.share = dshare;   // an explicit setting wins
+            }
+        }
+    }
     const bool few = a.nrhs < 8;   // 3-7 RHS: the larger site capacity
-    const int bs = share ? a.nrhs * (g_block / a.nrhs) : g_block;   // threads per workgroup
+    // sites per workgroup of the shared-link kernels: as many as the
+    // workgroup holds, up to the LDS capacity of the kernel instance
+    const int share_cap = few ? (share32 ? kShareSitesF32 : kShareSitesFew) : kShareSites;
+    const bool share = share_ok && rt == 1 && sh.share && a.nrhs >= kShareMinRhs && a.nrhs <= QKX_MAX_RHS &&
+                       (share32 || sh.block / a.nrhs <= share_cap);
+    const int bs = share ? a.nrhs * std::min(share_cap, sh.block / a.nrhs) : sh.block;   // threads per workgroup
     int nb = (sites_out * upsite + bs - 1) / bs;
     a.nblk8 = 0;
     if (g_xcd_remap && nb >= 64) {
@@ -1177,7 +1508,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     if (aux && aux != in) bytes += (double)nout * ws;
     a.tr = hop_trav(a.g, par_out, bytes / sites_out);
     qkx::prof_begin();
-    const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
+    const int depth = sh.depth;
     const dim3 grid(nb), blk(bs);
     hipStream_t st = qkx::g_stream;
 #define QKX_LAUNCH_HOP(DG, AX, DP) \
@@ -1190,7 +1521,6 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     } while (0)
     // the fused CG hop always has an aux field; RHS tiles exist for stored
     // UKQCD-basis batches only (chiral batches use one RHS per thread)
-    constexpr bool store = std::is_same_v<Fin, StoreFin>;
     constexpr bool tiles = store && B == QKX_BASIS_UKQCD;
     if (!tiles) rt = 1;
     if constexpr (bcg) {
@@ -1206,8 +1536,20 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
             if (dagger) hipLaunchKernelGGL((hop_share_kernel<B, true, true, Fin>), grid, blk, 0, st, a, fin);
             else hipLaunchKernelGGL((hop_share_kernel<B, false, true, Fin>), grid, blk, 0, st, a, fin);
         }
+    } else if (share && share32) {
+        if constexpr (share32) {
+#define QKX_LAUNCH_SHARE(DG, AX, NS) hipLaunchKernelGGL((hop_share_f_kernel<S, G, B, DG, AX, NS>), grid, blk, 0, st, a)
+            if (few) {
+                if (dagger) { if (aux) QKX_LAUNCH_SHARE(true, true, kShareSitesF32); else QKX_LAUNCH_SHARE(true, false, kShareSitesF32); }
+                else        { if (aux) QKX_LAUNCH_SHARE(false, true, kShareSitesF32); else QKX_LAUNCH_SHARE(false, false, kShareSitesF32); }
+            } else {
+                if (dagger) { if (aux) QKX_LAUNCH_SHARE(true, true, kShareSites); else QKX_LAUNCH_SHARE(true, false, kShareSites); }
+                else        { if (aux) QKX_LAUNCH_SHARE(false, true, kShareSites); else QKX_LAUNCH_SHARE(false, false, kShareSites); }
+            }
+#undef QKX_LAUNCH_SHARE
+        }
     } else if (share) {
-        if constexpr (share_ok) {
+        if constexpr (share16) {
 #define QKX_LAUNCH_SHARE(DG, AX, NS) \
     hipLaunchKernelGGL((hop_share_kernel<B, DG, AX, StoreFin, NS>), grid, blk, 0, st, a, fin)
             if (few) {
@@ -1242,6 +1584,83 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     return QKX_SUCCESS;
 }
 
+// n independent single-RHS hops in one launch (hop_multi_kernel), UKQCD
+// basis, each instance on its own lattice: the fixed default launch shape
+// (256 threads, the precision's pipeline depth, XCD remapping per instance).
+template <class S, class G>
+int launch_hop_multi_t(int n, qkx_spinor_s *const *out, const qkx_spinor_s *const *in,
+                       const qkx_spinor_s *const *aux, const qkx_gauge_s *const *U, int dagger, int par_out,
+                       const Epi *e, double *const *norm_partial, int *nblocks_out, const int *const *skip)
+{
+    MultiHop<S, G> m{};
+    m.n = n;
+    const int bs = 256;
+    int total = 0;
+    double bytes = 0;
+    for (int i = 0; i < n; i++) {
+        HopArgs<S, S, S, G, 1> &a = m.inst[i];
+        a.g = qkx::make_geom(U[i]->L, U[i]->bc);
+        a.in = make_spinor<S>(in[i]);
+        a.out = make_spinor<S>(out[i]);
+        a.aux = aux && aux[i] ? make_spinor<S>(aux[i]) : a.out;
+        a.U = make_gauge<G>(U[i]);
+        qkx::parity_offsets(in[i], a.in_off);
+        qkx::parity_offsets(out[i], a.out_off);
+        if (aux && aux[i]) qkx::parity_offsets(aux[i], a.aux_off);
+        else a.aux_off[0] = a.aux_off[1] = 0;
+        a.par_out = par_out;
+        a.nrhs = 1;
+        const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
+        a.nout = sites_out;
+        a.epi[0] = e[i];
+        a.skip = skip ? skip[i] : nullptr;
+        a.norm_partial = norm_partial ? norm_partial[i] : nullptr;
+        a.tr = HopTrav{};
+        int nb = (sites_out + bs - 1) / bs;
+        a.nblk8 = 0;
+        if (g_xcd_remap && nb >= 64) {
+            if (par_out >= 0) {
+                nb = (nb + 7) / 8 * 8;
+                a.nblk8 = nb / 8;
+            } else {
+                const int nbh = ((a.g.Vh + bs - 1) / bs + 7) / 8 * 8;
+                a.nblk8 = nbh / 8;
+                nb = 2 * nbh;
+            }
+        } else {
+            nb = (nb + 7) / 8 * 8;   // keep the next instance's range aligned to the XCD rounds
+        }
+        if (nblocks_out) nblocks_out[i] = nb;
+        m.blk0[i] = total;
+        total += nb;
+        const double ws = (double)qkx::spinor_elem_bytes(in[i]->prec) * qkx::spinor_nchunks(in[i]->prec) +
+                          (in[i]->prec == QKX_PREC_HALF ? 4.0 : 0.0);
+        const double wg = (U[i]->prec == QKX_PREC_DOUBLE ? 16.0 : U[i]->prec == QKX_PREC_SINGLE ? 8.0 : 4.0) *
+                          (U[i]->recon == QKX_RECON_18 ? 9 : 6);
+        bytes += (double)sites_out * ((par_out < 0 ? 4 : 8) * wg + 2 * ws) +
+                 (aux && aux[i] && aux[i] != in[i] ? (double)sites_out * ws : 0.0);
+    }
+    m.blk0[n] = total;
+    qkx::prof_begin();
+    const int depth = sizeof(typename S::Real) == 8 ? g_depth64 : g_depth32;
+    const dim3 grid(total), blk(bs);
+    hipStream_t st = qkx::g_stream;
+    constexpr int U0 = QKX_BASIS_UKQCD;
+#define QKX_LAUNCH_MULTI(DG, AX, DP) hipLaunchKernelGGL((hop_multi_kernel<S, G, U0, DG, AX, DP>), grid, blk, 0, st, m)
+    const bool ax = aux != nullptr;
+    if (depth == 2) {
+        if (dagger) { if (ax) QKX_LAUNCH_MULTI(true, true, 2); else QKX_LAUNCH_MULTI(true, false, 2); }
+        else        { if (ax) QKX_LAUNCH_MULTI(false, true, 2); else QKX_LAUNCH_MULTI(false, false, 2); }
+    } else {
+        if (dagger) { if (ax) QKX_LAUNCH_MULTI(true, true, 1); else QKX_LAUNCH_MULTI(true, false, 1); }
+        else        { if (ax) QKX_LAUNCH_MULTI(false, true, 1); else QKX_LAUNCH_MULTI(false, false, 1); }
+    }
+#undef QKX_LAUNCH_MULTI
+    qkx::prof_end(bytes, in[0]->prec);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
 template <class S, int B>
 int launch_site_t(qkx_spinor_s *out, const qkx_spinor_s *y, const qkx_spinor_s *x, int keep_par, const EpiSet &e)
 {
@@ -1268,6 +1687,32 @@ void set_rhs_tile(int rt) { g_rhs_tile = rt; }
 void set_site_order(int blocked) { g_site_order = blocked; }
 
 void set_batch_link_sharing(int on) { g_share_links = on; }
+
+void set_autotune(int mode)
+{
+    g_autotune = mode;
+    g_tuned.clear();
+}
+
+void set_no_tune(int on) { g_no_tune = on; }
+
+// "kind shape time default_time" lines of the tuned launch kinds
+std::string tune_report()
+{
+    std::string r;
+    for (const auto &kv : g_tuned) {
+        const TuneKey &k = kv.first;
+        const TuneEntry &t = kv.second;
+        char line[256];
+        snprintf(line, sizeof line,
+                 "spinor_prec=%d gauge_prec=%d recon=%d nrhs=%d sites=%d full=%d dagger=%d xpay=%d basis=%d : "
+                 "block=%d share=%d depth=%d %.2f us (default %.2f us)\n",
+                 k.sp, k.gp, k.gr, k.nrhs, k.sites, k.full, k.dag, k.aux, k.basis, t.shape.block, t.shape.share,
+                 t.shape.depth, t.us, t.default_us);
+        r += line;
+    }
+    return r;
+}
 
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
@@ -1413,6 +1858,39 @@ int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U
     if (sp == QKX_PREC_BF16 && U->prec == QKX_PREC_BF16 && U->recon == QKX_RECON_12)
         return launch_hop_cg<SpinorB, GaugeB12>(in, aux, U, dagger, par_out, e, f, nblocks_out);
     set_error("hop_cg: unsupported spinor/gauge precision combination");
+    return QKX_ERR_ARG;
+}
+
+// n independent single-RHS hops, each on its own lattice and links, in one
+// launch; see qkx_core.h
+int hop_multi(int n, qkx_spinor_s *const *out, const qkx_spinor_s *const *in, const qkx_spinor_s *const *aux,
+              const qkx_gauge_s *const *U, int dagger, int par_out, const double (*coef)[4], int g5out,
+              double *const *norm_partial, int *nblocks_out, const int *const *skip)
+{
+    if (n < 1 || n > kMaxInst) {
+        set_error("hop_multi: 1 to 8 instances");
+        return QKX_ERR_ARG;
+    }
+    Epi e[kMaxInst];
+    const int sp = in[0]->prec, gp = U[0]->prec, gr = U[0]->recon;
+    for (int i = 0; i < n; i++) {
+        if (in[i]->nrhs != 1 || out[i]->nrhs != 1 || in[i]->prec != sp || out[i]->prec != sp ||
+            (aux && (!aux[i] || aux[i]->prec != sp || aux[i]->nrhs != 1)) || U[i]->prec != gp || U[i]->recon != gr) {
+            set_error("hop_multi: single right-hand-side fields of one precision, links of one storage");
+            return QKX_ERR_ARG;
+        }
+        e[i] = Epi{coef[i][0], coef[i][1], coef[i][2], coef[i][3], g5out};
+    }
+    if (sp == QKX_PREC_DOUBLE && gp == QKX_PREC_DOUBLE)
+        return gr == QKX_RECON_18 ? launch_hop_multi_t<SpinorD, GaugeD18>(n, out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out, skip)
+                                  : launch_hop_multi_t<SpinorD, GaugeD12>(n, out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out, skip);
+    if (sp == QKX_PREC_SINGLE && gp == QKX_PREC_SINGLE && gr == QKX_RECON_12)
+        return launch_hop_multi_t<SpinorF, GaugeF12>(n, out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out, skip);
+    if (sp == QKX_PREC_HALF && gp == QKX_PREC_HALF && gr == QKX_RECON_12)
+        return launch_hop_multi_t<SpinorH, GaugeH12>(n, out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out, skip);
+    if (sp == QKX_PREC_BF16 && gp == QKX_PREC_BF16 && gr == QKX_RECON_12)
+        return launch_hop_multi_t<SpinorB, GaugeB12>(n, out, in, aux, U, dagger, par_out, e, norm_partial, nblocks_out, skip);
+    set_error("hop_multi: unsupported spinor/gauge precision combination");
     return QKX_ERR_ARG;
 }
 

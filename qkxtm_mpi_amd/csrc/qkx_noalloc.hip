@@ -102,7 +102,12 @@ int qkx_tm_xx_noalloc(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
     unsigned char *base = g_scratch_dev[dev];
     qkx_spinor_s t = field_at(base, in, pe == 0 ? QKX_SUBSET_ODD : QKX_SUBSET_EVEN);
     qkx_spinor_s y = field_at(base + one, in, pe == 0 ? QKX_SUBSET_EVEN : QKX_SUBSET_ODD);
-    return qkx::tm_xx_apply(out, in, U, p, basis, pe, normal, &t, normal ? &y : nullptr);
+    // the hop autotuner would allocate scratch fields for a kind it has not
+    // timed yet: such launches keep the cached or default shape here
+    qkx::set_no_tune(1);
+    const int rc = qkx::tm_xx_apply(out, in, U, p, basis, pe, normal, &t, normal ? &y : nullptr);
+    qkx::set_no_tune(0);
+    return rc;
 }
 
 } // extern "C"

@@ -25,6 +25,7 @@
 #include "qkx_internal.h"
 #include "qkx_core.h"
 #include <chrono>
+#include <algorithm>
 #include <memory>
 #include <cmath>
 #include <cstring>
@@ -550,6 +551,330 @@ double host_norm2(const qkx_spinor_s *f)
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// Several independent systems -- each on its own lattice, links, twisted mass
+// and kappa -- solved in lockstep: every CG step is one launch for all of them
+// (hop_multi for the stencil, the kernels below for the update and the
+// direction), each workgroup working for one instance.  Each instance keeps its
+// own slot (scalars, flags, iteration count); an instance whose flag is raised
+// skips its share of every launch until the host has dealt with it.
+constexpr int kMaxInst = 8;
+constexpr int kDone = 4;   // slot flag of an instance that has met its tolerance
+
+template <class SX, class SR, class S> struct MultiCg {
+    int n;
+    int blk0[kMaxInst + 1];        // workgroups [blk0[i], blk0[i + 1]) serve instance i
+    int nsites[kMaxInst];
+    SX x[kMaxInst];
+    SR r[kMaxInst];
+    S p[kMaxInst], Ap[kMaxInst];
+    const CGSlot *cur[kMaxInst];
+    CGSlot *next[kMaxInst];
+    const double *pap_part[kMaxInst];
+    int npap[kMaxInst];
+    double *r2_part[kMaxInst];     // written by the update, read by the direction step
+    int nr2[kMaxInst];
+    double stop2[kMaxInst];
+    double delta2;
+};
+
+__device__ __forceinline__ int multi_instance(const int *blk0, int n)
+{
+    int i = 0;
+    while (i + 1 < n && (int)blockIdx.x >= blk0[i + 1]) i++;
+    return i;
+}
+
+template <class SX, class SR, class S>
+__global__ __launch_bounds__(kThreads) void cgm_update_kernel(MultiCg<SX, SR, S> m)
+{
+    const int i = multi_instance(m.blk0, m.n);
+    const CGSlot *cur = m.cur[i];
+    if (cur->flag) return;
+    using T = typename SX::Real;
+    using TR = typename SR::Real;
+    using TP = typename S::Real;
+    const double pap = block_sum_array(m.pap_part[i], m.npap[i]);
+    const double alpha = cur->r2 / pap;
+    const T a = T(alpha);
+    const TR ar = TR(alpha);
+    const int bid = (int)blockIdx.x - m.blk0[i];
+    const int s = bid * blockDim.x + threadIdx.x;
+    double acc = 0;
+    if (s < m.nsites[i]) {
+        Cx<T> xv[12];
+        Cx<TR> rv[12];
+        Cx<TP> pv[12], av[12];
+        m.x[i].load(s, xv);
+        m.p[i].load(s, pv);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + a * T(pv[k].re), xv[k].im + a * T(pv[k].im)};
+        m.x[i].store(s, xv);
+        m.r[i].load(s, rv);
+        m.Ap[i].load(s, av);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            rv[k] = {rv[k].re - ar * TR(av[k].re), rv[k].im - ar * TR(av[k].im)};
+            acc += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+        }
+        m.r[i].store(s, rv);
+    }
+    block_partial(acc, m.r2_part[i] + bid);
+}
+
+template <class SX, class SR, class S>
+__global__ __launch_bounds__(kThreads) void cgm_dir_kernel(MultiCg<SX, SR, S> m)
+{
+    const int i = multi_instance(m.blk0, m.n);
+    const CGSlot *cur = m.cur[i];
+    CGSlot *next = m.next[i];
+    const int bid = (int)blockIdx.x - m.blk0[i];
+    if (cur->flag) {
+        if (bid == 0 && threadIdx.x == 0) *next = *cur;
+        return;
+    }
+    using T = typename SR::Real;
+    const int s = bid * blockDim.x + threadIdx.x;
+    const bool on = s < m.nsites[i];
+    Cx<typename S::Real> pv[12];
+    Cx<T> rv[12];
+    if (on) {
+        m.p[i].load(s, pv);
+        m.r[i].load(s, rv);
+    }
+    const double r2n = block_sum_array(m.r2_part[i], m.nr2[i]);
+    const double pap = block_sum_array(m.pap_part[i], m.npap[i]);
+    int flag = 0;
+    if (!(pap > 0.0) || !isfinite(r2n)) flag = 3;
+    else if (r2n <= m.stop2[i]) flag = 2;
+    else if (r2n < m.delta2 * cur->maxr2) flag = 1;
+    const double beta = r2n / cur->r2;
+    if (bid == 0 && threadIdx.x == 0) *next = CGSlot{r2n, cur->r2, pap, cur->maxr2, flag, cur->k + 1};
+    if (flag || !on) return;
+    const T b = T(beta);
+    Cx<typename S::Real> out[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        out[k] = {(typename S::Real)(rv[k].re + b * T(pv[k].re)), (typename S::Real)(rv[k].im + b * T(pv[k].im))};
+    m.p[i].store(s, out);
+}
+
+// per-instance pre-sums of many partials: workgroup i sums part[i][0..n[i])
+// into out[i] (compensated, fixed order)
+struct MultiSum {
+    const double *part[kMaxInst];
+    int cnt[kMaxInst];
+    double *out[kMaxInst];
+};
+__global__ __launch_bounds__(1024) void sum_partials_multi_kernel(MultiSum m)
+{
+    __shared__ double lh[1024], ll[1024];
+    const int i = blockIdx.x;
+    DD v{0.0, 0.0};
+    for (int k = threadIdx.x; k < m.cnt[i]; k += blockDim.x) v = dd_add(v, m.part[i][k]);
+    lh[threadIdx.x] = v.hi;
+    ll[threadIdx.x] = v.lo;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const DD c = dd_add(DD{lh[threadIdx.x], ll[threadIdx.x]}, DD{lh[threadIdx.x + st], ll[threadIdx.x + st]});
+            lh[threadIdx.x] = c.hi;
+            ll[threadIdx.x] = c.lo;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) m.out[i][0] = dd_value(DD{lh[0], ll[0]});
+}
+
+// Multi-instance solver state on the device: two rows of kMaxInst slots (the
+// slot parity of iteration j is j & 1), the per-instance partials of p.Ap and
+// |r|^2, and their pre-sums.
+struct MultiState {
+    CGSlot *slots = nullptr;        // [2][kMaxInst]
+    CGSlot *slots_host = nullptr;   // pinned [kMaxInst]
+    double *part = nullptr;         // per instance: [cap] p.Ap, then [cap] |r|^2
+    double *red = nullptr;          // [2 kMaxInst] pre-sums
+    size_t cap_total = 0;
+};
+MultiState g_ms;
+
+int ensure_multi(size_t part_doubles)
+{
+    if (!g_ms.slots) {
+        QKX_HIP(hipMalloc(&g_ms.slots, 2 * kMaxInst * sizeof(CGSlot)));
+        QKX_HIP(hipHostMalloc((void **)&g_ms.slots_host, kMaxInst * sizeof(CGSlot), hipHostMallocDefault));
+        QKX_HIP(hipMalloc(&g_ms.red, 2 * kMaxInst * sizeof(double)));
+    }
+    if (part_doubles > g_ms.cap_total) {
+        if (g_ms.part) (void)hipFree(g_ms.part);
+        g_ms.part = nullptr;
+        QKX_HIP(hipMalloc(&g_ms.part, part_doubles * sizeof(double)));
+        g_ms.cap_total = part_doubles;
+    }
+    return QKX_SUCCESS;
+}
+
+// One instance of a multi-instance solve: its operators, fields and scalars.
+struct MInst {
+    const qkx_gauge_s *Uop = nullptr, *Ui = nullptr, *Um = nullptr;
+    NormalOp Ah{}, Al{}, Am{};
+    std::vector<std::unique_ptr<qkx::Temp>> temps;
+    qkx_spinor_s *ho, *he, *lo, *le, *mo, *me, *sm, *rm, *bh, *bn, *xs, *s, *bo, *to, *r, *pp, *Ap, *xl;
+    double kappa = 0, a = 0, nrm = 1, b2 = 0, stop2 = 0;
+    double *pap = nullptr, *r2 = nullptr;   // partial regions
+    int npap = 0, nr2 = 0, nsites = 0, nrel = 0;
+    int state = 0;   // 0 running, 1 converged, 2 breakdown / non-finite
+    qkx_spinor_s *T(const int L[4], int prec, int subset, bool need = true)
+    {
+        if (!need) return nullptr;
+        temps.emplace_back(new qkx::Temp(L, prec, subset));
+        return temps.back()->f;
+    }
+};
+
+CGSlot *mslot(int j, int i) { return g_ms.slots + (j & 1) * kMaxInst + i; }
+
+template <class SX, class SR, class S>
+int enqueue_multi_iteration(int n, int j, MInst *I, double delta2)
+{
+    qkx_spinor_s *pp[kMaxInst], *Ap[kMaxInst], *lo[kMaxInst], *le[kMaxInst];
+    const qkx_gauge_s *Ui[kMaxInst];
+    const int *skip[kMaxInst];
+    double *pap[kMaxInst];
+    double c1p[kMaxInst][4], c2p[kMaxInst][4], c1m[kMaxInst][4], c2m[kMaxInst][4];
+    for (int i = 0; i < n; i++) {
+        pp[i] = I[i].pp;
+        Ap[i] = I[i].Ap;
+        lo[i] = I[i].lo;
+        le[i] = I[i].le;
+        Ui[i] = I[i].Ui;
+        skip[i] = &mslot(j, i)->flag;
+        pap[i] = I[i].pap;
+        const double k2 = I[i].kappa * I[i].kappa, a = I[i].a, nr = I[i].nrm;
+        const double a1[4] = {1.0 / nr, -a / nr, 0, 0}, a2[4] = {-k2, 0, 1.0, a};
+        const double b1[4] = {1.0 / nr, a / nr, 0, 0}, b2[4] = {-k2, 0, 1.0, -a};
+        for (int k = 0; k < 4; k++) {
+            c1p[i][k] = a1[k];
+            c2p[i][k] = a2[k];
+            c1m[i][k] = b1[k];
+            c2m[i][k] = b2[k];
+        }
+    }
+    // M^dag M = g5 M(-s) g5 M(s), every instance in each launch; the second
+    // hop of M(s) leaves each instance's |M p|^2 = p.Ap partials
+    int npap[kMaxInst];
+    int rc;
+    if ((rc = qkx::hop_multi(n, lo, pp, nullptr, Ui, 0, 1, c1p, 0, nullptr, nullptr, skip))) return rc;
+    if ((rc = qkx::hop_multi(n, le, lo, pp, Ui, 0, 0, c2p, 1, pap, npap, skip))) return rc;
+    if ((rc = qkx::hop_multi(n, lo, le, nullptr, Ui, 0, 1, c1m, 0, nullptr, nullptr, skip))) return rc;
+    if ((rc = qkx::hop_multi(n, Ap, lo, le, Ui, 0, 0, c2m, 1, nullptr, nullptr, skip))) return rc;
+    MultiCg<SX, SR, S> m{};
+    m.n = n;
+    m.delta2 = delta2;
+    MultiSum ps{}, rs{};
+    bool pre_pap = false, pre_r2 = false;
+    int tot = 0;
+    for (int i = 0; i < n; i++) {
+        m.blk0[i] = tot;
+        m.nsites[i] = I[i].nsites;
+        tot += nblk(I[i].nsites);
+        m.x[i] = make_spinor<SX>(I[i].xl ? I[i].xl : I[i].xs);
+        m.r[i] = make_spinor<SR>(I[i].r);
+        m.p[i] = make_spinor<S>(I[i].pp);
+        m.Ap[i] = make_spinor<S>(I[i].Ap);
+        m.cur[i] = mslot(j, i);
+        m.next[i] = mslot(j + 1, i);
+        m.pap_part[i] = I[i].pap;
+        m.npap[i] = npap[i];
+        m.r2_part[i] = I[i].r2;
+        m.nr2[i] = nblk(I[i].nsites);
+        m.stop2[i] = I[i].stop2;
+        ps.part[i] = I[i].pap;
+        ps.cnt[i] = npap[i];
+        ps.out[i] = g_ms.red + i;
+        rs.part[i] = I[i].r2;
+        rs.cnt[i] = m.nr2[i];
+        rs.out[i] = g_ms.red + kMaxInst + i;
+        if (npap[i] > kDirectPartials) pre_pap = true;
+        if (m.nr2[i] > kDirectPartials) pre_r2 = true;
+    }
+    m.blk0[n] = tot;
+    if (pre_pap) {
+        hipLaunchKernelGGL(sum_partials_multi_kernel, dim3(n), dim3(1024), 0, qkx::g_stream, ps);
+        for (int i = 0; i < n; i++) {
+            m.pap_part[i] = g_ms.red + i;
+            m.npap[i] = 1;
+        }
+    }
+    hipLaunchKernelGGL((cgm_update_kernel<SX, SR, S>), dim3(tot), dim3(kThreads), 0, qkx::g_stream, m);
+    if (pre_r2) {
+        hipLaunchKernelGGL(sum_partials_multi_kernel, dim3(n), dim3(1024), 0, qkx::g_stream, rs);
+        for (int i = 0; i < n; i++) {
+            m.r2_part[i] = g_ms.red + kMaxInst + i;
+            m.nr2[i] = 1;
+        }
+    }
+    hipLaunchKernelGGL((cgm_dir_kernel<SX, SR, S>), dim3(tot), dim3(kThreads), 0, qkx::g_stream, m);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
+int enqueue_multi(int lp, int n, int j, MInst *I, double delta2)
+{
+    switch (lp) {
+    case QKX_PREC_DOUBLE: return enqueue_multi_iteration<SpinorD, SpinorD, SpinorD>(n, j, I, delta2);
+    case QKX_PREC_SINGLE: return enqueue_multi_iteration<SpinorF, SpinorF, SpinorF>(n, j, I, delta2);
+    case QKX_PREC_HALF: return enqueue_multi_iteration<SpinorF, SpinorH, SpinorH>(n, j, I, delta2);
+    case QKX_PREC_BF16: return enqueue_multi_iteration<SpinorF, SpinorF, SpinorB>(n, j, I, delta2);
+    }
+    qkx::set_error("qkx_solve_multi: bad inner precision");
+    return QKX_ERR_ARG;
+}
+
+// (Re)start instance i from its residual r at iteration j: mode 0 p = r and
+// the slot from scratch, mode 1 p = r + beta p (beta = |r|^2 / r2_old);
+// writes the instance's slot of parity j + 1.
+int restart_inst(MInst &I, int i, int j, int mode)
+{
+    const int n = I.nsites;
+    int rc = with_spinor(I.r, [&](auto R) {
+        hipLaunchKernelGGL((norm_partial_kernel<decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, R, n,
+                           I.r2);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+    if (rc) return rc;
+    if (mode == 0 && (rc = qkx_spinor_copy(I.pp, I.r))) return rc;
+    const double *r2_src = I.r2;
+    int nr2 = nblk(n);
+    reduce_if_many(r2_src, nr2, g_ms.red + kMaxInst + i);
+    return with_spinor(I.pp, [&](auto P) {
+        return with_spinor(I.r, [&](auto R) {
+            hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0,
+                               qkx::g_stream, P, R, mode == 0 ? 0 : n, mslot(j, i), mslot(j + 1, i), r2_src, nr2,
+                               I.pap, 0, 0.0, 0.0, 1);
+            QKX_HIP(hipGetLastError());
+            return QKX_SUCCESS;
+        });
+    });
+}
+
+int read_mslots(int j, int n, CGSlot *h)
+{
+    QKX_HIP(hipMemcpyAsync(g_ms.slots_host, mslot(j, 0), n * sizeof(CGSlot), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    for (int i = 0; i < n; i++) h[i] = g_ms.slots_host[i];
+    return QKX_SUCCESS;
+}
+
+int write_mslots(int j, int n, const CGSlot *h)
+{
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));   // the pinned buffer may still feed an earlier copy
+    for (int i = 0; i < n; i++) g_ms.slots_host[i] = h[i];
+    QKX_HIP(hipMemcpyAsync(mslot(j, 0), g_ms.slots_host, n * sizeof(CGSlot), hipMemcpyHostToDevice, qkx::g_stream));
+    return QKX_SUCCESS;
+}
+
 } // namespace
 
 namespace qkx {
@@ -916,6 +1241,213 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         st->true_residual = tr;
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    return status;
+}
+
+int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, const qkx_gauge *const *U,
+                    const qkx_params *p, const qkx_solve_opts *o, qkx_solver_stats *st)
+{
+    QKX_CHECK_ARG(n >= 1 && n <= kMaxInst, "qkx_solve_multi: 1 to 8 instances");
+    QKX_CHECK_ARG(x && b && U && p && o, "qkx_solve_multi: null argument");
+    QKX_CHECK_ARG(o->eo, "qkx_solve_multi: the even-odd system only (eo = 1)");
+    QKX_CHECK_ARG(o->tol > 0 && o->maxIter > 0, "qkx_solve_multi: bad tol/maxIter");
+    const int lp = o->inner_prec;
+    QKX_CHECK_ARG(lp >= QKX_PREC_DOUBLE && lp <= QKX_PREC_BF16, "qkx_solve_multi: bad inner precision");
+    for (int i = 0; i < n; i++) {
+        QKX_CHECK_ARG(x[i] && b[i] && U[i], "qkx_solve_multi: null field");
+        QKX_CHECK_ARG(x[i]->prec == QKX_PREC_DOUBLE && b[i]->prec == QKX_PREC_DOUBLE &&
+                          x[i]->subset == QKX_SUBSET_FULL && b[i]->subset == QKX_SUBSET_FULL && x[i]->nrhs == 1 &&
+                          b[i]->nrhs == 1,
+                      "qkx_solve_multi: x and b must be single full FP64 fields");
+        QKX_CHECK_ARG(U[i]->prec == QKX_PREC_DOUBLE && U[i]->recon == QKX_RECON_18,
+                      "qkx_solve_multi: U must be FP64 recon 18");
+        for (int m = 0; m < 4; m++)
+            QKX_CHECK_ARG(x[i]->L[m] == U[i]->L[m] && b[i]->L[m] == U[i]->L[m], "qkx_solve_multi: lattices differ");
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const bool mixed = lp != QKX_PREC_DOUBLE;
+    const int xp = mixed ? QKX_PREC_SINGLE : QKX_PREC_DOUBLE;
+    const int rp = lp == QKX_PREC_BF16 ? QKX_PREC_SINGLE : lp;
+    const double tol2 = o->tol * o->tol;
+    int rc;
+    std::vector<MInst> I(n);
+    size_t part_need = 0;
+    for (int i = 0; i < n; i++) part_need += 2 * (size_t)(nblk(b[i]->Vh) + 16);
+    if ((rc = ensure_multi(part_need))) return rc;
+    size_t off = 0;
+    for (int i = 0; i < n; i++) {
+        MInst &M = I[i];
+        const int *L = b[i]->L;
+        M.nsites = b[i]->Vh;
+        M.pap = g_ms.part + off;
+        off += nblk(M.nsites) + 16;
+        M.r2 = g_ms.part + off;
+        off += nblk(M.nsites) + 16;
+        if ((rc = qkx::gauge_fp64_operator(U[i], &M.Uop))) return rc;
+        M.Ui = M.Uop;
+        M.Um = U[i];
+        qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U[i]);
+        if (mixed) {
+            if ((rc = qkx::gauge_cached(&Uc->inner, U[i], lp, QKX_RECON_12))) return rc;
+            M.Ui = Uc->inner;
+            if ((rc = qkx::gauge_cached(&Uc->mid, U[i], QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
+            M.Um = Uc->mid;
+        }
+        M.kappa = p[i].kappa;
+        M.a = 2.0 * p[i].kappa * p[i].mu * p[i].twistSign;
+        M.nrm = 1.0 + 4.0 * p[i].kappa * p[i].kappa * p[i].mu * p[i].mu;
+        const int D = QKX_PREC_DOUBLE, E = QKX_SUBSET_EVEN, O = QKX_SUBSET_ODD, F32 = QKX_PREC_SINGLE;
+        M.ho = M.T(L, D, O);
+        M.he = M.T(L, D, E);
+        M.lo = mixed ? M.T(L, lp, O) : M.ho;
+        M.le = mixed ? M.T(L, lp, E) : M.he;
+        M.mo = M.T(L, F32, O, mixed);
+        M.me = M.T(L, F32, E, mixed);
+        M.sm = M.T(L, F32, E, mixed);
+        M.rm = M.T(L, F32, E, mixed);
+        M.bh = M.T(L, D, E);
+        M.bn = M.T(L, D, E);
+        M.xs = M.T(L, D, E);
+        M.s = M.T(L, D, E);
+        M.bo = M.T(L, D, O);
+        M.to = M.T(L, D, O);
+        M.r = M.T(L, rp, E);
+        M.pp = M.T(L, lp, E);
+        M.Ap = M.T(L, lp, E);
+        M.xl = M.T(L, xp, E, mixed);
+        for (auto &t : M.temps)
+            if (!t->f) return QKX_ERR_NOMEM;
+        M.Ah = NormalOp{M.Uop, M.kappa, M.a, M.nrm, true, nullptr, M.ho, M.he};
+        M.Al = NormalOp{M.Ui, M.kappa, M.a, M.nrm, true, nullptr, M.lo, M.le};
+        M.Am = NormalOp{M.Um, M.kappa, M.a, M.nrm, true, nullptr, M.mo, M.me};
+        M.b2 = host_norm2(b[i]);
+        if (!std::isfinite(M.b2)) {
+            qkx::set_error("qkx_solve_multi: non-finite source");
+            return QKX_ERR_NONFINITE;
+        }
+        // bh_e = b_e + kappa H_eo A^-1 b_o; normal-equation source bn = M^dag bh
+        if ((rc = qkx_spinor_copy(M.bh, b[i])) || (rc = qkx_spinor_copy(M.bo, b[i]))) return rc;
+        if ((rc = qkx::site(M.to, M.bo, nullptr, QKX_BASIS_UKQCD, -1, 1.0 / M.nrm, -M.a / M.nrm, 0, 0, 0))) return rc;
+        if ((rc = qkx::hop(M.bh, M.to, M.bh, M.Uop, QKX_BASIS_UKQCD, 0, 0, M.kappa, 0, 1.0, 0, 0))) return rc;
+        if (mixed) {
+            if ((rc = qkx_spinor_copy(M.sm, M.bh)) || (rc = M.Am.apply_dag(M.rm, M.sm)) ||
+                (rc = qkx_spinor_copy(M.bn, M.rm)))
+                return rc;
+        } else if ((rc = M.Ah.apply_dag(M.bn, M.bh))) {
+            return rc;
+        }
+        if ((rc = qkx_spinor_zero(M.xs)) || (M.xl && (rc = qkx_spinor_zero(M.xl)))) return rc;
+        if ((rc = qkx_spinor_copy(M.r, M.bn))) return rc;
+        M.stop2 = tol2 * host_norm2(M.bn);
+        if (!(M.b2 > 0)) M.state = 1;   // zero source: x = 0
+    }
+    // slots: parity 0 from scratch, every instance restarted into parity 1
+    std::vector<CGSlot> h(n, CGSlot{0, 0, 0, 0, 0, 0});
+    if ((rc = write_mslots(0, n, h.data()))) return rc;
+    for (int i = 0; i < n; i++)
+        if ((rc = restart_inst(I[i], i, 0, 0))) return rc;
+    int j = 1;
+    if ((rc = read_mslots(j, n, h.data()))) return rc;
+    for (int i = 0; i < n; i++)
+        if (I[i].state) h[i].flag = kDone;
+    if ((rc = write_mslots(j, n, h.data()))) return rc;
+    const double delta2 = mixed ? o->delta * o->delta : 0.0;
+    auto running = [&]() {
+        for (int i = 0; i < n; i++)
+            if (!I[i].state && h[i].k < o->maxIter) return true;
+        return false;
+    };
+    while (running()) {
+        int kmax = 0;
+        for (int i = 0; i < n; i++)
+            if (!I[i].state) kmax = std::max(kmax, h[i].k);
+        const int batch = std::max(1, std::min(qkx::cg_batch_size, o->maxIter - kmax));
+        for (int it = 0; it < batch; it++, j++)
+            if ((rc = enqueue_multi(lp, n, j, I.data(), delta2))) return rc;
+        if ((rc = read_mslots(j, n, h.data()))) return rc;
+        std::vector<int> restart_mode(n, -1);
+        bool touched = false;
+        for (int i = 0; i < n; i++) {
+            MInst &M = I[i];
+            if (M.state || h[i].flag == 0 || h[i].flag == kDone) continue;
+            touched = true;
+            if (h[i].flag == 3) {
+                M.state = 2;
+                h[i].flag = kDone;
+                continue;
+            }
+            // reliable update or iterated stop: the true residual in FP64
+            if (mixed) {
+                if ((rc = launch_accum(M.xs, M.xl)) || (rc = qkx_spinor_zero(M.xl))) return rc;
+            }
+            if ((rc = M.Ah.apply_op(M.s, M.xs))) return rc;
+            if ((rc = qkx::axpby_dev(M.s, M.bh, -1.0, 1.0, nullptr, 0))) return rc;
+            const double s2 = host_norm2(M.s);
+            if (!std::isfinite(s2)) {
+                M.state = 2;
+                h[i].flag = kDone;
+                continue;
+            }
+            if (s2 <= tol2 * M.b2) {
+                M.state = 1;
+                h[i].flag = kDone;
+                continue;
+            }
+            if (h[i].flag == 2) M.stop2 = std::min(M.stop2, h[i].r2) * std::max(1e-4, 0.5 * tol2 * M.b2 / s2);
+            if (mixed) {
+                M.nrel++;
+                if ((rc = qkx_spinor_copy(M.sm, M.s))) return rc;
+                if (M.r->prec == QKX_PREC_SINGLE) {
+                    if ((rc = M.Am.apply_dag(M.r, M.sm))) return rc;
+                } else if ((rc = M.Am.apply_dag(M.rm, M.sm)) || (rc = qkx_spinor_copy(M.r, M.rm))) {
+                    return rc;
+                }
+            }
+            restart_mode[i] = 1;
+        }
+        if (!touched) continue;
+        // next parity: the host's view of every instance (flags cleared for
+        // the restarted ones, done for the finished ones), then the restarts
+        // overwrite their own slots
+        std::vector<CGSlot> nx(h);
+        for (int i = 0; i < n; i++)
+            if (restart_mode[i] >= 0) nx[i].flag = 0;
+        if ((rc = write_mslots(j + 1, n, nx.data()))) return rc;
+        for (int i = 0; i < n; i++)
+            if (restart_mode[i] >= 0 && (rc = restart_inst(I[i], i, j, restart_mode[i]))) return rc;
+        j++;
+        if ((rc = read_mslots(j, n, h.data()))) return rc;
+    }
+    int status = QKX_SUCCESS;
+    for (int i = 0; i < n; i++) {
+        MInst &M = I[i];
+        if (M.state == 0 && mixed) {   // maxIter: keep the increments accumulated so far
+            if ((rc = launch_accum(M.xs, M.xl)) || (rc = qkx_spinor_zero(M.xl))) return rc;
+        }
+        // x_o = A^-1 (b_o + kappa H_oe x_e)
+        if ((rc = qkx::hop(M.to, M.xs, M.bo, M.Uop, QKX_BASIS_UKQCD, 0, 1, M.kappa / M.nrm, -M.a * M.kappa / M.nrm,
+                           1.0 / M.nrm, -M.a / M.nrm, 0)))
+            return rc;
+        if ((rc = qkx_spinor_copy(x[i], M.xs)) || (rc = qkx_spinor_copy(x[i], M.to))) return rc;
+        double tr = -1;
+        if ((rc = qkx_true_residual(&tr, x[i], b[i], U[i], &p[i]))) return rc;
+        const int code = M.state == 1 ? QKX_SUCCESS : M.state == 2 ? QKX_ERR_BREAKDOWN : QKX_NOT_CONVERGED;
+        if (code == QKX_ERR_BREAKDOWN) status = QKX_ERR_BREAKDOWN;
+        else if (code == QKX_NOT_CONVERGED && status == QKX_SUCCESS) status = QKX_NOT_CONVERGED;
+        if (st) {
+            st[i].iterations = h[i].k;
+            st[i].reliable_updates = M.nrel;
+            st[i].converged = M.state == 1;
+            st[i].iterated_r2 = h[i].r2;
+            st[i].true_residual = tr;
+            st[i].fallback_iterations = 0;
+        }
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st)
+        for (int i = 0; i < n; i++) st[i].seconds = secs;
+    if (status == QKX_ERR_BREAKDOWN) qkx::set_error("qkx_solve_multi: breakdown (p.Ap <= 0 or non-finite residual)");
+    else if (status == QKX_NOT_CONVERGED) qkx::set_error("qkx_solve_multi: maxIter reached");
     return status;
 }
 

@@ -61,10 +61,10 @@ def test_batch_operators_match_oracle(q, small, nrhs):
 
 @pytest.mark.parametrize("prec", ["SINGLE", "HALF", "BF16"])
 def test_batch_normal_op_equals_single_fields(q, small, prec):
-    """A batch with one right-hand side per thread gives bit-identical results
-    to single fields (each right-hand side sees the same arithmetic); the
-    RHS-tiled kernels (2, 3 right-hand sides per thread) agree to the
-    storage precision."""
+    """A batch with one right-hand side per thread and per-thread links gives
+    bit-identical results to single fields (each right-hand side sees the
+    same arithmetic); the RHS-tiled kernels (2, 3 right-hand sides per
+    thread) and the shared-link kernels agree to the storage precision."""
     U, _ = small
     P = getattr(q, prec)
     u = q.Gauge(LS, boundaryT=-1, host=U)
@@ -83,6 +83,7 @@ def test_batch_normal_op_equals_single_fields(q, small, prec):
         single.append(os_.download())
     tol = {"SINGLE": 1e-6, "HALF": 3e-3, "BF16": 2e-2}[prec]
     try:
+        q.set_batch_link_sharing(0)
         for rt in (1, 2, 3):
             q.set_rhs_tile(rt)
             out.tm_xxDagTm_xx(b, ul, p, q.EE)
@@ -92,8 +93,15 @@ def test_batch_normal_op_equals_single_fields(q, small, prec):
                     assert np.array_equal(got[r], single[r]), f"{prec} rhs {r}"
                 else:
                     assert rel(got[r], single[r]) <= tol, f"{prec} tile {rt} rhs {r}"
+        q.set_rhs_tile(0)
+        q.set_batch_link_sharing(2)
+        out.tm_xxDagTm_xx(b, ul, p, q.EE)
+        got = out.download()
+        for r in range(nrhs):
+            assert rel(got[r], single[r]) <= tol, f"{prec} shared links rhs {r}"
     finally:
         q.set_rhs_tile(0)
+        q.set_batch_link_sharing(1)
 
 
 def test_batch_rejects_mismatched_fields(q, small):
