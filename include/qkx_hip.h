@@ -223,6 +223,10 @@ int qkx_spinor_download_rhs(const qkx_spinor *s, int rhs, double *psi_aos);
 /* Vector::zero :443-447, Vector::copy :449-453 (copy also converts precision
  * and moves a parity between full and subset fields). */
 int qkx_spinor_zero(qkx_spinor *s);
+/* Gaussian noise, variance 1 per real component, generated on the device
+ * from (seed, entry, component) by a counter-based hash: the learned
+ * preconditioner's random training vectors.  No reference counterpart. */
+int qkx_spinor_gaussian(qkx_spinor *s, unsigned long long seed);
 int qkx_spinor_copy(qkx_spinor *dst, const qkx_spinor *src);
 
 /* ---- Dirac operators (Vector methods, include/lattice_util.h:102-137) ------ */
@@ -376,6 +380,13 @@ int qkx_net_set_bf16(qkx_net *n, int on);
 /* out = M in; single full-lattice FP64 or FP32 spinors */
 int qkx_net_apply(qkx_net *n, qkx_spinor *out, const qkx_spinor *in,
                   const qkx_gauge *U);
+/* out = N_e in (adjoint: N_e^dag in), N_e = P_e M P_e the network on the even
+ * sites: in and out even-parity fields (single or batched, any precision,
+ * the same for both; out may be in).  With the input zero on the odd sites,
+ * the hidden channels are needed at the odd sites only, and C0 = W2[0] W1[0]
+ * carries the rest. */
+int qkx_net_apply_eo(qkx_net *n, qkx_spinor *out, const qkx_spinor *in,
+                     const qkx_gauge *U, int adjoint);
 /* Adds the gradient of |M r - e|^2 with respect to the weights
  * (dL/dRe W + i dL/dIm W) and the loss to the network's accumulators. */
 int qkx_net_accumulate_grad(qkx_net *n, const qkx_spinor *r,
@@ -403,6 +414,25 @@ typedef struct {
 int qkx_solve_gcr(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
                   const qkx_gauge *U_net, qkx_net *net, const qkx_params *p,
                   const qkx_gcr_opts *o, qkx_solver_stats *st);
+
+/* The learned, mixed-precision, multi-RHS solver.  Solves D_r x_r = b_r for
+ * every right-hand side r of a batch (as qkx_solve_batch: x, b full FP64
+ * batches, one mu) on the even-odd Schur system Mh, with the network as
+ * right preconditioner: the inner CG (low precision, o->inner_prec) runs on
+ * (Mh N)^dag (Mh N) y = (Mh N)^dag bh with N = P_e M P_e, the network
+ * restricted to the even sites (its hidden channels live on the odd sites),
+ * and x = N y is accumulated in FP64 at every reliable update (o->delta).  The
+ * FP64 true residual judges: when a reliable-update cycle cuts the worst true
+ * residual of the batch by less than `stall` (0 < stall <= 1), the solve goes
+ * on as the plain CG from its current solution (st->fallback_iterations
+ * counts those iterations).  net = NULL is qkx_solve_batch.  o->eo must be 1
+ * and o->inner_prec FP32, FP16 or BF16; U_net is the network's FP32 gauge
+ * field.  No reference counterpart: it replaces the reference's loop of
+ * single CG solves over a propagator's 12 sources
+ * (bin/test_invert_prop.cpp:83-97, lib/solvers.cpp:16-81). */
+int qkx_solve_learned(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U,
+                      const qkx_gauge *U_net, qkx_net *net, const qkx_params *p,
+                      const qkx_solve_opts *o, double stall, qkx_solver_stats *st);
 
 /* No reference counterpart: the solvers' global reduction, exposed so that
  * its compensation can be checked (the reference sums with MPI_Allreduce,

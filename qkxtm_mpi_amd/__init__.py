@@ -110,6 +110,10 @@ SIGNATURES = {
     "qkx_net_create": [ctypes.POINTER(_P), _I], "qkx_net_destroy": [_P], "qkx_net_nweights": [_P, _IP],
     "qkx_net_set_weights": [_P, _P], "qkx_net_get_weights": [_P, _P], "qkx_net_set_bf16": [_P, _I],
     "qkx_net_apply": [_P, _P, _P, _P], "qkx_net_accumulate_grad": [_P, _P, _P, _P],
+    "qkx_net_apply_eo": [_P, _P, _P, _P, _I],
+    "qkx_solve_learned": [_P, _P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolveOpts), _D,
+                          ctypes.POINTER(SolverStats)],
+    "qkx_spinor_gaussian": [_P, ctypes.c_ulonglong],
     "qkx_net_read_grad": [_P, _P, _DP], "qkx_net_zero_grad": [_P],
     "qkx_net_adam": [_P, _D, _D, _D, _D, _I],
     "qkx_solve_gcr": [_P, _P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(GcrOpts),
@@ -335,6 +339,10 @@ class Spinor:
 
     def copy(self, src: "Spinor"):
         _check(lib().qkx_spinor_copy(self.h, src.h), "qkx_spinor_copy")
+
+    def gaussian(self, seed):
+        """Device-generated Gaussian noise (variance 1 per real component)."""
+        _check(lib().qkx_spinor_gaussian(self.h, int(seed)), "qkx_spinor_gaussian")
 
     def norm2(self):
         v = ctypes.c_double()
@@ -598,6 +606,10 @@ class Net:
     def apply(self, out: Spinor, inp: Spinor, u_net: Gauge):
         _check(lib().qkx_net_apply(self.h, out.h, inp.h, u_net.h), "qkx_net_apply")
 
+    def apply_eo(self, out: Spinor, inp: Spinor, u_net: Gauge, adjoint=False):
+        """out = N_e inp (or N_e^dag inp): the network on even-parity fields."""
+        _check(lib().qkx_net_apply_eo(self.h, out.h, inp.h, u_net.h, int(adjoint)), "qkx_net_apply_eo")
+
     def accumulate_grad(self, r: Spinor, e: Spinor, u_net: Gauge):
         _check(lib().qkx_net_accumulate_grad(self.h, r.h, e.h, u_net.h), "qkx_net_accumulate_grad")
 
@@ -636,6 +648,22 @@ def solve_gcr(x: Spinor, b: Spinor, u: Gauge, p: Params, net: "Net | None" = Non
     if not (rc == NOT_CONVERGED and allow_not_converged):
         _check(rc, "qkx_solve_gcr")
     return st
+
+
+def solve_learned(x: Spinor, b: Spinor, u: Gauge, p: Params, net: "Net | None", u_net: Gauge | None,
+                  inner_prec=SINGLE, delta=0.1, tol=1e-10, maxIter=100000, stall=0.5, allow_not_converged=False):
+    """The learned, mixed-precision, multi-RHS solver: even-odd CG on
+    (Mh N)^dag (Mh N) y = (Mh N)^dag bh, x = N y, for every right-hand side of
+    the batch, falling back to the plain CG when a reliable-update cycle cuts
+    the worst true residual by less than ``stall``; net=None is solve_batch.
+    Returns a list of SolverStats."""
+    o = SolveOpts(1, inner_prec, RECON_12, delta, 50, int(maxIter), tol)
+    st = (SolverStats * b.nrhs)()
+    rc = lib().qkx_solve_learned(x.h, b.h, u.h, u_net.h if u_net else None, net.h if net else None,
+                                 ctypes.byref(p), ctypes.byref(o), float(stall), st)
+    if not (rc == NOT_CONVERGED and allow_not_converged):
+        _check(rc, "qkx_solve_learned")
+    return list(st)
 
 
 def true_residual(x: Spinor, b: Spinor, u: Gauge, p: Params):

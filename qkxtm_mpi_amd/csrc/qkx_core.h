@@ -6,6 +6,7 @@
 
 struct qkx_spinor_s;
 struct qkx_gauge_s;
+struct qkx_net_s;
 
 namespace qkx {
 
@@ -68,6 +69,13 @@ bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const 
                             const qkx_spinor_s *r);
 int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
                  const double coef[][4], int g5out, const CgBatchFuse &f, int *nblocks_out);
+// the learned network on even-parity batched fields (qkx_learned.hip):
+// net_prep_eo builds the forward and adjoint weight tables from the current
+// weights; net_apply_eo: out = N_e in (adjoint: N_e^dag in), out may be in;
+// both launches return at once while *skip != 0 (skip optional)
+int net_prep_eo(qkx_net_s *n);
+int net_apply_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, int adjoint,
+                 const int *skip = nullptr);
 void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);

@@ -81,6 +81,38 @@ __global__ void spinor_zero_range(S dst, int n, int shift)
     dst.store(i + shift, v);
 }
 
+// Gaussian noise (variance 1 per real component) from a counter-based hash of
+// (seed, entry, component): splitmix64 finalisers (Steele, Lea and Flood,
+// OOPSLA 2014) give two uniforms, the Box-Muller transform the pair of
+// normals.  The same seed gives the same field on any grid.
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+template <class S>
+__global__ void spinor_gaussian(S dst, int n, unsigned long long seed)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using R = typename S::Real;
+    Cx<R> v[12];
+    const uint64_t base = mix64(seed);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const uint64_t h1 = mix64(base ^ (((uint64_t)i << 4) | (uint64_t)k));
+        const uint64_t h2 = mix64(h1);
+        const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);   // (0, 1)
+        const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+        const double rad = sqrt(-2.0 * log(u1)), ph = 6.283185307179586 * u2;
+        v[k] = {(R)(rad * cos(ph)), (R)(rad * sin(ph))};
+    }
+    dst.store(i, v);
+}
+
 // ---- gauge ------------------------------------------------------------------
 // Store one link U[9] (complex) at (mu, s) in the requested format.
 template <int PREC, int RECON>
@@ -515,6 +547,18 @@ int qkx_spinor_zero(qkx_spinor *f)
     int n = entries(f);
     return dispatch_spinor(f, [&](auto S) {
         hipLaunchKernelGGL((spinor_zero<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream, S, n);
+        QKX_HIP(hipGetLastError());
+        return QKX_SUCCESS;
+    });
+}
+
+int qkx_spinor_gaussian(qkx_spinor *f, unsigned long long seed)
+{
+    QKX_CHECK_ARG(f, "qkx_spinor_gaussian: null field");
+    int n = entries(f);
+    return dispatch_spinor(f, [&](auto S) {
+        hipLaunchKernelGGL((spinor_gaussian<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream, S, n,
+                           seed);
         QKX_HIP(hipGetLastError());
         return QKX_SUCCESS;
     });

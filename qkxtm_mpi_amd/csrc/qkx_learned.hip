@@ -36,6 +36,9 @@ struct qkx_net_s {
     float2 *fin, *fh, *fout, *gout, *gh;   // work buffers: 4ch, ch, 4ch, 4ch, ch channels
     double *part;
     int part_cap;
+    float2 *weo;           // even-odd weight tables (forward, then adjoint), see eo_weights_kernel
+    float2 *ehid;          // even-odd hidden channels on the odd sites, [3 ch][Vh][nrhs]
+    size_t ehid_cap;       // float2 entries of ehid
 };
 
 namespace {
@@ -394,6 +397,188 @@ template <class S> __global__ __launch_bounds__(kT) void feat_to_spinor(const fl
     f.store(s, v);
 }
 
+// ---------------------------------------------------------------------------
+// The network on even-parity fields, N_e = P_e L2 L1 P_e (P_e: restriction to
+// the even sites), batched over right-hand sides (RHS innermost).  With the
+// input zero on the odd sites, L1's hidden channels are W1[0] in(x) at an even
+// site x and sum_{p>0} W1[p] (T_p in)(y) at an odd site y (all neighbours of y
+// are even), and L2's output at x is
+//   C0 in(x) + sum_{p>0} W2[p] (T_p h)(x),   C0 = W2[0] W1[0]  (4 x 4),
+// whose hops read only odd-site hidden channels.  So N_e is two launches:
+// eo_hidden_kernel (odd sites) and eo_out_kernel (even sites).  The adjoint
+// N_e^dag = P_e L1^dag L2^dag P_e has the same form with the adjoint layers'
+// weights, A[p][a][b] = conj(W[rev p][b][a]).
+//
+// Weight table of one direction (forward or adjoint), complex:
+//   [8][CH][4] first-layer hops (p = 1..8), [8][4][CH] second-layer hops,
+//   [4][4] C0.
+constexpr int eo_table(int ch) { return 8 * ch * 4 * 2 + 16; }
+
+__device__ __forceinline__ float2 net_w(const float2 *w, const uint32_t *wb, int bf, int i)
+{
+    if (!bf) return w[i];
+    const uint32_t h = wb[i];
+    return make_float2(bf2f(h & 0xffffu), bf2f(h >> 16));
+}
+
+// Both tables from the FP32 master weights or their BF16 copies: thread i
+// writes entry i of [forward table][adjoint table].
+__global__ void eo_weights_kernel(const float2 *w, const uint32_t *wb, int bf, int ch, float2 *out)
+{
+    const int T = eo_table(ch), i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * T) return;
+    const int adj = i / T, j = i % T, n1 = 9 * ch * 4;
+    auto W1 = [&](int p, int a, int b) { return net_w(w, wb, bf, (p * ch + a) * 4 + b); };    // [9][ch][4]
+    auto W2 = [&](int p, int a, int b) { return net_w(w, wb, bf, n1 + (p * 4 + a) * ch + b); };   // [9][4][ch]
+    auto cj = [](float2 v) { return make_float2(v.x, -v.y); };
+    const int h1 = 8 * ch * 4;
+    float2 v;
+    if (j < h1) {                       // first layer, p = 1..8: [a < ch][b < 4]
+        const int p = j / (ch * 4) + 1, a = (j / 4) % ch, b = j % 4;
+        v = adj ? cj(W2(rev_path(p), b, a)) : W1(p, a, b);
+    } else if (j < 2 * h1) {            // second layer: [a < 4][b < ch]
+        const int k = j - h1, p = k / (4 * ch) + 1, a = (k / ch) % 4, b = k % ch;
+        v = adj ? cj(W1(rev_path(p), b, a)) : W2(p, a, b);
+    } else {                            // C0 = W2[0] W1[0], or its adjoint
+        const int k = j - 2 * h1, a = adj ? k % 4 : k / 4, b = adj ? k / 4 : k % 4;
+        float re = 0.f, im = 0.f;
+        for (int c = 0; c < ch; c++) {
+            const float2 x = W2(0, a, c), y = W1(0, c, b);
+            re += x.x * y.x - x.y * y.y;
+            im += x.x * y.y + x.y * y.x;
+        }
+        v = adj ? make_float2(re, -im) : make_float2(re, im);
+    }
+    out[i] = v;
+}
+
+// Hidden channels at the odd sites: h_a(y) = sum_{p>0} sum_b w[p][a][b] (T_p in_b)(y).
+// One thread per (odd site, RHS); in is an even-parity field.
+template <int CH, class S>
+__global__ __launch_bounds__(kT) void eo_hidden_kernel(S in, GaugeF18 U, Geom g, const float2 *__restrict__ wsrc,
+                                                       float2 *__restrict__ hid, int nr, const int *skip)
+{
+    if (skip && *skip) return;   // uniform over the grid
+    __shared__ float2 w[8 * CH * 4];
+    for (int i = threadIdx.x; i < 8 * CH * 4; i += blockDim.x) w[i] = wsrc[i];
+    __syncthreads();
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= g.Vh * nr) return;
+    const int cb = u / nr, r = u - cb * nr;
+    const int s = g.Vh + cb;
+    int x[4];
+    coords_of(g, s, x);
+    Cx<float> acc[CH][3];
+#pragma unroll
+    for (int a = 0; a < CH; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[a][c] = {0.f, 0.f};
+    for (int p = 1; p < 9; p++) {
+        const Hop h = path_hop(g, x, s, p);   // h.sn: an even site, = its checkerboard index
+        Cx<float> uu[9];
+        load_link(U, h, p, uu);
+        Cx<typename S::Real> v[12];
+        in.load(h.sn * nr + r, v);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const Cx<float> vb[3] = {{(float)v[b * 3].re, (float)v[b * 3].im},
+                                     {(float)v[b * 3 + 1].re, (float)v[b * 3 + 1].im},
+                                     {(float)v[b * 3 + 2].re, (float)v[b * 3 + 2].im}};
+            Cx<float> t[3];
+            matvec(uu, vb, t);
+#pragma unroll
+            for (int a = 0; a < CH; a++) {
+                const float2 wv = w[((p - 1) * CH + a) * 4 + b];
+                const Cx<float> wc{wv.x, wv.y};
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[a][c] = acc[a][c] + cmul(wc, t[c]);
+            }
+        }
+    }
+    const size_t stride = (size_t)g.Vh * nr;
+#pragma unroll
+    for (int a = 0; a < CH; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) hid[(size_t)(a * 3 + c) * stride + u] = make_float2(acc[a][c].re, acc[a][c].im);
+}
+
+// Output at the even sites: out(x) = C0 in(x) + sum_{p>0} (T_p m_p)(x) with
+// m_p = w[p] h at the odd neighbour (the channels are mixed first, then the
+// four colour vectors transported).  One thread per (even site, RHS).
+template <int CH, class S>
+__global__ __launch_bounds__(kT) void eo_out_kernel(S in, S out, GaugeF18 U, Geom g, const float2 *__restrict__ wsrc,
+                                                    const float2 *__restrict__ hid, int nr, const int *skip)
+{
+    if (skip && *skip) return;   // uniform over the grid
+    __shared__ float2 w[8 * 4 * CH + 16];
+    for (int i = threadIdx.x; i < 8 * 4 * CH + 16; i += blockDim.x) w[i] = wsrc[i];
+    __syncthreads();
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= g.Vh * nr) return;
+    const int cb = u / nr, r = u - cb * nr;
+    int x[4];
+    coords_of(g, cb, x);
+    Cx<float> acc[4][3];
+    {
+        Cx<typename S::Real> v[12];
+        in.load(u, v);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                Cx<float> z{0.f, 0.f};
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const float2 wv = w[8 * 4 * CH + a * 4 + b];
+                    z = z + cmul(Cx<float>{wv.x, wv.y}, Cx<float>{(float)v[b * 3 + c].re, (float)v[b * 3 + c].im});
+                }
+                acc[a][c] = z;
+            }
+    }
+    const size_t stride = (size_t)g.Vh * nr;
+    for (int p = 1; p < 9; p++) {
+        const Hop h = path_hop(g, x, cb, p);   // h.sn: an odd site
+        const size_t hu = (size_t)(h.sn - g.Vh) * nr + r;
+        Cx<float> m[4][3];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) m[a][c] = {0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            Cx<float> hv[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float2 t = hid[(size_t)(k * 3 + c) * stride + hu];
+                hv[c] = {t.x, t.y};
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const float2 wv = w[((p - 1) * 4 + a) * CH + k];
+                const Cx<float> wc{wv.x, wv.y};
+#pragma unroll
+                for (int c = 0; c < 3; c++) m[a][c] = m[a][c] + cmul(wc, hv[c]);
+            }
+        }
+        Cx<float> uu[9];
+        load_link(U, h, p, uu);
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            Cx<float> t[3];
+            matvec(uu, m[a], t);
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[a][c] = acc[a][c] + t[c];
+        }
+    }
+    using R = typename S::Real;
+    Cx<R> o[12];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[a * 3 + c] = {(R)acc[a][c].re, (R)acc[a][c].im};
+    out.store(u, o);
+}
+
 int nblk(int n) { return (n + kT - 1) / kT; }
 
 int net_buffers(qkx_net_s *n, const int L[4])
@@ -516,6 +701,22 @@ int from_feat(const float2 *in, qkx_spinor_s *f)
     });
 }
 
+template <int CH, class S>
+int launch_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, int adjoint,
+              const int *skip)
+{
+    const Geom g = qkx::make_geom(U->L, U->bc);
+    const int nr = in->nrhs, units = g.Vh * nr;
+    const GaugeF18 G = make_gauge<GaugeF18>(U);
+    const float2 *tab = n->weo + (adjoint ? eo_table(CH) : 0);
+    hipLaunchKernelGGL((eo_hidden_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream, make_spinor<S>(in), G,
+                       g, tab, n->ehid, nr, skip);
+    hipLaunchKernelGGL((eo_out_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream, make_spinor<S>(in),
+                       make_spinor<S>(out), G, g, tab + 8 * CH * 4, (const float2 *)n->ehid, nr, skip);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
 } // namespace
 
 namespace qkx {
@@ -528,6 +729,49 @@ int net_apply(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const qkx
     if ((rc = to_feat(in, n->fin))) return rc;
     if ((rc = net_forward(n, U))) return rc;
     return from_feat(n->fout, out);
+}
+// The even-odd weight tables from the current weights (BF16 copies when
+// the network runs in BF16); a solve prepares them once.
+int net_prep_eo(qkx_net_s *n)
+{
+    const int T = eo_table(n->ch);
+    if (!n->weo) QKX_HIP(hipMalloc(&n->weo, sizeof(float2) * 2 * T));
+    hipLaunchKernelGGL(eo_weights_kernel, dim3((2 * T + 255) / 256), dim3(256), 0, qkx::g_stream, n->w, n->wb,
+                       n->bf16, n->ch, n->weo);
+    QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
+// out = N_e in (adjoint: N_e^dag in) on even-parity fields of one precision
+// and batch size; net_prep_eo must have run since the weights last changed.
+// out may be in.  skip (optional, device): both launches return at once
+// while *skip != 0.
+int net_apply_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, int adjoint,
+                 const int *skip)
+{
+    int rc;
+    if ((rc = check_net_gauge(U))) return rc;
+    if (in->subset != QKX_SUBSET_EVEN || out->subset != QKX_SUBSET_EVEN || in->prec != out->prec ||
+        in->nrhs != out->nrhs || in->L[0] != U->L[0] || in->L[1] != U->L[1] || in->L[2] != U->L[2] ||
+        in->L[3] != U->L[3] || out->L[0] != U->L[0] || out->L[1] != U->L[1] || out->L[2] != U->L[2] ||
+        out->L[3] != U->L[3]) {
+        qkx::set_error("net (even-odd): in and out must be even-parity fields of one precision and batch size on "
+                       "the gauge's lattice");
+        return QKX_ERR_ARG;
+    }
+    const size_t need = (size_t)n->ch * 3 * (size_t)in->Vh * in->nrhs;
+    if (need > n->ehid_cap) {
+        if (n->ehid) (void)hipFree(n->ehid);
+        n->ehid = nullptr;
+        n->ehid_cap = 0;
+        QKX_HIP(hipMalloc(&n->ehid, sizeof(float2) * need));
+        n->ehid_cap = need;
+    }
+    return dispatch_prec(in->prec, [&](auto tag) {
+        using S = decltype(tag);
+        return n->ch == 16 ? launch_eo<16, S>(n, out, in, U, adjoint, skip)
+                           : launch_eo<8, S>(n, out, in, U, adjoint, skip);
+    });
 }
 } // namespace qkx
 
@@ -568,7 +812,8 @@ int qkx_net_create(qkx_net **out, int ch)
 int qkx_net_destroy(qkx_net *n)
 {
     if (!n) return QKX_SUCCESS;
-    void *ptrs[] = {n->w, n->wb, n->grad, n->m, n->v, n->loss, n->fin, n->fh, n->fout, n->gout, n->gh, n->part};
+    void *ptrs[] = {n->w,    n->wb, n->grad, n->m,  n->v,    n->loss, n->fin,
+                    n->fh,   n->fout, n->gout, n->gh, n->part, n->weo, n->ehid};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
@@ -617,6 +862,14 @@ int qkx_net_apply(qkx_net *n, qkx_spinor *out, const qkx_spinor *in, const qkx_g
     QKX_CHECK_ARG(n && out && in && U, "qkx_net_apply: null argument");
     QKX_CHECK_ARG(out != in, "qkx_net_apply: out must differ from in");
     return qkx::net_apply(n, out, in, U);
+}
+
+int qkx_net_apply_eo(qkx_net *n, qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, int adjoint)
+{
+    QKX_CHECK_ARG(n && out && in && U, "qkx_net_apply_eo: null argument");
+    int rc = qkx::net_prep_eo(n);
+    if (rc) return rc;
+    return qkx::net_apply_eo(n, out, in, U, adjoint ? 1 : 0, nullptr);
 }
 
 int qkx_net_accumulate_grad(qkx_net *n, const qkx_spinor *r, const qkx_spinor *e, const qkx_gauge *U)

@@ -498,6 +498,22 @@ int enqueue_biter_fused(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spino
     return QKX_SUCCESS;
 }
 
+// the update and direction kernels of one iteration, for the inner precisions
+int enqueue_bblas_any(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, qkx_spinor_s *Ap, int npap)
+{
+    const int px = x->prec, pr = r->prec, pp = p->prec;
+    if (px == QKX_PREC_DOUBLE && pr == QKX_PREC_DOUBLE && pp == QKX_PREC_DOUBLE)
+        return enqueue_bblas<SpinorD, SpinorD, SpinorD>(E, x, r, p, Ap, npap);
+    if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_SINGLE)
+        return enqueue_bblas<SpinorF, SpinorF, SpinorF>(E, x, r, p, Ap, npap);
+    if (px == QKX_PREC_SINGLE && pr == QKX_PREC_HALF && pp == QKX_PREC_HALF)
+        return enqueue_bblas<SpinorF, SpinorH, SpinorH>(E, x, r, p, Ap, npap);
+    if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_BF16)
+        return enqueue_bblas<SpinorF, SpinorF, SpinorB>(E, x, r, p, Ap, npap);
+    qkx::set_error("batched cg: unsupported precision combination");
+    return QKX_ERR_ARG;
+}
+
 int enqueue_biter(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
                   qkx_spinor_s *Ap)
 {
@@ -508,19 +524,41 @@ int enqueue_biter(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spinor_s *r
     int rc = A.apply(Ap, p, g_bpart, &npap);
     qkx::set_hop_skip(nullptr);
     if (rc) return rc;
-    const int px = x->prec, pr = r->prec, pp = p->prec;
-    if (px == QKX_PREC_DOUBLE && pr == QKX_PREC_DOUBLE && pp == QKX_PREC_DOUBLE)
-        rc = enqueue_bblas<SpinorD, SpinorD, SpinorD>(E, x, r, p, Ap, npap);
-    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_SINGLE)
-        rc = enqueue_bblas<SpinorF, SpinorF, SpinorF>(E, x, r, p, Ap, npap);
-    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_HALF && pp == QKX_PREC_HALF)
-        rc = enqueue_bblas<SpinorF, SpinorH, SpinorH>(E, x, r, p, Ap, npap);
-    else if (px == QKX_PREC_SINGLE && pr == QKX_PREC_SINGLE && pp == QKX_PREC_BF16)
-        rc = enqueue_bblas<SpinorF, SpinorF, SpinorB>(E, x, r, p, Ap, npap);
-    else {
-        qkx::set_error("batched cg: unsupported precision combination");
-        return QKX_ERR_ARG;
+    rc = enqueue_bblas_any(E, x, r, p, Ap, npap);
+    E.j++;
+    return rc;
+}
+
+// The learned right preconditioner of the even-odd system: the inner CG
+// runs on (Mh N)^dag (Mh N) y = (Mh N)^dag r with N = N_e the network on the
+// even sites, and x = N y.  While `on`, every inner operator, the iterate's
+// accumulation and the residual of each reliable update go through N; once
+// switched off (the fallback) the solver is the plain CG.
+struct NetPre {
+    qkx_net_s *net = nullptr;
+    const qkx_gauge_s *Un = nullptr;   // the network's FP32 links
+    bool on = false;
+    qkx_spinor_s *t = nullptr, *w = nullptr;   // inner-precision even temporaries
+    qkx_spinor_s *t32 = nullptr;               // FP32 even temporary: N applied to the iterate
+};
+
+// One iteration on N^dag Mh^dag Mh N: p.Ap = |Mh N p|^2 from the partials of
+// the first Schur application.  The network launches follow the hops' skip
+// flag, so iterations enqueued past an event cost only their dispatches.
+int enqueue_biter_net(BEngine &E, const BatchOp &A, NetPre &np, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
+                      qkx_spinor_s *Ap)
+{
+    const int *skip = &E.cur()->flag;
+    qkx::set_hop_skip(skip);
+    int npap = 0, rc;
+    if ((rc = qkx::net_apply_eo(np.net, np.t, p, np.Un, 0, skip)) ||
+        (rc = A.schur(A.tmp_even, np.t, +1, 1, g_bpart, &npap)) || (rc = A.schur(np.w, A.tmp_even, -1, 1)) ||
+        (rc = qkx::net_apply_eo(np.net, Ap, np.w, np.Un, 1, skip))) {
+        qkx::set_hop_skip(nullptr);
+        return rc;
     }
+    qkx::set_hop_skip(nullptr);
+    rc = enqueue_bblas_any(E, x, r, p, Ap, npap);
     E.j++;
     return rc;
 }
@@ -568,8 +606,13 @@ int sum_rows_dev(const double *part, int nblocks, int nr, double *dst)
 
 extern "C" {
 
-int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_gauge *U_inner_arg,
-                    const qkx_params *p, const double *mu, const qkx_solve_opts *o, qkx_solver_stats *st)
+// qkx_solve_batch, and with np the learned-preconditioned solver
+// (qkx_solve_learned): np->on selects the preconditioned operator; stall is
+// the true-residual reduction a reliable-update cycle must reach for the
+// network to be kept (otherwise the solve goes on as the plain CG).
+static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_gauge *U_inner_arg,
+                            const qkx_params *p, const double *mu, const qkx_solve_opts *o, qkx_solver_stats *st,
+                            NetPre *np, double stall)
 {
     QKX_CHECK_ARG(x && b && U && p && o, "qkx_solve_batch: null argument");
     QKX_CHECK_ARG(x->prec == QKX_PREC_DOUBLE && b->prec == QKX_PREC_DOUBLE && x->subset == QKX_SUBSET_FULL &&
@@ -667,18 +710,29 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     const int S32 = QKX_PREC_SINGLE;
     qkx_spinor_s *mo = T(S32, QKX_SUBSET_ODD, mixed && eo), *me = T(S32, QKX_SUBSET_EVEN, mixed && eo),
                  *sm = T(S32, sub, mixed), *rm = T(S32, sub, mixed);
+    if (np) {
+        np->t = T(lp, QKX_SUBSET_EVEN);
+        np->w = T(lp, QKX_SUBSET_EVEN);
+        np->t32 = T(S32, QKX_SUBSET_EVEN);
+    }
     if (nomem) return QKX_ERR_NOMEM;
     Ah.tmp_full = hf; Ah.tmp_odd = ho; Ah.tmp_even = he;
     Al.tmp_full = lf; Al.tmp_odd = lo; Al.tmp_even = le;
     Am.tmp_full = nullptr; Am.tmp_odd = mo; Am.tmp_even = me;   // Op^dag needs no full temporary
     // r_out = Op^dag s in FP32 (mixed) or FP64, s an FP64 field; r_out any precision
-    auto dag_residual = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
+    auto dag_residual0 = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
         if (!mixed) return Ah.apply_dag(r_out, src64);
         int e = qkx_spinor_copy(sm, src64);
         if (e) return e;
         if (r_out->prec == QKX_PREC_SINGLE) return Am.apply_dag(r_out, sm);
         if ((e = Am.apply_dag(rm, sm))) return e;
         return qkx_spinor_copy(r_out, rm);
+    };
+    // with the network: r_out = N^dag Op^dag s
+    auto dag_residual = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
+        int e = dag_residual0(r_out, src64);
+        if (e || !np || !np->on) return e;
+        return qkx::net_apply_eo(np->net, r_out, r_out, np->Un, 1);
     };
 
     std::vector<double> b2, bn2, s2;
@@ -700,7 +754,10 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     } else {
         if ((rc = qkx_spinor_copy(bh, b))) return rc;
     }
-    if ((rc = dag_residual(bn, bh))) return rc;
+    // normal-equation source Op^dag bh; with the network N^dag Op^dag bh
+    std::vector<double> bn2_plain;
+    if ((rc = dag_residual0(bn, bh)) || (rc = norms(bn, bn2_plain))) return rc;
+    if (np && np->on && (rc = qkx::net_apply_eo(np->net, bn, bn, np->Un, 1))) return rc;
     if ((rc = qkx_spinor_zero(xs)) || (xl && (rc = qkx_spinor_zero(xl)))) return rc;
     if ((rc = qkx_spinor_copy(r, bn))) return rc;
     if ((rc = norms(bn, bn2))) return rc;
@@ -718,8 +775,9 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     if ((rc = read_bslot(E, h))) return rc;
 
     int status = QKX_NOT_CONVERGED;
-    int nrel = 0;
-    std::vector<double> true_res(nr, -1.0);
+    int nrel = 0, k_fallback = -1, weak_cycles = 0;
+    std::vector<double> true_res(nr, -1.0), prev_s2(nr, 0.0);
+    if ((rc = norms(bh, prev_s2))) return rc;   // the first cycle's reduction is against |bh|
     qkx_spinor_s *xiter = mixed ? xl : xs;
     auto all_done = [&]() {
         for (int q = 0; q < nr; q++)
@@ -729,8 +787,11 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
     if (all_done()) status = QKX_SUCCESS;
     while (status != QKX_SUCCESS && h.k < o->maxIter) {
         int batch = std::min(qkx::cg_batch_size, o->maxIter - h.k);
-        for (int i = 0; i < batch; i++)
-            if ((rc = enqueue_biter(E, Al, xiter, r, pp, Ap))) return rc;
+        for (int i = 0; i < batch; i++) {
+            rc = (np && np->on) ? enqueue_biter_net(E, Al, *np, xiter, r, pp, Ap)
+                                : enqueue_biter(E, Al, xiter, r, pp, Ap);
+            if (rc) return rc;
+        }
         if ((rc = read_bslot(E, h))) return rc;
         if (h.flag == 0) continue;
         if (h.flag == 3) {
@@ -740,7 +801,12 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         }
         if (mixed) {
             const int n = entries(xs);
-            rc = with_spinor(xl, [&](auto S) {
+            qkx_spinor_s *xacc = xl;
+            if (np && np->on) {   // x = N y
+                if ((rc = qkx::net_apply_eo(np->net, np->t32, xl, np->Un, 0))) return rc;
+                xacc = np->t32;
+            }
+            rc = with_spinor(xacc, [&](auto S) {
                 hipLaunchKernelGGL((accumb_kernel<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
                                    make_spinor<SpinorD>(xs), S, n);
                 QKX_HIP(hipGetLastError());
@@ -761,13 +827,35 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
         }
         if (nonfinite) { status = QKX_ERR_NONFINITE; qkx::set_error("qkx_solve_batch: non-finite residual"); break; }
         if (all_done()) { status = QKX_SUCCESS; break; }
+        // The network stays while the preconditioned cycles make progress:
+        // the plain CG takes over from the current solution (fresh
+        // directions, the plain normal-equation stopping threshold) when a
+        // cycle that ended on its reliable-update trigger cut the worst active
+        // true residual by less than `stall`, when a cycle left it unchanged,
+        // or after three weak cycles in a row.  (A cycle that ended on the
+        // iterated stopping test near convergence may cut little and is
+        // not judged alone: its threshold is tightened instead.)
+        bool switched = false;
+        if (np && np->on) {
+            double worst = 0.0;
+            for (int q = 0; q < nr; q++)
+                if (h.active[q] && prev_s2[q] > 0) worst = std::max(worst, std::sqrt(s2[q] / prev_s2[q]));
+            weak_cycles = worst > stall ? weak_cycles + 1 : 0;
+            if ((h.flag == 1 && worst > stall) || worst > 0.99 || weak_cycles >= 3) {
+                np->on = false;
+                switched = true;
+                k_fallback = h.k;
+                for (int q = 0; q < nr; q++) h.stop2[q] = tol2 * bn2_plain[q];
+            }
+        }
+        prev_s2 = s2;
         h.flag = 0;
         if ((rc = write_bslot(E, h))) return rc;
         if (mixed) {
             nrel++;
             if ((rc = dag_residual(r, s))) return rc;
         }
-        if ((rc = brestart(E, pp, r, 0))) return rc;
+        if ((rc = brestart(E, pp, r, switched ? 1 : 0))) return rc;
         if ((rc = read_bslot(E, h))) return rc;
     }
     if (status == QKX_NOT_CONVERGED) qkx::set_error("qkx_solve_batch: maxIter reached");
@@ -808,8 +896,35 @@ int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, cons
             st[q].iterated_r2 = h.r2[q];
             st[q].true_residual = true_res[q];
             st[q].seconds = secs;
+            st[q].fallback_iterations = k_fallback >= 0 ? h.k - k_fallback : 0;
         }
     return status;
+}
+
+int qkx_solve_batch(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_gauge *U_inner_arg,
+                    const qkx_params *p, const double *mu, const qkx_solve_opts *o, qkx_solver_stats *st)
+{
+    return solve_batch_impl(x, b, U, U_inner_arg, p, mu, o, st, nullptr, 0.0);
+}
+
+int qkx_solve_learned(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_gauge *U_net, qkx_net *net,
+                      const qkx_params *p, const qkx_solve_opts *o, double stall, qkx_solver_stats *st)
+{
+    QKX_CHECK_ARG(x && b && U && p && o, "qkx_solve_learned: null argument");
+    QKX_CHECK_ARG(o->eo == 1 && o->inner_prec != QKX_PREC_DOUBLE,
+                  "qkx_solve_learned: even-odd with a low-precision inner solver required");
+    QKX_CHECK_ARG(stall > 0 && stall <= 1, "qkx_solve_learned: stall must be in (0, 1]");
+    if (!net) return solve_batch_impl(x, b, U, nullptr, p, nullptr, o, st, nullptr, 0.0);
+    QKX_CHECK_ARG(U_net && U_net->prec == QKX_PREC_SINGLE && U_net->recon == QKX_RECON_18 &&
+                      U_net->L[0] == b->L[0] && U_net->L[1] == b->L[1] && U_net->L[2] == b->L[2] &&
+                      U_net->L[3] == b->L[3],
+                  "qkx_solve_learned: U_net must be FP32 recon 18 on the lattice of b");
+    if (int rc = qkx::net_prep_eo(net)) return rc;
+    NetPre np;
+    np.net = net;
+    np.Un = U_net;
+    np.on = true;
+    return solve_batch_impl(x, b, U, nullptr, p, nullptr, o, st, &np, stall);
 }
 
 } // extern "C"
