@@ -18,6 +18,11 @@ inner operator (4 hops each) after 10 warm-ups, median of 5 trials, against
 its compulsory traffic; `traffic` is the measured HBM traffic per launch from
 the hardware counters (FETCH_SIZE, WRITE_SIZE, calibrated on a copy kernel).
 
+The `learned` object reports the learned, mixed-precision, multi-RHS solver
+(qkx_solve_learned) on the same inputs: its network trained at 16^3x32 on
+random pairs (untimed, seconds reported), fine-tuned here on 128 pairs, then
+the headline source and a batch of 12 Z2 sources solved with and without it.
+
 Prints ONE JSON line.
 """
 import argparse
@@ -47,6 +52,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-solver section")
     ap.add_argument("--workload", default="single",
                     choices=["single", "learned", "propagator", "production", "smear", "batch"],
                     help="single: the 24^3x48 single-RHS solve (default, the headline line); "
@@ -66,6 +72,8 @@ def parse():
     ap.add_argument("--batch-kernels-only", action="store_true",
                     help="only apply the FP16 even-odd normal operator to a batch of 12 right-hand sides at "
                          "24^3x48, three times (hardware-counter runs of the batched hop)")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
+                    help="hop autotuner: 0 fixed launch shapes, 1 batched launches (default), 2 every launch")
     ap.add_argument("--batch-share", type=int, default=1, choices=[0, 1, 2],
                     help="links shared per workgroup in batched hops: 0 never, 1 automatic (default), 2 always")
     return ap.parse_args()
@@ -245,6 +253,111 @@ def run_learned(q, args):
     print(json.dumps(out))
 
 
+def train_transfer(q, log_to=sys.stderr):
+    """The learned preconditioner's network: 8 hidden channels trained at
+    16^3x32, kappa = 0.1625 (the learned workload's lattice), on 1024 random
+    pairs from 4 gauge fields, 4 epochs.  Returns (net, seconds, losses)."""
+    from qkxtm_mpi_amd import learned
+    Ll, kap, mu = (16, 16, 16, 32), 0.1625, 0.004
+    gauges = []
+    for gs in (101, 102, 103, 104):
+        u = q.Gauge(Ll, boundaryT=-1, host=q.gen_gauge(Ll, 0.3, gs))
+        u32 = q.Gauge(Ll, q.SINGLE, q.RECON_18, -1)
+        u32.convert_from(u)
+        gauges.append(u32)
+    net = q.Net(8, seed=1)
+    q.sync()
+    t0 = time.perf_counter()
+    hist = learned.train_random(net, gauges, kap, mu, 1024, epochs=4, batch=16, lr=2e-3, seed=3,
+                                log=lambda m: print(m, file=log_to, flush=True))
+    q.sync()
+    return net, time.perf_counter() - t0, hist
+
+
+def fine_tune(q, net, u32, kappa, mu, n_pairs, log_to=sys.stderr):
+    """n_pairs random pairs on the target lattice's own links, one epoch."""
+    from qkxtm_mpi_amd import learned
+    q.sync()
+    t0 = time.perf_counter()
+    before = learned.random_loss(net, [u32], kappa, mu, 16)
+    learned.train_random(net, [u32], kappa, mu, n_pairs, epochs=1, batch=16, lr=5e-4, seed=9,
+                         log=lambda m: print(m, file=log_to, flush=True))
+    after = learned.random_loss(net, [u32], kappa, mu, 16)
+    q.sync()
+    return {"pairs": n_pairs, "seconds": time.perf_counter() - t0, "loss_before": before, "loss_after": after}
+
+
+def learned_section(q, u, p, b, xref, nref, gold, results):
+    """The learned, mixed-precision, multi-RHS solver at 24^3x48 (config-2
+    inputs): the network trained at 16^3x32 and fine-tuned here on 128 random
+    pairs, as right preconditioner of the FP16-inner even-odd CG, on the
+    headline source (a batch of one) and on a batch of 12 Z2 sources, each
+    beside the same solver without the network.  Its rows join
+    time_to_solution; the network's own launch cost is timed too."""
+    net, t_train, hist = train_transfer(q)
+    u32 = q.Gauge(L, q.SINGLE, q.RECON_18, -1)
+    u32.convert_from(u)
+    tune = fine_tune(q, net, u32, KAPPA, MU, 128)
+    delta = DELTA["fp16"]
+    b1 = q.Spinor(L, nrhs=1)
+    b1.copy(b)
+    x1 = q.Spinor(L, nrhs=1)
+    rows = {}
+    for name, nn in (("learned_fp16_eo", net), ("learned_fp16_eo_no_net", None)):
+        fn = lambda: q.solve_learned(x1, b1, u, p, nn, u32 if nn else None, inner_prec=q.HALF, delta=delta,   # noqa: E731
+                                     tol=TOL, maxIter=20000, stall=0.5)
+        x1.zero()
+        fn()
+        times = []
+        for _ in range(3):
+            x1.zero()
+            q.sync()
+            t = time.perf_counter()
+            st = fn()[0]
+            q.sync()
+            times.append(time.perf_counter() - t)
+        xs = x1.download()
+        rows[name] = {"time_s": float(np.median(times)), "iterations": st.iterations,
+                      "fallback_iterations": st.fallback_iterations, "reliable_updates": st.reliable_updates,
+                      "true_residual": st.true_residual, "delta": delta,
+                      "err_vs_fp64": float(np.linalg.norm(xs - xref) / nref), "err_vs_reference": gold(xs)}
+        results[name] = rows[name]
+    # a batch of 12 Z2 sources
+    nb = 12
+    bb = q.Spinor(L, host=np.stack([q.gen_z2(L, 500 + r) for r in range(nb)]), nrhs=nb)
+    xb = q.Spinor(L, nrhs=nb)
+    batch = {}
+    for name, nn in (("with_network", net), ("without_network", None)):
+        fn = lambda: q.solve_learned(xb, bb, u, p, nn, u32 if nn else None, inner_prec=q.HALF, delta=delta,   # noqa: E731
+                                     tol=TOL, maxIter=20000, stall=0.5)
+        t, sts = timed(q, fn, 2)
+        batch[name] = {"time_s": t, "time_per_rhs_s": t / nb, "iterations": sts[0].iterations,
+                       "fallback_iterations": sts[0].fallback_iterations,
+                       "max_true_residual": max(s.true_residual for s in sts)}
+    # the network's launch cost (both kernels of N_e), FP16 fields
+    e1, f1 = q.Spinor(L, q.HALF, q.EVEN), q.Spinor(L, q.HALF, q.EVEN)
+    e1.copy(b)
+    us1 = loop_time(q, lambda: net.apply_eo(f1, e1, u32), 1, calls=20, trials=3, warm=3)
+    eb, fb = q.Spinor(L, q.HALF, q.EVEN, nrhs=nb), q.Spinor(L, q.HALF, q.EVEN, nrhs=nb)
+    eb.copy(bb)
+    usb = loop_time(q, lambda: net.apply_eo(fb, eb, u32), 1, calls=10, trials=3, warm=2)
+    fp64 = results["fp64_cg_full"]["time_s"]
+    return {
+        "solver": "qkx_solve_learned: FP16-inner even-odd CG on (Mh N)^dag (Mh N), N the PTC network on the even "
+                  "sites (8 hidden channels, BF16 weights, FP32 accumulation), FP64 reliable updates, plain-CG "
+                  "fallback on stall",
+        "training": {"lattice": [16, 16, 16, 32], "kappa": 0.1625, "pairs": 1024, "epochs": 4,
+                     "cost": "|M D e - e|^2 over random unit vectors e", "seconds": t_train, "loss_history": hist},
+        "fine_tune_24x24x24x48": tune,
+        "single_rhs": rows,
+        "batch12": batch,
+        "speedup_vs_fp64_cg_single": fp64 / rows["learned_fp16_eo"]["time_s"],
+        "speedup_vs_fp64_cg_batch12_per_rhs": fp64 / batch["with_network"]["time_per_rhs_s"],
+        "network_launch_us": {"single_rhs": us1, "batch12": usb,
+                              "note": "one N_e application (two launches), HIP events, FP16 even fields"},
+    }
+
+
 def point_batch(q, L, x0, first, n):
     """Spin-colour point sources first..first+n-1 at site x0, as one batch."""
     V = int(np.prod(L))
@@ -272,7 +385,7 @@ def timed(q, fn, reps):
     return float(np.median(ts)), st
 
 
-def run_propagator(q, args, L, kap, mu, eps, mu_batches):
+def run_propagator(q, args, L, kap, mu, eps, mu_batches, learned=False):
     """Twelve spin-colour point sources solved as batches of `chunk`
     right-hand sides (even-odd, low-precision inner, FP64 reliable updates),
     against one FP64 CG solve and one single-RHS mixed solve scaled by 12."""
@@ -326,6 +439,30 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
     us_rescale = loop_time(q, lambda: Pa.rescale(1.0), 1, calls=20, trials=3, warm=3)
     us_rotate = loop_time(q, lambda: Pa.rotateToPhysicalBasePlus(), 1, calls=20, trials=3, warm=3)
     del Pa
+    # the batched solution is not needed beyond this point: free it before the
+    # learned solver allocates its own 12-column fields
+    x_last = None
+    x = None
+    learned_row = None
+    if learned:
+        # the learned solver on the same 12 sources: network trained at
+        # 16^3x32, transferred, fine-tuned on 256 random pairs at this volume
+        net, t_train, hist = train_transfer(q)
+        u32 = q.Gauge(L, q.SINGLE, q.RECON_18, -1)
+        u32.convert_from(u)
+        tune = fine_tune(q, net, u32, kap, mu, 256)
+        net.release_buffers()     # the training features at this volume (7 GB)
+        bl = point_batch(q, L, x0, 0, 12)
+        xl = q.Spinor(L, nrhs=12)
+        t_l, stl = timed(q, lambda: q.solve_learned(xl, bl, u, p, net, u32, inner_prec=P, delta=delta, tol=TOL,
+                                                     maxIter=50000, stall=0.5), reps)
+        learned_row = {"time_s": t_l, "iterations": stl[0].iterations,
+                       "fallback_iterations": stl[0].fallback_iterations,
+                       "max_true_residual": max(s.true_residual for s in stl),
+                       "training_16x16x16x32": {"seconds": t_train, "loss_history": hist},
+                       "fine_tune": tune,
+                       "device_memory_GB": q.mem_info()[0] / 1e9}
+        del xl, bl, u32, net
     mu_rows = []
     for Lm in mu_batches:
         Um = q.gen_gauge(Lm, 0.3, 3)
@@ -390,6 +527,8 @@ def run_propagator(q, args, L, kap, mu, eps, mu_batches):
                    f"{args.inner}_eo_hop_us": uslo, f"{args.inner}_eo_hop_GBs": bylo / nlo / (uslo * 1e-6) / 1e9},
         "multi_mu": mu_rows,
         "uneven_instances": uneven,
+        "learned_batch12": learned_row,
+        "autotuned_kinds": q.autotune_report().splitlines(),
         "assembly": {"rescale_us": us_rescale, "rotate_us": us_rotate,
                      "rescale_GBs": bya / (us_rescale * 1e-6) / 1e9, "rotate_GBs": bya / (us_rotate * 1e-6) / 1e9,
                      "note": "Vector::rescale(2 kappa) and rotateToPhysicalBasePlus on the 12-column propagator"},
@@ -604,6 +743,7 @@ def main():
         sys.exit(2)
     import qkxtm_mpi_amd as q
     q.init(0)
+    q.set_autotune(args.autotune)
     if args.workload == "learned":
         run_learned(q, args)
         return
@@ -611,7 +751,7 @@ def main():
         run_propagator(q, args, (32, 32, 32, 64), 0.1632, 0.004, 0.3, [(16, 16, 16, 32), (24, 24, 24, 48)])
         return
     if args.workload == "production":
-        run_propagator(q, args, (48, 48, 48, 96), 0.1632, 0.004, 0.25, [])
+        run_propagator(q, args, (48, 48, 48, 96), 0.1632, 0.004, 0.25, [], learned=not args.no_learned)
         return
     if args.workload == "smear":
         run_smear(q, args)
@@ -745,6 +885,8 @@ def main():
                 if line.startswith("hbm_bytes_per_launch:"):
                     traffic = float(line.split(":", 1)[1])
 
+    learned_out = None if args.no_learned else learned_section(q, u, p, b, xref, nref, gold, results)
+
     cpu = None if args.no_cpu else cpu_baseline(U, src, results["fp64_cg_full"]["iterations"])
 
     out = {
@@ -779,6 +921,7 @@ def main():
         "speedup_vs_fp64_cg": results["fp64_cg_full"]["time_s"] / med,
         "dslash_fp64_tm": {"avg_launch_us": us64, "GB/s": gbs64, "frac": gbs64 / HBM_PEAK_GBS,
                            "bytes_per_site": by64 / n64 / V},
+        "learned": learned_out,
         "timed_total_s": total,
     }
     print(json.dumps(out))

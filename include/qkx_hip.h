@@ -373,6 +373,14 @@ typedef struct qkx_net_s qkx_net;
 int qkx_net_create(qkx_net **n, int ch);                  /* ch = 8 or 16 */
 int qkx_net_destroy(qkx_net *n);
 int qkx_net_nweights(const qkx_net *n, int *ncomplex);
+/* Frees the network's work buffers (training features on its last lattice,
+ * the even-odd hidden channels); the next call that needs them allocates
+ * them again.  Weights and optimiser state stay. */
+int qkx_net_release_buffers(qkx_net *n);
+/* Bound on the even-odd hidden-channel buffer (bytes; 0: 4 GiB): batches
+ * whose hidden channels do not fit are processed a few right-hand sides at a
+ * time (at least one), with the same results. */
+int qkx_net_set_hidden_limit(qkx_net *n, unsigned long long bytes);
 int qkx_net_set_weights(qkx_net *n, const float *w);      /* resets the optimiser */
 int qkx_net_get_weights(const qkx_net *n, float *w);
 /* forward passes read BF16 weights (1, default; FP32 accumulation) or FP32 (0) */

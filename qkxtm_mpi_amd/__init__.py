@@ -110,7 +110,8 @@ SIGNATURES = {
     "qkx_net_create": [ctypes.POINTER(_P), _I], "qkx_net_destroy": [_P], "qkx_net_nweights": [_P, _IP],
     "qkx_net_set_weights": [_P, _P], "qkx_net_get_weights": [_P, _P], "qkx_net_set_bf16": [_P, _I],
     "qkx_net_apply": [_P, _P, _P, _P], "qkx_net_accumulate_grad": [_P, _P, _P, _P],
-    "qkx_net_apply_eo": [_P, _P, _P, _P, _I],
+    "qkx_net_apply_eo": [_P, _P, _P, _P, _I], "qkx_net_release_buffers": [_P],
+    "qkx_net_set_hidden_limit": [_P, ctypes.c_ulonglong],
     "qkx_solve_learned": [_P, _P, _P, _P, _P, ctypes.POINTER(Params), ctypes.POINTER(SolveOpts), _D,
                           ctypes.POINTER(SolverStats)],
     "qkx_spinor_gaussian": [_P, ctypes.c_ulonglong],
@@ -605,6 +606,14 @@ class Net:
 
     def apply(self, out: Spinor, inp: Spinor, u_net: Gauge):
         _check(lib().qkx_net_apply(self.h, out.h, inp.h, u_net.h), "qkx_net_apply")
+
+    def release_buffers(self):
+        """Free the work buffers (training features, even-odd hidden channels)."""
+        _check(lib().qkx_net_release_buffers(self.h), "qkx_net_release_buffers")
+
+    def set_hidden_limit(self, nbytes):
+        """Bound the even-odd hidden buffer (0: the 4 GiB default)."""
+        _check(lib().qkx_net_set_hidden_limit(self.h, int(nbytes)), "qkx_net_set_hidden_limit")
 
     def apply_eo(self, out: Spinor, inp: Spinor, u_net: Gauge, adjoint=False):
         """out = N_e inp (or N_e^dag inp): the network on even-parity fields."""

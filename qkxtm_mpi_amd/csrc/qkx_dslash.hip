@@ -1359,8 +1359,9 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
 // Time the candidate shapes of one launch kind on scratch fields of that
 // kind (same precision, parity subset and right-hand-side count; links as
 // given) and record the fastest in g_tuned.  Nothing is recorded when the
-// stream is being captured into a graph, when instrumentation is on, or when
-// the scratch fields cannot be had; the launch then keeps its default shape.
+// stream is being captured into a graph or when instrumentation is on (the
+// launch keeps its default shape and a later launch tunes); when the scratch
+// fields cannot be had, the default shape is recorded.
 template <class S, class G, int B>
 void tune_hop(const TuneKey &key, const qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *aux,
               const qkx_gauge_s *U, int dagger, int par_out, const Epi *e, const HopShape &dflt, bool shareable)
@@ -1415,7 +1416,10 @@ void tune_hop(const TuneKey &key, const qkx_spinor_s *out, const qkx_spinor_s *i
     if (si) qkx::pool_release(si);
     if (sa) qkx::pool_release(sa);
     (void)hipGetLastError();
+    // a kind that could not be timed (no memory for its scratch fields) keeps
+    // the default shape for good rather than retrying at every launch
     if (ok && best < 1e300) g_tuned[key] = TuneEntry{pick, best, dflt_us};
+    else g_tuned[key] = TuneEntry{dflt, -1.0, -1.0};
 }
 
 template <class S, class G, int B, class Fin>

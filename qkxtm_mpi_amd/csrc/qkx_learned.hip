@@ -17,6 +17,7 @@
 // copy is FP32, inference reads a BF16 copy (FP32 accumulation).
 #include "qkx_internal.h"
 #include "qkx_core.h"
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <type_traits>
@@ -39,6 +40,7 @@ struct qkx_net_s {
     float2 *weo;           // even-odd weight tables (forward, then adjoint), see eo_weights_kernel
     float2 *ehid;          // even-odd hidden channels on the odd sites, [3 ch][Vh][nrhs]
     size_t ehid_cap;       // float2 entries of ehid
+    size_t ehid_limit;     // bound on ehid (float2 entries); 0: kEoHiddenCap
 };
 
 namespace {
@@ -456,15 +458,17 @@ __global__ void eo_weights_kernel(const float2 *w, const uint32_t *wb, int bf, i
 // One thread per (odd site, RHS); in is an even-parity field.
 template <int CH, class S>
 __global__ __launch_bounds__(kT) void eo_hidden_kernel(S in, GaugeF18 U, Geom g, const float2 *__restrict__ wsrc,
-                                                       float2 *__restrict__ hid, int nr, const int *skip)
+                                                       float2 *__restrict__ hid, int nr, int r0, int nc,
+                                                       const int *skip)
 {
     if (skip && *skip) return;   // uniform over the grid
     __shared__ float2 w[8 * CH * 4];
     for (int i = threadIdx.x; i < 8 * CH * 4; i += blockDim.x) w[i] = wsrc[i];
     __syncthreads();
+    // units of this launch: (odd site, rhs r0 .. r0 + nc - 1) of a batch of nr
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= g.Vh * nr) return;
-    const int cb = u / nr, r = u - cb * nr;
+    if (u >= g.Vh * nc) return;
+    const int cb = u / nc, r = r0 + (u - cb * nc);
     const int s = g.Vh + cb;
     int x[4];
     coords_of(g, s, x);
@@ -495,7 +499,7 @@ __global__ __launch_bounds__(kT) void eo_hidden_kernel(S in, GaugeF18 U, Geom g,
             }
         }
     }
-    const size_t stride = (size_t)g.Vh * nr;
+    const size_t stride = (size_t)g.Vh * nc;
 #pragma unroll
     for (int a = 0; a < CH; a++)
 #pragma unroll
@@ -507,21 +511,23 @@ __global__ __launch_bounds__(kT) void eo_hidden_kernel(S in, GaugeF18 U, Geom g,
 // four colour vectors transported).  One thread per (even site, RHS).
 template <int CH, class S>
 __global__ __launch_bounds__(kT) void eo_out_kernel(S in, S out, GaugeF18 U, Geom g, const float2 *__restrict__ wsrc,
-                                                    const float2 *__restrict__ hid, int nr, const int *skip)
+                                                    const float2 *__restrict__ hid, int nr, int r0, int nc,
+                                                    const int *skip)
 {
     if (skip && *skip) return;   // uniform over the grid
     __shared__ float2 w[8 * 4 * CH + 16];
     for (int i = threadIdx.x; i < 8 * 4 * CH + 16; i += blockDim.x) w[i] = wsrc[i];
     __syncthreads();
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= g.Vh * nr) return;
-    const int cb = u / nr, r = u - cb * nr;
+    if (u >= g.Vh * nc) return;
+    const int cb = u / nc, rc = u - cb * nc;
+    const int e = cb * nr + r0 + rc;   // the unit's entry in the batch fields
     int x[4];
     coords_of(g, cb, x);
     Cx<float> acc[4][3];
     {
         Cx<typename S::Real> v[12];
-        in.load(u, v);
+        in.load(e, v);
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -535,10 +541,10 @@ __global__ __launch_bounds__(kT) void eo_out_kernel(S in, S out, GaugeF18 U, Geo
                 acc[a][c] = z;
             }
     }
-    const size_t stride = (size_t)g.Vh * nr;
+    const size_t stride = (size_t)g.Vh * nc;
     for (int p = 1; p < 9; p++) {
         const Hop h = path_hop(g, x, cb, p);   // h.sn: an odd site
-        const size_t hu = (size_t)(h.sn - g.Vh) * nr + r;
+        const size_t hu = (size_t)(h.sn - g.Vh) * nc + rc;
         Cx<float> m[4][3];
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -576,7 +582,7 @@ __global__ __launch_bounds__(kT) void eo_out_kernel(S in, S out, GaugeF18 U, Geo
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int c = 0; c < 3; c++) o[a * 3 + c] = {(R)acc[a][c].re, (R)acc[a][c].im};
-    out.store(u, o);
+    out.store(e, o);
 }
 
 int nblk(int n) { return (n + kT - 1) / kT; }
@@ -701,18 +707,35 @@ int from_feat(const float2 *in, qkx_spinor_s *f)
     });
 }
 
+// right-hand sides per launch of the even-odd network: all of them while
+// their hidden channels fit the bound (default kEoHiddenCap float2 entries,
+// 4 GiB), else fewer
+constexpr size_t kEoHiddenCap = size_t(1) << 29;
+int eo_chunk(const qkx_net_s *n, const qkx_spinor_s *in)
+{
+    const size_t per_rhs = (size_t)n->ch * 3 * (size_t)in->Vh;
+    const size_t cap = n->ehid_limit ? n->ehid_limit : kEoHiddenCap;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)in->nrhs, cap / per_rhs));
+}
+
 template <int CH, class S>
 int launch_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_gauge_s *U, int adjoint,
               const int *skip)
 {
     const Geom g = qkx::make_geom(U->L, U->bc);
-    const int nr = in->nrhs, units = g.Vh * nr;
+    const int nr = in->nrhs, nc = eo_chunk(n, in);
     const GaugeF18 G = make_gauge<GaugeF18>(U);
     const float2 *tab = n->weo + (adjoint ? eo_table(CH) : 0);
-    hipLaunchKernelGGL((eo_hidden_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream, make_spinor<S>(in), G,
-                       g, tab, n->ehid, nr, skip);
-    hipLaunchKernelGGL((eo_out_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream, make_spinor<S>(in),
-                       make_spinor<S>(out), G, g, tab + 8 * CH * 4, (const float2 *)n->ehid, nr, skip);
+    // the right-hand sides in chunks of nc, so that the hidden channels of a
+    // chunk fit the buffer (out may be in: each chunk reads only its own
+    // columns before writing them)
+    for (int r0 = 0; r0 < nr; r0 += nc) {
+        const int c = std::min(nc, nr - r0), units = g.Vh * c;
+        hipLaunchKernelGGL((eo_hidden_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream,
+                           make_spinor<S>(in), G, g, tab, n->ehid, nr, r0, c, skip);
+        hipLaunchKernelGGL((eo_out_kernel<CH, S>), dim3(nblk(units)), dim3(kT), 0, qkx::g_stream, make_spinor<S>(in),
+                           make_spinor<S>(out), G, g, tab + 8 * CH * 4, (const float2 *)n->ehid, nr, r0, c, skip);
+    }
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -759,7 +782,7 @@ int net_apply_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const 
                        "the gauge's lattice");
         return QKX_ERR_ARG;
     }
-    const size_t need = (size_t)n->ch * 3 * (size_t)in->Vh * in->nrhs;
+    const size_t need = (size_t)n->ch * 3 * (size_t)in->Vh * eo_chunk(n, in);
     if (need > n->ehid_cap) {
         if (n->ehid) (void)hipFree(n->ehid);
         n->ehid = nullptr;
@@ -817,6 +840,27 @@ int qkx_net_destroy(qkx_net *n)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
+    return QKX_SUCCESS;
+}
+
+int qkx_net_release_buffers(qkx_net *n)
+{
+    QKX_CHECK_ARG(n, "qkx_net_release_buffers: null argument");
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));   // no launch may still use them
+    float2 **bufs[6] = {&n->fin, &n->fh, &n->fout, &n->gout, &n->gh, &n->ehid};
+    for (auto b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    n->ehid_cap = 0;
+    n->L[0] = n->L[1] = n->L[2] = n->L[3] = 0;
+    return QKX_SUCCESS;
+}
+
+int qkx_net_set_hidden_limit(qkx_net *n, unsigned long long bytes)
+{
+    QKX_CHECK_ARG(n, "qkx_net_set_hidden_limit: null argument");
+    n->ehid_limit = (size_t)(bytes / sizeof(float2));
     return QKX_SUCCESS;
 }
 

@@ -77,6 +77,32 @@ def test_net_eo_matches_oracle(q, ch, prec, tol, adjoint):
     assert rel(x1.download(), got[1]) <= tol
 
 
+def test_net_eo_rhs_chunks_identical(q):
+    """A hidden-buffer bound that forces the batch through in chunks of one
+    and of two right-hand sides gives the same bits as one launch pair."""
+    L, nrhs = (4, 6, 4, 8), 5
+    U = q.gen_gauge(L, 0.4, 45)
+    u32 = q.Gauge(L, q.SINGLE, q.RECON_18, -1)
+    u32.convert_from(q.Gauge(L, boundaryT=-1, host=U))
+    W1, W2 = rand_weights(8, 88)
+    net = q.Net(8, W1, W2)
+    srcs = np.stack([q.gen_gaussian(L, 70 + r) for r in range(nrhs)])
+    per_rhs = 8 * 3 * (np.prod(L) // 2) * 8       # bytes of one right-hand side's hidden channels
+    outs = []
+    for limit in (0, per_rhs, 2 * per_rhs):
+        net.set_hidden_limit(limit)
+        for adj in (False, True):
+            x = q.Spinor(L, q.HALF, q.EVEN, host=srcs, nrhs=nrhs)
+            y = q.Spinor(L, q.HALF, q.EVEN, nrhs=nrhs)
+            net.apply_eo(y, x, u32, adj)
+            net.apply_eo(x, x, u32, adj)          # in place, chunk by chunk
+            outs.append((y.download(), x.download()))
+    net.set_hidden_limit(0)
+    for k in range(2, len(outs)):
+        ref = outs[k % 2]
+        assert np.array_equal(outs[k][0], ref[0]) and np.array_equal(outs[k][1], ref[0])
+
+
 def test_net_eo_bf16_weights(q):
     """Inference with the BF16 copies of the weights: the oracle with the
     weights rounded to BF16."""
