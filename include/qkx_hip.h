@@ -370,7 +370,7 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
  * the host (2 * qkx_net_nweights floats).  The network's gauge field is FP32
  * RECON_18 (its boundaryT as for the operators). */
 typedef struct qkx_net_s qkx_net;
-int qkx_net_create(qkx_net **n, int ch);                  /* ch = 8 or 16 */
+int qkx_net_create(qkx_net **n, int ch);                  /* ch = 4, 8 or 16 */
 int qkx_net_destroy(qkx_net *n);
 int qkx_net_nweights(const qkx_net *n, int *ncomplex);
 /* Frees the network's work buffers (training features on its last lattice,

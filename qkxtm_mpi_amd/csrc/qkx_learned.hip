@@ -644,6 +644,10 @@ int net_forward(qkx_net_s *n, const qkx_gauge_s *U)
         if ((rc = launch_ptc<4, 16, false>(n->fin, n->fh, U, w1, bf))) return rc;
         return launch_ptc<16, 4, false>(n->fh, n->fout, U, w2, bf);
     }
+    if (n->ch == 4) {
+        if ((rc = launch_ptc<4, 4, false>(n->fin, n->fh, U, w1, bf))) return rc;
+        return launch_ptc<4, 4, false>(n->fh, n->fout, U, w2, bf);
+    }
     if ((rc = launch_ptc<4, 8, false>(n->fin, n->fh, U, w1, bf))) return rc;
     return launch_ptc<8, 4, false>(n->fh, n->fout, U, w2, bf);
 }
@@ -792,8 +796,9 @@ int net_apply_eo(qkx_net_s *n, qkx_spinor_s *out, const qkx_spinor_s *in, const 
     }
     return dispatch_prec(in->prec, [&](auto tag) {
         using S = decltype(tag);
-        return n->ch == 16 ? launch_eo<16, S>(n, out, in, U, adjoint, skip)
-                           : launch_eo<8, S>(n, out, in, U, adjoint, skip);
+        if (n->ch == 16) return launch_eo<16, S>(n, out, in, U, adjoint, skip);
+        if (n->ch == 4) return launch_eo<4, S>(n, out, in, U, adjoint, skip);
+        return launch_eo<8, S>(n, out, in, U, adjoint, skip);
     });
 }
 } // namespace qkx
@@ -802,7 +807,7 @@ extern "C" {
 
 int qkx_net_create(qkx_net **out, int ch)
 {
-    QKX_CHECK_ARG(out && (ch == 8 || ch == 16), "qkx_net_create: hidden channels must be 8 or 16");
+    QKX_CHECK_ARG(out && (ch == 4 || ch == 8 || ch == 16), "qkx_net_create: hidden channels must be 4, 8 or 16");
     if (int rc = qkx::ensure_init()) return rc;
     qkx_net_s *n = new qkx_net_s{};
     n->ch = ch;
@@ -938,6 +943,11 @@ int qkx_net_accumulate_grad(qkx_net *n, const qkx_spinor *r, const qkx_spinor *e
         if ((rc = launch_wgrad<16, 4>(n->gout, n->fh, U, n, g2))) return rc;
         if ((rc = launch_ptc<4, 16, true>(n->gout, n->gh, U, w2, bf))) return rc;
         return launch_wgrad<4, 16>(n->gh, n->fin, U, n, g1);
+    }
+    if (n->ch == 4) {
+        if ((rc = launch_wgrad<4, 4>(n->gout, n->fh, U, n, g2))) return rc;
+        if ((rc = launch_ptc<4, 4, true>(n->gout, n->gh, U, w2, bf))) return rc;
+        return launch_wgrad<4, 4>(n->gh, n->fin, U, n, g1);
     }
     if ((rc = launch_wgrad<8, 4>(n->gout, n->fh, U, n, g2))) return rc;
     if ((rc = launch_ptc<4, 8, true>(n->gout, n->gh, U, w2, bf))) return rc;

@@ -42,7 +42,7 @@ def rand_weights(ch, seed, scale=0.3):
     return fp32(W1), fp32(W2)
 
 
-@pytest.mark.parametrize("ch", [8, 16])
+@pytest.mark.parametrize("ch", [4, 8, 16])
 @pytest.mark.parametrize("bf16", [False, True])
 def test_net_apply_matches_oracle(q, lat, ch, bf16):
     U, Ub, u64, u32 = lat
@@ -60,7 +60,7 @@ def test_net_apply_matches_oracle(q, lat, ch, bf16):
     assert np.array_equal(got1, W1) and np.array_equal(got2, W2)
 
 
-@pytest.mark.parametrize("ch", [8, 16])
+@pytest.mark.parametrize("ch", [4, 8, 16])
 def test_net_gradient_matches_oracle(q, lat, ch):
     U, Ub, u64, u32 = lat
     W1, W2 = rand_weights(ch, 11 + ch)

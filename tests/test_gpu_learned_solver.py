@@ -48,7 +48,7 @@ def oracle_eo(psi, Ub, L, W1, W2, adjoint):
     return out * m
 
 
-@pytest.mark.parametrize("ch", [8, 16])
+@pytest.mark.parametrize("ch", [4, 8, 16])
 @pytest.mark.parametrize("prec,tol", [("DOUBLE", 2e-6), ("SINGLE", 2e-6), ("HALF", 1e-2), ("BF16", 2e-2)])
 @pytest.mark.parametrize("adjoint", [False, True])
 def test_net_eo_matches_oracle(q, ch, prec, tol, adjoint):
@@ -159,9 +159,8 @@ def _check_solution(q, x, srcs, U, L, kappa, mu, st):
 
 @pytest.fixture(scope="module")
 def trained(q):
-    """A network of 8 hidden channels trained at 16^3x32, kappa = 0.1625, on
-    1024 random pairs from 4 gauge fields (the learned workload's recipe, fewer
-    pairs)."""
+    """A network of 4 hidden channels trained at 16^3x32, kappa = 0.1625, on
+    1024 random pairs from 4 gauge fields (the benchmark's recipe)."""
     from qkxtm_mpi_amd import learned
     L, kappa, mu = (16, 16, 16, 32), 0.1625, 0.004
     gauges = []
@@ -170,7 +169,7 @@ def trained(q):
         u32 = q.Gauge(L, q.SINGLE, q.RECON_18, -1)
         u32.convert_from(u)
         gauges.append(u32)
-    net = q.Net(8, seed=1)
+    net = q.Net(4, seed=1)
     loss0 = learned.random_loss(net, gauges, kappa, mu, 64)
     hist = learned.train_random(net, gauges, kappa, mu, 1024, epochs=4, batch=16, lr=2e-3, seed=3)
     loss1 = learned.random_loss(net, gauges, kappa, mu, 64)
@@ -229,7 +228,7 @@ def test_learned_transfer_and_fine_tune_24x24x24x48(q, trained):
     L, kappa, mu = (24, 24, 24, 48), 0.1632, 0.004
     U, u, u32, srcs = _system(q, L, 1, kappa, mu, 2)
     W1, W2 = trained.get_weights()
-    net = q.Net(8, W1, W2)
+    net = q.Net(4, W1, W2)
     before = learned.random_loss(net, [u32], kappa, mu, 16)
     learned.train_random(net, [u32], kappa, mu, 128, epochs=1, batch=16, lr=5e-4, seed=9)
     after = learned.random_loss(net, [u32], kappa, mu, 16)
