@@ -79,38 +79,67 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_ranks():
+    """MPI ranks for the reference CPU run: the host cores this process may use
+    (at most 16, the CPU share of one GPU on the benchmark host), reduced to a
+    divisor of L_t (the lattice is split in t, P = (1,1,1,N), as the reference
+    drivers do)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    n = max(1, min(cores, 16))
+    while L[3] % n:
+        n -= 1
+    return n
+
+
 def cpu_baseline(U, src, fp64_iters):
     """Reference CPU path on this host: per-iteration time of the reference CG
-    (oracle/_ref/ref_harness, the reference's own lib/ code, one MPI rank) over
-    a bounded sample of iterations, scaled by the iteration count the FP64 CG
-    needs for this system.  Falls back to the C restatement (oracle) if the
-    reference build cannot run here."""
+    (oracle/_ref/ref_harness, the reference's own lib/ code) over a bounded
+    sample of iterations, scaled by the iteration count the FP64 CG needs for
+    this system.  It runs under mpiexec on all usable host cores (one rank per
+    core, P = (1,1,1,N)) when mpiexec is there, else as one rank; it falls back
+    to the C restatement (oracle) if the reference build cannot run here."""
+    import shutil
     import oracle as orc
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    sample_iters = 10
     if os.path.exists(harness):
-        try:
-            import struct
-            with tempfile.TemporaryDirectory() as d:
-                fin = os.path.join(d, "in.bin")
-                with open(fin, "wb") as f:
-                    f.write(struct.pack("<4i", *L))
-                    f.write(struct.pack("<dd", KAPPA, MU))
-                    f.write(struct.pack("<ii", 1, -1))
-                    f.write(struct.pack("<d", TOL))
-                    f.write(struct.pack("<i", sample_iters))
-                    f.write(np.ascontiguousarray(U).tobytes())
-                    f.write(np.ascontiguousarray(src).tobytes())
-                    f.write(np.zeros_like(src).tobytes())
-                r = subprocess.run([harness, "time_cg", fin, str(sample_iters)], capture_output=True, text=True,
-                                   timeout=300, env=dict(os.environ, OMP_NUM_THREADS="1"))
-                per_iter = float(r.stderr.strip().splitlines()[-1])
-            return {"value": per_iter * fp64_iters, "unit": "s", "cores": 1, "kind": "reference",
-                    "sample": f"{sample_iters} iterations of the reference CG (lib/solvers.cpp) on D^dag D at 24^3x48,"
-                              f" one MPI rank, {per_iter:.3f} s/iteration, scaled to the {fp64_iters} iterations"
-                              f" the FP64 CG needs"}
-        except Exception as e:  # noqa: BLE001
-            print(f"reference harness unavailable ({e}); timing the oracle instead", file=sys.stderr)
+        mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.path.exists("/opt/conda/bin/mpiexec")
+                                               else None)
+        runs = [cpu_ranks(), 1] if mpiexec else [1]
+        for nranks in dict.fromkeys(runs):
+            sample_iters = 30 if nranks > 1 else 10
+            try:
+                import struct
+                with tempfile.TemporaryDirectory() as d:
+                    fin = os.path.join(d, "in.bin")
+                    with open(fin, "wb") as f:
+                        f.write(struct.pack("<4i", *L))
+                        f.write(struct.pack("<dd", KAPPA, MU))
+                        f.write(struct.pack("<ii", 1, -1))
+                        f.write(struct.pack("<d", TOL))
+                        f.write(struct.pack("<i", sample_iters))
+                        f.write(np.ascontiguousarray(U).tobytes())
+                        f.write(np.ascontiguousarray(src).tobytes())
+                        f.write(np.zeros_like(src).tobytes())
+                    cmd = [harness, "time_cg", fin, str(sample_iters)]
+                    if nranks > 1:
+                        cmd = [mpiexec, "-n", str(nranks)] + cmd
+                    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120 if nranks > 1 else 300,
+                                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+                    if r.returncode != 0:
+                        raise RuntimeError(f"exit {r.returncode}: {r.stderr.strip()[-200:]}")
+                    per_iter = float(r.stderr.strip().splitlines()[-1])
+                how = (f"{nranks} MPI ranks (mpiexec, P = (1,1,1,{nranks}), one rank per host core)" if nranks > 1
+                       else "one MPI rank")
+                return {"value": per_iter * fp64_iters, "unit": "s", "cores": nranks, "kind": "reference",
+                        "sample": f"{sample_iters} iterations of the reference CG (lib/solvers.cpp) on D^dag D at "
+                                  f"24^3x48, {how}, {per_iter:.4f} s/iteration, scaled to the {fp64_iters} "
+                                  f"iterations the FP64 CG needs"}
+            except Exception as e:  # noqa: BLE001
+                print(f"reference harness on {nranks} rank(s) unavailable ({e})", file=sys.stderr)
+        print("timing the oracle instead", file=sys.stderr)
     Ub = orc.apply_bc(U, L, -1)
     t = time.time()
     n = 2

@@ -54,7 +54,10 @@ int g_part_cap = 0;
 double *g_red = nullptr;         // [0] p.Ap, [1] |r|^2 pre-summed when the partials are many
 // Above this many partials a single-workgroup kernel sums them first, so that
 // each consuming workgroup reads one value instead of all of them.
-constexpr int kDirectPartials = 2048;
+#ifndef QKX_DIRECT_PARTIALS
+#define QKX_DIRECT_PARTIALS 2048
+#endif
+constexpr int kDirectPartials = QKX_DIRECT_PARTIALS;
 
 int ensure_state(int nblocks)
 {
