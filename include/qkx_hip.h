@@ -251,7 +251,10 @@ int qkx_twist_add_dslash(qkx_spinor *out, const qkx_spinor *dslash,
                          int basis, int kind, int parity);
 
 /* tm :1925, tm_chiral :1930, tmDag :1935, tmDagtm :1940, Wilson :1949.
- * The hop and the site term run as one fused kernel.  out != in. */
+ * The hop and the site term run as one fused kernel.  out != in.  FP64 fields
+ * on FP64 18-real links run on the library's cached 12-real copy while the
+ * links are SU(3) to 1e-14 (the result moves by about 1e-15 relative); see
+ * qkx_gauge_su3_deviation. */
 int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U,
            const qkx_params *p, int basis, int op);
 

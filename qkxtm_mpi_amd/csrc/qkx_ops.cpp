@@ -155,6 +155,12 @@ int qkx_tm(qkx_spinor *out, const qkx_spinor *in, const qkx_gauge *U, const qkx_
     QKX_CHECK_ARG(in->subset == QKX_SUBSET_FULL && out->subset == QKX_SUBSET_FULL, "qkx_tm: full fields required");
     QKX_CHECK_ARG(same_lattice(out, U->L) && same_lattice(in, U->L), "qkx_tm: lattices differ");
     if (int rc = check_basis(basis)) return rc;
+    // FP64 fields: the stencil runs on the cached 12-real copy of U while its
+    // links are SU(3) to 1e-14 (row 2 rebuilt as conj(row0 x row1), so the
+    // result moves by about 1e-15 relative; 384 instead of 576 link bytes per
+    // site); other links, and the other precisions, use U as given
+    if (in->prec == QKX_PREC_DOUBLE)
+        if (int rc = qkx::gauge_fp64_operator(U, &U)) return rc;
     double k = p->kappa, a = twist_a(p);
     switch (op) {
     case QKX_OP_TM: return qkx::hop(out, in, in, U, basis, 0, -1, -k, 0, 1.0, a, 0);
