@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--batch-kernels-only", action="store_true",
                     help="only apply the FP16 even-odd normal operator to a batch of 12 right-hand sides at "
                          "24^3x48, three times (hardware-counter runs of the batched hop)")
+    ap.add_argument("--cg-graph", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="CG iterations from captured HIP graphs: bit 0 single-RHS solves, bit 1 batched (default 0)")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
                     help="hop autotuner: 0 fixed launch shapes, 1 batched launches (default), 2 every launch")
     ap.add_argument("--batch-share", type=int, default=1, choices=[0, 1, 2],
@@ -775,6 +777,7 @@ def main():
     import qkxtm_mpi_amd as q
     q.init(0)
     q.set_autotune(args.autotune)
+    q.set_cg_graph(args.cg_graph)
     if args.workload == "learned":
         run_learned(q, args)
         return

@@ -162,10 +162,13 @@ int  qkx_set_cg_batch(int iterations);
  * last hop of the normal operator (no Ap field, one launch less per
  * iteration); 0: separate update kernel. */
 int  qkx_set_cg_fuse(int on);
-/* 1: CG iterations are replayed two at a time from captured HIP graphs (one
- * launch per pair); 0 (default): every kernel launched on its own.  The
- * results are identical; on MI355X the replay measured no faster. */
-int  qkx_set_cg_graph(int on);
+/* CG iterations replayed two at a time from captured HIP graphs (one launch
+ * per pair) instead of every kernel launched on its own; the results are
+ * identical.  mode bit 0: the single-RHS solvers (qkx_solve, qkx_cg), bit 1:
+ * the batched solvers (qkx_solve_batch, qkx_solve_learned).  Default 0: on
+ * MI355X the eager launches already run ahead of the GPU, and the replay
+ * measured no faster. */
+int  qkx_set_cg_graph(int mode);
 /* Right-hand sides per thread in batched hops (1..4, each link fetched once
  * per tile; a tile that does not divide nrhs falls back to 1; 0 = automatic,
  * which is 1: wider tiles were measured slower on MI355X). */

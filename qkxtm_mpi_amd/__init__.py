@@ -717,9 +717,12 @@ def set_cg_fuse(on=True):
     _check(lib().qkx_set_cg_fuse(int(on)), "qkx_set_cg_fuse")
 
 
-def set_cg_graph(on=True):
-    """Replay CG iterations from captured HIP graphs (default off)."""
-    _check(lib().qkx_set_cg_graph(int(on)), "qkx_set_cg_graph")
+def set_cg_graph(mode=3):
+    """Replay CG iterations from captured HIP graphs: mode bit 0 the
+    single-RHS solvers, bit 1 the batched solvers (library default 0);
+    True means 3, False 0."""
+    mode = 3 if mode is True else 0 if mode is False else int(mode)
+    _check(lib().qkx_set_cg_graph(mode), "qkx_set_cg_graph")
 
 
 def set_rhs_tile(rt=0):

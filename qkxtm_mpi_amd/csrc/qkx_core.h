@@ -65,6 +65,7 @@ int gauge_recon_deviation(const qkx_gauge_s *U, double *dev);
 // its cached FP64 12-real copy while U's links are SU(3) to 1e-14
 int gauge_fp64_operator(const qkx_gauge_s *U, const qkx_gauge_s **op);
 int cg_fuse_on();   // qkx_set_cg_fuse: 1 when the CG update is fused into the last hop
+bool cg_graph_batched();   // qkx_set_cg_graph: batched solves replay captured iterations
 bool hop_cg_batch_available(const qkx_spinor_s *in, const qkx_gauge_s *U, const qkx_spinor_s *x,
                             const qkx_spinor_s *r);
 int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,

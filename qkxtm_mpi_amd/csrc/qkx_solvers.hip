@@ -46,7 +46,8 @@ struct CGSlot {
 };
 
 int g_cg_fuse = 1;               // fuse the CG update into the operator's last hop
-int g_cg_graph = 0;              // replay pairs of CG iterations from captured graphs (off: measured slower)
+int g_cg_graph = 0;              // replay pairs of CG iterations from captured graphs: bit 0 single-RHS
+                                 // solves, bit 1 batched solves (both off: measured no faster)
 CGSlot *g_slots = nullptr;       // device, two slots
 CGSlot *g_slots_host = nullptr;  // pinned mirror
 double *g_part = nullptr;        // partials: [0, cap) p.Ap, [cap, 2 cap) |r|^2
@@ -469,7 +470,7 @@ struct IterGraphs {
             int n)
     {
         int rc;
-        const bool use = g_cg_graph && !qkx::prof_active();
+        const bool use = (g_cg_graph & 1) && !qkx::prof_active();
         while (n > 0) {
             const int par = E.j & 1;
             if (use && !broken && n >= 2) {
@@ -883,6 +884,7 @@ int write_mslots(int j, int n, const CGSlot *h)
 namespace qkx {
 int cg_batch_size = 16;
 int cg_fuse_on() { return g_cg_fuse; }
+bool cg_graph_batched() { return (g_cg_graph & 2) != 0; }
 }
 
 extern "C" {
@@ -893,9 +895,10 @@ int qkx_set_cg_fuse(int on)
     return QKX_SUCCESS;
 }
 
-int qkx_set_cg_graph(int on)
+int qkx_set_cg_graph(int mode)
 {
-    g_cg_graph = on ? 1 : 0;
+    QKX_CHECK_ARG(mode >= 0 && mode <= 3, "qkx_set_cg_graph: mode 0-3");
+    g_cg_graph = mode;
     return QKX_SUCCESS;
 }
 

@@ -563,6 +563,89 @@ int enqueue_biter_net(BEngine &E, const BatchOp &A, NetPre &np, qkx_spinor_s *x,
     return rc;
 }
 
+// One batched iteration: the preconditioned form while the network is on.
+int enqueue_any(BEngine &E, const BatchOp &A, NetPre *np, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
+                qkx_spinor_s *Ap)
+{
+    return (np && np->on) ? enqueue_biter_net(E, A, *np, x, r, p, Ap) : enqueue_biter(E, A, x, r, p, Ap);
+}
+
+// Pairs of batched iterations replayed from captured HIP graphs.  After two
+// iterations the slot parity is back where it started, so one executable per
+// starting parity serves the whole solve: every per-RHS scalar the kernels
+// use (alpha, beta, the stopping thresholds and flags) lives in the device
+// slots, not in kernel arguments, so nothing has to be captured again -- only
+// when the iteration changes form (the network's fallback).  The captured
+// work is kernels only (no memset or copy nodes).  The first iterations of a
+// solve are launched one by one, so that the hop autotuner has timed every
+// launch kind before a capture records its shapes.  Captures that fail (or
+// instrumentation, whose events cannot be captured) fall back to launching
+// the iterations one by one.
+struct BIterGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool broken = false;
+    bool net_on = false;   // the iteration form the executables hold
+    int eager = 2;         // iterations still to launch one by one
+    BIterGraphs() = default;
+    BIterGraphs(const BIterGraphs &) = delete;
+    BIterGraphs &operator=(const BIterGraphs &) = delete;
+    ~BIterGraphs() { drop(); }
+    void drop()
+    {
+        for (auto &e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        exec[0] = exec[1] = nullptr;
+    }
+    bool capture(BEngine &E, const BatchOp &A, NetPre *np, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
+                 qkx_spinor_s *Ap, int par)
+    {
+        if (hipStreamBeginCapture(qkx::g_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const int j0 = E.j;
+        int rc = enqueue_any(E, A, np, x, r, p, Ap);
+        if (!rc) rc = enqueue_any(E, A, np, x, r, p, Ap);
+        E.j = j0;   // nothing ran yet: the launch of the graph runs the pair
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(qkx::g_stream, &g);
+        bool ok = !rc && ec == hipSuccess && g;
+        if (ok && hipGraphInstantiate(&exec[par], g, nullptr, nullptr, 0) != hipSuccess) {
+            exec[par] = nullptr;
+            ok = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return ok;
+    }
+    int run(BEngine &E, const BatchOp &A, NetPre *np, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
+            qkx_spinor_s *Ap, int n)
+    {
+        const bool on = np && np->on;
+        if (on != net_on) {
+            drop();
+            net_on = on;
+        }
+        const bool use = qkx::cg_graph_batched() && !qkx::prof_active();
+        while (n > 0) {
+            const int par = E.j & 1;
+            if (use && !broken && eager <= 0 && n >= 2) {
+                if (!exec[par] && !capture(E, A, np, x, r, p, Ap, par)) broken = true;
+                if (!broken) {
+                    QKX_HIP(hipGraphLaunch(exec[par], qkx::g_stream));
+                    E.j += 2;
+                    n -= 2;
+                    continue;
+                }
+            }
+            if (int rc = enqueue_any(E, A, np, x, r, p, Ap)) return rc;
+            eager--;
+            n--;
+        }
+        return QKX_SUCCESS;
+    }
+};
+
 int brestart(BEngine &E, qkx_spinor_s *p, qkx_spinor_s *r, int initial)
 {
     const int n = entries(r), nr = r->nrhs;
@@ -785,13 +868,10 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
         return true;
     };
     if (all_done()) status = QKX_SUCCESS;
+    BIterGraphs graphs;
     while (status != QKX_SUCCESS && h.k < o->maxIter) {
         int batch = std::min(qkx::cg_batch_size, o->maxIter - h.k);
-        for (int i = 0; i < batch; i++) {
-            rc = (np && np->on) ? enqueue_biter_net(E, Al, *np, xiter, r, pp, Ap)
-                                : enqueue_biter(E, Al, xiter, r, pp, Ap);
-            if (rc) return rc;
-        }
+        if ((rc = graphs.run(E, Al, np, xiter, r, pp, Ap, batch))) return rc;
         if ((rc = read_bslot(E, h))) return rc;
         if (h.flag == 0) continue;
         if (h.flag == 3) {

@@ -350,3 +350,35 @@ def test_solves_are_deterministic(q, share):
     (a1, ab, ia, iba), (c1, cb, ic, ibc) = runs
     assert ia == ic and iba == ibc
     assert np.array_equal(a1, c1) and np.array_equal(ab, cb)
+
+
+def test_batched_graph_replay_identical_to_eager(q):
+    """Batched iterations replayed from captured HIP graphs give the same solutions, bit for bit, and the same
+    iteration counts as the same iterations launched one by one: two solves in
+    a row each, the plain batch and the learned solver (whose iterations add
+    the network's launches)."""
+    L, nrhs = (8, 8, 8, 16), 4
+    U = q.gen_gauge(L, 0.3, 3)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    u32 = q.Gauge(L, q.SINGLE, q.RECON_18, -1)
+    u32.convert_from(u)
+    p = q.params(L, 0.1632, 0.004, 1, -1)
+    b = q.Spinor(L, host=np.stack([q.gen_z2(L, 40 + r) for r in range(nrhs)]), nrhs=nrhs)
+    net = q.Net(4, seed=2)      # close to the identity
+    out = {}
+    try:
+        for mode in (0, 2, 0, 2):
+            q.set_cg_graph(mode)
+            x, y = q.Spinor(L, nrhs=nrhs), q.Spinor(L, nrhs=nrhs)
+            sb = q.solve_batch(x, b, u, p, eo=True, inner_prec=q.HALF, delta=0.05, tol=1e-10)
+            sl = q.solve_learned(y, b, u, p, net, u32, inner_prec=q.SINGLE, delta=0.1, tol=1e-10, stall=0.5)
+            print(f"graph mode {mode}: batch {sb[0].iterations} iterations, learned {sl[0].iterations}", flush=True)
+            out.setdefault(mode, []).append((x.download(), [s.iterations for s in sb], y.download(),
+                                             [s.iterations for s in sl]))
+    finally:
+        q.set_cg_graph(0)
+    ref = out[0][0]
+    for mode, runs in out.items():
+        for v in runs:
+            assert np.array_equal(v[0], ref[0]) and v[1] == ref[1], mode
+            assert np.array_equal(v[2], ref[2]) and v[3] == ref[3], mode

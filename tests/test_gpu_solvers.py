@@ -164,7 +164,7 @@ def test_graph_replay_identical_to_eager(q):
     out = {}
     try:
         for on in (False, True):
-            q.set_cg_graph(on)
+            q.set_cg_graph(3 if on else 0)
             for rep in range(2):
                 print(f"graph={on} rep={rep}", flush=True)
                 x = q.Spinor(L)
@@ -174,7 +174,7 @@ def test_graph_replay_identical_to_eager(q):
                 sc = cg(y, b)
                 out[(on, rep)] = (x.download(), st.iterations, st.reliable_updates, y.download(), sc.iterations)
     finally:
-        q.set_cg_graph(False)
+        q.set_cg_graph(0)
     ref = out[(False, 0)]
     for key, v in out.items():
         assert np.array_equal(v[0], ref[0]) and v[1:3] == ref[1:3], key
