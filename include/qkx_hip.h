@@ -124,11 +124,12 @@ int  qkx_prof_reset(void);
  * the software-pipeline depth (1 or 2 hops in flight) of the FP64 and of the
  * FP32/FP16/BF16 kernels. */
 int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
-/* Order in which the hopping kernels walk the sites of a parity: 1 (default)
- * blocked -- x, then blocks of up to 8 y rows and up to 8 t slices, then z --
- * so neighbour spinors and backward links are reused from L2; 0: checkerboard
- * index order.  Results per site are identical; reductions sum in another
- * order. */
+/* Order in which the hopping kernels walk the sites of a parity.  1: blocked
+ * -- x, then blocks of up to 8 y rows and up to 8 t slices, then z -- so that
+ * neighbour spinors and backward links are reused from L2; 0: checkerboard
+ * index order; 2 (default): blocked for the full-lattice FP64 hops (tm and
+ * its family, the full-lattice CG), checkerboard order for the rest.  Results
+ * per site are identical; reductions sum in another order. */
 int  qkx_set_site_order(int blocked);
 /* Hops on batches of 3-16 FP16 right-hand sides with FP16 links can fetch
  * and reconstruct each site's 8 links once per workgroup, into LDS, for all

@@ -749,9 +749,11 @@ def set_batch_link_sharing(mode=1):
     _check(lib().qkx_set_batch_link_sharing(int(mode)), "qkx_set_batch_link_sharing")
 
 
-def set_site_order(blocked=True):
-    """Blocked (default) or checkerboard-index site order of the hopping kernels."""
-    _check(lib().qkx_set_site_order(int(blocked)), "qkx_set_site_order")
+def set_site_order(mode=2):
+    """Site order of the hopping kernels: 1 (or True) blocked, 0 (or False)
+    checkerboard index order, 2 (default) blocked for the full-lattice FP64
+    hops only."""
+    _check(lib().qkx_set_site_order(int(mode)), "qkx_set_site_order")
 
 
 def set_tuning(xcd_remap=True, block=256, depth64=2, depth32=2):

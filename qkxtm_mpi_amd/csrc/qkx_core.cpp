@@ -321,7 +321,7 @@ int qkx_autotune_report(char *buf, int len, int *needed)
 
 int qkx_set_site_order(int blocked)
 {
-    QKX_CHECK_ARG(blocked == 0 || blocked == 1, "qkx_set_site_order: 0 or 1");
+    QKX_CHECK_ARG(blocked >= 0 && blocked <= 2, "qkx_set_site_order: 0, 1 or 2");
     qkx::set_site_order(blocked);
     return QKX_SUCCESS;
 }

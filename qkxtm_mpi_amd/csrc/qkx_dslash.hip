@@ -22,6 +22,11 @@
 #include <vector>
 #include "qkx_core.h"
 
+// FP64 hop outputs are written with non-temporal stores (1) or plain ones (0)
+#ifndef QKX_NT_OUT64
+#define QKX_NT_OUT64 1
+#endif
+
 namespace {
 
 enum : int { K0 = 0, KP1, KM1, KPI, KMI, K2 };
@@ -752,7 +757,10 @@ __device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI>
         }
         fin.r.store(so, rv);
     } else {
-        a.out.store(a.out_off[p] + cb * nr + rr, res);
+        if constexpr (std::is_same_v<SOut, SpinorD> && QKX_NT_OUT64)
+            a.out.store_nt(a.out_off[p] + cb * nr + rr, res);
+        else
+            a.out.store(a.out_off[p] + cb * nr + rr, res);
         if (a.norm_partial) {
 #pragma unroll
             for (int k = 0; k < 12; k++)
@@ -1284,7 +1292,12 @@ bool share_links_for(int nrhs, long units)
     if (!g_share_links || nrhs < kShareMinRhs || g_block / nrhs > kShareSitesFew) return false;
     return g_share_links == 2 || units >= kShareMinUnits;
 }
-int g_site_order = 0;   // 1: blocked walk (HopTrav), 0 (default): checkerboard index order
+#ifndef QKX_SITE_ORDER_DEFAULT
+#define QKX_SITE_ORDER_DEFAULT 2
+#endif
+// 0: checkerboard index order; 1: blocked walk (HopTrav) for every hop;
+// 2: blocked walk for the full-lattice FP64 hops only
+int g_site_order = QKX_SITE_ORDER_DEFAULT;
 const int *g_skip = nullptr;
 
 // Launch shape of one hop: workgroup size, links shared through LDS by the
@@ -1330,10 +1343,12 @@ thread_local int g_no_tune = 0;     // callers that must not allocate (scratch f
 // +z neighbour and its use as a -z neighbour -- under 1.5 MB, well inside
 // the 4 MB L2 of one XCD.
 constexpr double kTravWindow = 1.5e6;
+constexpr double kTravAutoBytes = 700.0;   // FP64 full-lattice hops move 768-1152 B per site
 HopTrav hop_trav(const Geom &g, int par_out, double site_bytes)
 {
     HopTrav tr{};
     if (!g_site_order) return tr;
+    if (g_site_order == 2 && !(par_out < 0 && site_bytes >= kTravAutoBytes)) return tr;
     int yb = 1;
     for (int d = 2; d <= 8; d++)
         if (g.L[1] % d == 0) yb = d;

@@ -277,6 +277,16 @@ struct SpinorD {
 #pragma unroll
         for (int k = 0; k < 12; k++) st(p + (size_t)k * stride, (uint32_t)s, make_double2(v[k].re, v[k].im));
     }
+    // the same, marked non-temporal: the 192 B per site of a hop's output
+    // stream to memory instead of displacing the neighbour spinors and links
+    // that the launch is still re-reading from L2
+    __device__ __forceinline__ void store_nt(int s, const Cx<double> v[12]) const
+    {
+        typedef __attribute__((ext_vector_type(2))) double d2v;
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            __builtin_nontemporal_store(d2v{v[k].re, v[k].im}, reinterpret_cast<d2v *>(p + (size_t)k * stride + s));
+    }
 };
 
 // FP32: 6 x float4, chunk j = components 2j, 2j+1

@@ -100,16 +100,17 @@ def test_blocked_site_order_identical(q, L):
     names = ("tm", "tmDag", "tm_chiral", "tm_xx_EE", "tm_xxDagTm_xx_OO", "applyDslash_xx_OE", "applyDslash")
     got = {}
     try:
-        for order in (0, 1):
+        for order in (0, 1, 2):
             q.set_site_order(order)
             out = [gpu_op(q, n, u, ps, ph, p) for n in names]
             yh.tm_xxDagTm_xx(xh, uh, p, q.EE)
             yb.tm_xxDagTm_xx(xb, u, p, q.EE)
             got[order] = out + [yh.download(), yb.download()]
     finally:
-        q.set_site_order(0)
-    for k, (a, b) in enumerate(zip(got[0], got[1])):
-        assert np.array_equal(a, b), k
+        q.set_site_order(2)
+    for order in (1, 2):
+        for k, (a, b) in enumerate(zip(got[0], got[order])):
+            assert np.array_equal(a, b), (order, k)
     for n, g in zip(names, got[1]):
         assert rel(g, oracle_op(n, Ub, psi, phi, L, kappa, mu, 1)) <= TOL64, n
 
