@@ -1018,6 +1018,17 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
     if (valid) hs = hop_site_walk(g, a.tr, p, v, cb);
     const int q = p ^ 1;
     const int qoff = q * g.Vh, poff = p * g.Vh;
+    const int in_q = a.in_off[q];
+    const S &in = a.in;
+    constexpr int FW = DAG ? 1 : 0;
+    constexpr int BW = DAG ? 0 : 1;
+    // neighbour spinors one hop ahead of the arithmetic; the first two are
+    // requested before the link phase (their latency overlaps it)
+    typename S::Raw f0{}, f1{};
+    if (valid) {
+        f0 = in.template fetch<spin_mask<B, 0, FW>()>(in_q + hs.nb[0] * nr + rr);
+        f1 = in.template fetch<spin_mask<B, 0, BW>()>(in_q + hs.nb[1] * nr + rr);
+    }
     if (rr == 0) {
 #pragma unroll
         for (int d = 0; d < 8; d++) {
@@ -1041,13 +1052,9 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
     __syncthreads();
     double nrm2 = 0.0;
     if (valid) {
-        const int in_q = a.in_off[q];
         Cx<T> R[12];
 #pragma unroll
         for (int k = 0; k < 12; k++) R[k] = {T(0), T(0)};
-        constexpr int FW = DAG ? 1 : 0;
-        constexpr int BW = DAG ? 0 : 1;
-        const S &in = a.in;
         // one hop with the link from LDS: project, multiply, reconstruct
         auto hop = [&](auto MUC, auto KINDC, auto DAGLC, const typename S::Raw &raw, int d) {
             constexpr int MU = decltype(MUC)::value, KIND = decltype(KINDC)::value;
@@ -1075,9 +1082,6 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
         using IB = std::integral_constant<int, BW>;
         using BT = std::true_type;
         using BF = std::false_type;
-        // neighbour spinors one hop ahead of the arithmetic
-        auto f0 = in.template fetch<spin_mask<B, 0, FW>()>(in_q + hs.nb[0] * nr + rr);
-        auto f1 = in.template fetch<spin_mask<B, 0, BW>()>(in_q + hs.nb[1] * nr + rr);
         hop(I0{}, IF{}, BF{}, f0, 0);
         QKX_SCHED_FENCE();
         auto f2 = in.template fetch<spin_mask<B, 1, FW>()>(in_q + hs.nb[2] * nr + rr);
