@@ -396,19 +396,23 @@ struct Epi {
 template <int B, class T>
 __device__ __forceinline__ void epilogue(const Epi &e, const Cx<T> R[12], const Cx<T> *x, Cx<T> out[12])
 {
+    // every term is always formed (a zero coefficient adds an exact zero), as
+    // explicit FMAs in a fixed order, so that no per-component selects are
+    // needed and every kernel that shares this epilogue rounds identically
     const T cR = T(e.cR), gR = T(e.gR), cX = T(e.cX), gX = T(e.gX);
     Cx<T> g5R[12], g5x[12];
-    if (gR != T(0)) gamma5<B>(R, g5R);
-    if (x && gX != T(0)) gamma5<B>(x, g5x);
+    gamma5<B>(R, g5R);
+    if (x) gamma5<B>(x, g5x);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-        Cx<T> v = cR * R[k];
-        if (gR != T(0)) v = v + gR * timesi(g5R[k]);
+        // cR R + gR i g5R (+ cX x + gX i g5x), i (a, b) = (-b, a)
+        T re = fma(gR, -g5R[k].im, cR * R[k].re);
+        T im = fma(gR, g5R[k].re, cR * R[k].im);
         if (x) {
-            v = v + cX * x[k];
-            if (gX != T(0)) v = v + gX * timesi(g5x[k]);
+            re = fma(gX, -g5x[k].im, fma(cX, x[k].re, re));
+            im = fma(gX, g5x[k].re, fma(cX, x[k].im, im));
         }
-        out[k] = v;
+        out[k] = {re, im};
     }
     if (e.g5out) {
         Cx<T> w[12];
