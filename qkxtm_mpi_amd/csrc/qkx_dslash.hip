@@ -445,6 +445,7 @@ struct HopArgs {
     int par_out;   // -1: all sites; 0/1: that parity only
     int nout;      // output entries (sites x right-hand sides)
     int nrhs;      // right-hand sides; thread unit u -> site u / nrhs, rhs u % nrhs
+    FastDiv dnr;   // division by nrhs
     int nblk8;     // grid size / 8 when XCD remapping is on, else 0
     HopTrav tr;    // order in which the sites of a parity are walked
     const int *skip;   // when non-null and *skip != 0 the launch does nothing
@@ -478,7 +479,9 @@ __device__ __forceinline__ void block_partial_rhs(double v, int nr, int u0, doub
     __syncthreads();
     if ((int)threadIdx.x < nr) {
         double s = 0;
-        for (int t = ((int)threadIdx.x - u0 + nr) % nr; t < (int)blockDim.x; t += nr) s += vals[t];
+        int t = (int)threadIdx.x - u0;   // threadIdx.x and u0 are both below nr
+        if (t < 0) t += nr;
+        for (; t < (int)blockDim.x; t += nr) s += vals[t];
         dst[threadIdx.x] = s;
     }
 }
@@ -659,7 +662,7 @@ __device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI>
     constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
     const Geom &g = a.g;
     const int nr = a.nrhs;
-    const int v = nr == 1 ? u : u / nr;   // position of the site in the walk
+    const int v = nr == 1 ? u : fdiv(u, a.dnr);   // position of the site in the walk
     const int rr = u - v * nr;
     double nrm2 = 0.0;
     int cb = v;
@@ -779,7 +782,8 @@ __device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI>
             block_partial(nrm2, a.norm_partial + bid);
         } else {
             // one partial per right-hand side, summed in thread order
-            const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+            const int ub = u - (int)threadIdx.x;   // the workgroup's first unit, >= 0
+            const int u0 = ub - fdiv(ub, a.dnr) * nr;
             block_partial_rhs(nrm2, nr, u0, a.norm_partial + bid * nr);
         }
     }
@@ -794,7 +798,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     int p, u;   // parity and unit (site x rhs) index within it
     hop_unit(a.g, a.par_out, a.nblk8, a.nrhs, p, u, blockIdx.x);
     Epi ep = a.epi[0];                 // uniform (scalar) load for a single rhs
-    if (a.nrhs > 1) ep = a.epi[u % a.nrhs];
+    if (a.nrhs > 1) ep = a.epi[u - fdiv(u, a.dnr) * a.nrhs];
     hop_body<B, DAG, AUX, DEPTH>(a, fin, blockIdx.x, p, u, ep);
 }
 
@@ -868,9 +872,9 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
     const int nr = a.nrhs;
     int p, u;
     hop_unit(g, a.par_out, a.nblk8, nr, p, u, blockIdx.x);
-    const int v = u / nr;
+    const int v = fdiv(u, a.dnr);
     const int rr = u - v * nr;
-    const int j = threadIdx.x / nr;   // site within the workgroup
+    const int j = fdiv((int)threadIdx.x, a.dnr);   // site within the workgroup
     int cb = v;
     HopSite hs{};
     const bool valid = v < g.Vh;
@@ -884,7 +888,7 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
     }
     __syncthreads();
     // (site, direction) tasks, dense over the workgroup's threads
-    const int nsites = blockDim.x / nr;
+    const int nsites = fdiv((int)blockDim.x, a.dnr);
     for (int t = threadIdx.x; t < nsites * 8; t += blockDim.x) {
         const int js = t >> 3, d = t & 7;
         const int sl = lsite[js][d];
@@ -986,7 +990,8 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
     }
     if (CG || a.norm_partial) {
         // one partial per right-hand side, summed in thread order
-        const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+        const int ub = u - (int)threadIdx.x;   // the workgroup's first unit, >= 0
+        const int u0 = ub - fdiv(ub, a.dnr) * nr;
         double *dst = CG ? fin_r2_part(fin) : a.norm_partial;
         block_partial_rhs(nrm2, nr, u0, dst + blockIdx.x * nr);
     }
@@ -1013,9 +1018,9 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
     const int nr = a.nrhs;
     int p, u;
     hop_unit(g, a.par_out, a.nblk8, nr, p, u, blockIdx.x);
-    const int v = u / nr;
+    const int v = fdiv(u, a.dnr);
     const int rr = u - v * nr;
-    const int j = threadIdx.x / nr;
+    const int j = fdiv((int)threadIdx.x, a.dnr);
     int cb = v;
     HopSite hs{};
     const bool valid = v < g.Vh;
@@ -1041,7 +1046,7 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
         }
     }
     __syncthreads();
-    const int nsites = blockDim.x / nr;
+    const int nsites = fdiv((int)blockDim.x, a.dnr);
     for (int t = threadIdx.x; t < nsites * 8; t += blockDim.x) {
         const int js = t >> 3, d = t & 7;
         const int sl = lsite[js][d];
@@ -1125,7 +1130,8 @@ __global__ __launch_bounds__(256) void hop_share_f_kernel(HopArgs<S, S, S, G> a)
         }
     }
     if (a.norm_partial) {
-        const int u0 = (((u - (int)threadIdx.x) % nr) + nr) % nr;
+        const int ub = u - (int)threadIdx.x;   // the workgroup's first unit, >= 0
+        const int u0 = ub - fdiv(ub, a.dnr) * nr;
         block_partial_rhs(nrm2, nr, u0, a.norm_partial + blockIdx.x * nr);
     }
 }
@@ -1462,6 +1468,7 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     else a.aux_off[0] = a.aux_off[1] = 0;
     a.par_out = par_out;
     a.nrhs = in->nrhs;
+    a.dnr = make_fastdiv(a.nrhs);
     a.nout = (par_out < 0 ? a.g.V : a.g.Vh) * a.nrhs;
     for (int r = 0; r < a.nrhs; r++) a.epi[r] = e[r];
     a.skip = g_skip;
@@ -1637,6 +1644,7 @@ int launch_hop_multi_t(int n, qkx_spinor_s *const *out, const qkx_spinor_s *cons
         else a.aux_off[0] = a.aux_off[1] = 0;
         a.par_out = par_out;
         a.nrhs = 1;
+        a.dnr = make_fastdiv(1);
         const int sites_out = par_out < 0 ? a.g.V : a.g.Vh;
         a.nout = sites_out;
         a.epi[0] = e[i];
