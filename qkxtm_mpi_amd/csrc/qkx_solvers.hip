@@ -1206,11 +1206,17 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
             E.stop2 = std::min(E.stop2, h.r2) * std::max(1e-4, 0.5 * tol2 * b2 / s2);
         }
         if (mixed) {
-            // reliable update: r = Op^dag s in FP64, into the low-precision r;
-            // p = r + beta p and the slot restart on the device
+            // reliable update: r = Op^dag s, s the FP64 true residual, into the
+            // low-precision r; p = r + beta p and the slot restart on the device.
+            // An FP16 r is formed by the FP16 operator from s rounded to FP16
+            // (Ap is free between iterations): its rounding is that of the FP16
+            // storage, and it is cheaper than the FP32 operator.
             nrel++;
-            if ((rc = qkx_spinor_copy(sm->f, s.f))) return rc;
-            if (r.f->prec == QKX_PREC_SINGLE) {
+            if (r.f->prec == QKX_PREC_HALF) {
+                if ((rc = qkx_spinor_copy(Ap.f, s.f)) || (rc = Al.apply_dag(r.f, Ap.f))) return rc;
+            } else if ((rc = qkx_spinor_copy(sm->f, s.f))) {
+                return rc;
+            } else if (r.f->prec == QKX_PREC_SINGLE) {
                 if ((rc = Am.apply_dag(r.f, sm->f))) return rc;
             } else if ((rc = Am.apply_dag(rm->f, sm->f)) || (rc = qkx_spinor_copy(r.f, rm->f))) {
                 return rc;
@@ -1401,9 +1407,13 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
             }
             if (h[i].flag == 2) M.stop2 = std::min(M.stop2, h[i].r2) * std::max(1e-4, 0.5 * tol2 * M.b2 / s2);
             if (mixed) {
+                // as in qkx_solve: an FP16 r from the FP16 operator on s rounded to FP16
                 M.nrel++;
-                if ((rc = qkx_spinor_copy(M.sm, M.s))) return rc;
-                if (M.r->prec == QKX_PREC_SINGLE) {
+                if (M.r->prec == QKX_PREC_HALF) {
+                    if ((rc = qkx_spinor_copy(M.Ap, M.s)) || (rc = M.Al.apply_dag(M.r, M.Ap))) return rc;
+                } else if ((rc = qkx_spinor_copy(M.sm, M.s))) {
+                    return rc;
+                } else if (M.r->prec == QKX_PREC_SINGLE) {
                     if ((rc = M.Am.apply_dag(M.r, M.sm))) return rc;
                 } else if ((rc = M.Am.apply_dag(M.rm, M.sm)) || (rc = qkx_spinor_copy(M.r, M.rm))) {
                     return rc;

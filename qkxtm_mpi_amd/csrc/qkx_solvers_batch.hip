@@ -802,10 +802,17 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
     Ah.tmp_full = hf; Ah.tmp_odd = ho; Ah.tmp_even = he;
     Al.tmp_full = lf; Al.tmp_odd = lo; Al.tmp_even = le;
     Am.tmp_full = nullptr; Am.tmp_odd = mo; Am.tmp_even = me;   // Op^dag needs no full temporary
-    // r_out = Op^dag s in FP32 (mixed) or FP64, s an FP64 field; r_out any precision
+    // r_out = Op^dag s in FP32 (mixed) or FP64, s an FP64 field; r_out any
+    // precision.  An FP16 r_out is formed by the FP16 operator from s rounded
+    // to FP16 (Ap is free between iterations): the rounding of its storage.
     auto dag_residual0 = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
         if (!mixed) return Ah.apply_dag(r_out, src64);
-        int e = qkx_spinor_copy(sm, src64);
+        int e;
+        if (r_out->prec == QKX_PREC_HALF) {
+            if ((e = qkx_spinor_copy(Ap, src64))) return e;
+            return Al.apply_dag(r_out, Ap);
+        }
+        e = qkx_spinor_copy(sm, src64);
         if (e) return e;
         if (r_out->prec == QKX_PREC_SINGLE) return Am.apply_dag(r_out, sm);
         if ((e = Am.apply_dag(rm, sm))) return e;
