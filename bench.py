@@ -952,7 +952,9 @@ def main():
                                "warm-ups, median of 5 trials",
                      "in_solve": {"launches": n_in, "avg_launch_us": ms_in / max(n_in, 1) * 1e3,
                                   "GB/s": by_in / max(ms_in, 1e-30) / 1e6,
-                                  "note": "per-launch events inside the timed solves (adds event overhead)"}},
+                                  "note": "per-launch events inside the timed solves (adds event overhead); "
+                                          "compulsory bytes, with the x, r and p traffic of the CG update fused "
+                                          "into the last hop of each normal operator"}},
         "cpu_baseline": cpu,
         "time_to_solution": results,
         "speedup_vs_fp64_cg": results["fp64_cg_full"]["time_s"] / med,

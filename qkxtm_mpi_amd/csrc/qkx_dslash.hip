@@ -631,6 +631,15 @@ template <class SX, class SR, class SP> struct BCgFin {
     const double *pap;
     double *r2_part;
 };
+// bytes of one site of a spinor field in each storage (FP16: 48 B and the scale)
+template <class S> constexpr double site_bytes_of()
+{
+    if constexpr (std::is_same_v<S, SpinorD>) return 192.0;
+    else if constexpr (std::is_same_v<S, SpinorF>) return 96.0;
+    else if constexpr (std::is_same_v<S, SpinorH>) return 52.0;
+    else return 48.0;
+}
+
 template <class F> struct IsBCg : std::false_type {};
 template <class SX, class SR, class SP> struct IsBCg<BCgFin<SX, SR, SP>> : std::true_type {};
 
@@ -1541,6 +1550,11 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     double bytes = (double)sites_out * (par_out < 0 ? 4 : 8) * wg + (double)nout * 2 * ws;
     if (aux && aux != in) bytes += (double)nout * ws;
     a.tr = hop_trav(a.g, par_out, bytes / sites_out);
+    // a fused CG update also reads x, r and p and writes x and r at each
+    // output (site, rhs)
+    if constexpr (!store)
+        bytes += (double)nout * (2 * site_bytes_of<decltype(fin.x)>() + 2 * site_bytes_of<decltype(fin.r)>() +
+                                 site_bytes_of<decltype(fin.p)>());
     qkx::prof_begin();
     const int depth = sh.depth;
     const dim3 grid(nb), blk(bs);
