@@ -1082,18 +1082,20 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     qkx::Temp lf(L, lp, QKX_SUBSET_FULL), lo_(L, lp, QKX_SUBSET_ODD), le(L, lp, QKX_SUBSET_EVEN);
     NormalOp Al{Ui, kappa, a, nrm, o->eo != 0, lf.f, lo_.f, le.f};
 
-    // FP32 operator for the parts whose result is stored at FP16/FP32 anyway:
-    // the normal-equation right-hand side and the residual of each reliable
-    // update (r = Op^dag s); s itself, the true residual, stays FP64
+    // FP32 operator for the parts whose result is stored at FP32 (or BF16)
+    // anyway: the normal-equation right-hand side and the residual of each
+    // reliable update (r = Op^dag s); s itself, the true residual, stays FP64.
+    // An FP16 solve forms both with its FP16 operator and needs none of it.
+    const bool need_mid = mixed && lp != QKX_PREC_HALF;
     const qkx_gauge_s *Um = U;
-    if (mixed) {
+    if (need_mid) {
         qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
         if ((rc = qkx::gauge_cached(&Uc->mid, U, QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
         Um = Uc->mid;
     }
     std::unique_ptr<qkx::Temp> mo, me, mf, sm, rm;   // FP32 temporaries of a mixed solve
     NormalOp Am{Um, kappa, a, nrm, o->eo != 0, nullptr, nullptr, nullptr};
-    if (mixed) {
+    if (need_mid) {
         const int S = QKX_PREC_SINGLE;
         if (o->eo) {
             mo.reset(new qkx::Temp(L, S, QKX_SUBSET_ODD));
@@ -1140,20 +1142,25 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
     } else {
         if ((rc = qkx_spinor_copy(bh.f, b))) return rc;
     }
-    // normal-equation rhs bn = Op^dag bh (FP32 for a mixed solve: it only
-    // seeds r and the first stopping guess); x = 0; r = bn; p = r
-    if (mixed) {
-        if ((rc = qkx_spinor_copy(sm->f, bh.f)) || (rc = Am.apply_dag(rm->f, sm->f)) ||
-            (rc = qkx_spinor_copy(bn.f, rm->f)))
+    // normal-equation rhs bn = Op^dag bh (in the inner storage for an FP16
+    // solve, FP32 for the other mixed solves: it only seeds r and the first
+    // stopping guess); x = 0; r = bn; p = r
+    if (r.f->prec == QKX_PREC_HALF) {
+        if ((rc = qkx_spinor_copy(Ap.f, bh.f)) || (rc = Al.apply_dag(r.f, Ap.f))) return rc;
+    } else {
+        if (mixed) {
+            if ((rc = qkx_spinor_copy(sm->f, bh.f)) || (rc = Am.apply_dag(rm->f, sm->f)) ||
+                (rc = qkx_spinor_copy(bn.f, rm->f)))
+                return rc;
+        } else if ((rc = Ah.apply_dag(bn.f, bh.f))) {
             return rc;
-    } else if ((rc = Ah.apply_dag(bn.f, bh.f))) {
-        return rc;
+        }
+        if ((rc = qkx_spinor_copy(r.f, bn.f))) return rc;
     }
     if ((rc = qkx_spinor_zero(xs.f)) || (rc = qkx_spinor_zero(xl.f))) return rc;
-    if ((rc = qkx_spinor_copy(r.f, bn.f))) return rc;
 
     const double tol2 = o->tol * o->tol;
-    const double bn2 = host_norm2(bn.f);
+    const double bn2 = host_norm2(r.f->prec == QKX_PREC_HALF ? r.f : bn.f);
     Engine E;
     E.delta2 = mixed ? o->delta * o->delta : 0.0;
     E.stop2 = tol2 * bn2;   // first guess, tightened when the true residual disagrees
