@@ -732,11 +732,13 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
         }
     }
 
-    // FP32 links for the parts whose result is stored at FP16/FP32 anyway:
-    // the normal-equation right-hand side and the residual r = Op^dag s of
-    // each reliable update (s itself, the true residual, stays FP64)
+    // FP32 links for the parts whose result is stored at FP32 (or BF16)
+    // anyway: the normal-equation right-hand side and the residual r = Op^dag s
+    // of each reliable update (s itself, the true residual, stays FP64).  An
+    // FP16 solve forms both with its FP16 operator and needs none of it.
+    const bool need_mid = mixed && lp != QKX_PREC_HALF;
     const qkx_gauge_s *Um = U;
-    if (mixed) {
+    if (need_mid) {
         qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U);
         if ((rc = qkx::gauge_cached(&Uc->mid, U, QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
         Um = Uc->mid;
@@ -785,14 +787,15 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
         lo = ho;
         le = he;
     }
-    qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub), *xs = T(QKX_PREC_DOUBLE, sub),
+    qkx_spinor_s *bh = T(QKX_PREC_DOUBLE, sub), *bn = T(QKX_PREC_DOUBLE, sub, rp != QKX_PREC_HALF),
+                 *xs = T(QKX_PREC_DOUBLE, sub),
                  *s = T(QKX_PREC_DOUBLE, sub),
                  *bo = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo), *to = T(QKX_PREC_DOUBLE, QKX_SUBSET_ODD, eo),
                  *r = T(rp, sub), *pp = T(lp, sub), *Ap = T(lp, sub), *xl = T(xp, sub, mixed),
                  *tf = eo ? T(QKX_PREC_DOUBLE, QKX_SUBSET_FULL) : hf;
     const int S32 = QKX_PREC_SINGLE;
-    qkx_spinor_s *mo = T(S32, QKX_SUBSET_ODD, mixed && eo), *me = T(S32, QKX_SUBSET_EVEN, mixed && eo),
-                 *sm = T(S32, sub, mixed), *rm = T(S32, sub, mixed);
+    qkx_spinor_s *mo = T(S32, QKX_SUBSET_ODD, need_mid && eo), *me = T(S32, QKX_SUBSET_EVEN, need_mid && eo),
+                 *sm = T(S32, sub, need_mid), *rm = T(S32, sub, need_mid);
     if (np) {
         np->t = T(lp, QKX_SUBSET_EVEN);
         np->w = T(lp, QKX_SUBSET_EVEN);
@@ -844,13 +847,15 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
     } else {
         if ((rc = qkx_spinor_copy(bh, b))) return rc;
     }
-    // normal-equation source Op^dag bh; with the network N^dag Op^dag bh
+    // normal-equation source Op^dag bh; with the network N^dag Op^dag bh.  An
+    // FP16 solve forms it in r itself, with its FP16 operator.
     std::vector<double> bn2_plain;
-    if ((rc = dag_residual0(bn, bh)) || (rc = norms(bn, bn2_plain))) return rc;
-    if (np && np->on && (rc = qkx::net_apply_eo(np->net, bn, bn, np->Un, 1))) return rc;
+    qkx_spinor_s *src = r->prec == QKX_PREC_HALF ? r : bn;
+    if ((rc = dag_residual0(src, bh)) || (rc = norms(src, bn2_plain))) return rc;
+    if (np && np->on && (rc = qkx::net_apply_eo(np->net, src, src, np->Un, 1))) return rc;
     if ((rc = qkx_spinor_zero(xs)) || (xl && (rc = qkx_spinor_zero(xl)))) return rc;
-    if ((rc = qkx_spinor_copy(r, bn))) return rc;
-    if ((rc = norms(bn, bn2))) return rc;
+    if (src != r && (rc = qkx_spinor_copy(r, bn))) return rc;
+    if ((rc = norms(src, bn2))) return rc;
 
     const double tol2 = o->tol * o->tol;
     BEngine E;
