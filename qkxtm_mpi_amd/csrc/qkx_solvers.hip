@@ -1306,9 +1306,14 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
         M.Ui = M.Uop;
         M.Um = U[i];
         qkx_gauge_s *Uc = const_cast<qkx_gauge_s *>(U[i]);
+        // FP16 instances form their normal-equation source and reliable-update
+        // residuals with the FP16 operator; the others use an FP32 one
+        const bool need_mid = mixed && lp != QKX_PREC_HALF;
         if (mixed) {
             if ((rc = qkx::gauge_cached(&Uc->inner, U[i], lp, QKX_RECON_12))) return rc;
             M.Ui = Uc->inner;
+        }
+        if (need_mid) {
             if ((rc = qkx::gauge_cached(&Uc->mid, U[i], QKX_PREC_SINGLE, QKX_RECON_12))) return rc;
             M.Um = Uc->mid;
         }
@@ -1320,12 +1325,12 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
         M.he = M.T(L, D, E);
         M.lo = mixed ? M.T(L, lp, O) : M.ho;
         M.le = mixed ? M.T(L, lp, E) : M.he;
-        M.mo = M.T(L, F32, O, mixed);
-        M.me = M.T(L, F32, E, mixed);
-        M.sm = M.T(L, F32, E, mixed);
-        M.rm = M.T(L, F32, E, mixed);
+        M.mo = M.T(L, F32, O, need_mid);
+        M.me = M.T(L, F32, E, need_mid);
+        M.sm = M.T(L, F32, E, need_mid);
+        M.rm = M.T(L, F32, E, need_mid);
         M.bh = M.T(L, D, E);
-        M.bn = M.T(L, D, E);
+        M.bn = M.T(L, D, E, rp != QKX_PREC_HALF);
         M.xs = M.T(L, D, E);
         M.s = M.T(L, D, E);
         M.bo = M.T(L, D, O);
@@ -1348,16 +1353,20 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
         if ((rc = qkx_spinor_copy(M.bh, b[i])) || (rc = qkx_spinor_copy(M.bo, b[i]))) return rc;
         if ((rc = qkx::site(M.to, M.bo, nullptr, QKX_BASIS_UKQCD, -1, 1.0 / M.nrm, -M.a / M.nrm, 0, 0, 0))) return rc;
         if ((rc = qkx::hop(M.bh, M.to, M.bh, M.Uop, QKX_BASIS_UKQCD, 0, 0, M.kappa, 0, 1.0, 0, 0))) return rc;
-        if (mixed) {
-            if ((rc = qkx_spinor_copy(M.sm, M.bh)) || (rc = M.Am.apply_dag(M.rm, M.sm)) ||
-                (rc = qkx_spinor_copy(M.bn, M.rm)))
+        if (rp == QKX_PREC_HALF) {
+            if ((rc = qkx_spinor_copy(M.Ap, M.bh)) || (rc = M.Al.apply_dag(M.r, M.Ap))) return rc;
+        } else {
+            if (mixed) {
+                if ((rc = qkx_spinor_copy(M.sm, M.bh)) || (rc = M.Am.apply_dag(M.rm, M.sm)) ||
+                    (rc = qkx_spinor_copy(M.bn, M.rm)))
+                    return rc;
+            } else if ((rc = M.Ah.apply_dag(M.bn, M.bh))) {
                 return rc;
-        } else if ((rc = M.Ah.apply_dag(M.bn, M.bh))) {
-            return rc;
+            }
+            if ((rc = qkx_spinor_copy(M.r, M.bn))) return rc;
         }
         if ((rc = qkx_spinor_zero(M.xs)) || (M.xl && (rc = qkx_spinor_zero(M.xl)))) return rc;
-        if ((rc = qkx_spinor_copy(M.r, M.bn))) return rc;
-        M.stop2 = tol2 * host_norm2(M.bn);
+        M.stop2 = tol2 * host_norm2(rp == QKX_PREC_HALF ? M.r : M.bn);
         if (!(M.b2 > 0)) M.state = 1;   // zero source: x = 0
     }
     // slots: parity 0 from scratch, every instance restarted into parity 1
