@@ -76,6 +76,9 @@ def parse():
                     help="CG iterations from captured HIP graphs: bit 0 single-RHS solves, bit 1 batched (default 0)")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
                     help="hop autotuner: 0 fixed launch shapes, 1 batched launches (default), 2 every launch")
+    ap.add_argument("--staging", type=int, default=0, choices=[0, 1, 2],
+                    help="x and y neighbours of the single-RHS FP16 half-lattice hop staged in LDS: 0 per-thread "
+                         "gathers (default), 1 per workgroup, 2 per wave")
     ap.add_argument("--batch-share", type=int, default=1, choices=[0, 1, 2],
                     help="links shared per workgroup in batched hops: 0 never, 1 automatic (default), 2 always")
     return ap.parse_args()
@@ -781,6 +784,7 @@ def main():
     q.init(0)
     q.set_autotune(args.autotune)
     q.set_cg_graph(args.cg_graph)
+    q.set_neighbour_staging(args.staging)
     if args.workload == "learned":
         run_learned(q, args)
         return
@@ -943,10 +947,12 @@ def main():
         "data": "synthetic (seeded SU(3) gauge eps=0.3, Z2 source)",
         "config": {"workload": "24^3x48 single-RHS twisted-mass solve to true residual 1e-10, even-odd "
                                "preconditioned, kappa=0.1632 mu=0.004 antiperiodic t",
-                   "lattice": list(L), "solver": best, "reliable_delta": kw.get("delta"), "sites": V},
+                   "lattice": list(L), "solver": best, "reliable_delta": kw.get("delta"), "sites": V,
+                   "neighbour_staging": args.staging},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hop_kernel, FP16 even-odd hop (16-bit 12-real links) of the inner operator",
+                     "kernel": ("hop_kernel" if not args.staging else "hop_stage_kernel") +
+                               ", FP16 even-odd hop (16-bit 12-real links) of the inner operator",
                      "avg_launch_us": us16, "bytes_per_launch": by16 / n16,
                      "timing": "HIP events, 100 back-to-back normal-operator calls (4 hops each) after 10 "
                                "warm-ups, median of 5 trials",

@@ -131,6 +131,12 @@ int  qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32);
  * its family, the full-lattice CG), checkerboard order for the rest.  Results
  * per site are identical; reductions sum in another order. */
 int  qkx_set_site_order(int blocked);
+/* Single-RHS FP16 hops on one parity (FP16 12-real links, checkerboard site
+ * order, x row of at most 64 sites): 1 stages each workgroup's x and y
+ * neighbour spinors in LDS with one coalesced read of the input range they
+ * span, 2 does the same per wave (no workgroup barrier); 0 (default): every
+ * thread gathers its own.  The results are identical. */
+int  qkx_set_neighbour_staging(int mode);
 /* Hops on batches of 3-16 FP16 right-hand sides with FP16 links can fetch
  * and reconstruct each site's 8 links once per workgroup, into LDS, for all
  * the site's right-hand sides (and fuse the batched CG update into the last

@@ -76,7 +76,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qkx_init": [_I], "qkx_finalize": [], "qkx_last_error": [], "qkx_device_count": [_IP], "qkx_sync": [], "qkx_mem_info": [_DP, _DP],
     "qkx_timer_start": [], "qkx_timer_stop": [_DP], "qkx_prof_enable": [_I], "qkx_prof_read": [_IP, _DP, _DP],
-    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_gauge_su3_deviation": [_P, _DP], "qkx_set_batch_link_sharing": [_I], "qkx_set_autotune": [_I], "qkx_autotune_report": [ctypes.c_char_p, _I, _IP], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
+    "qkx_prof_read_prec": [_I, _IP, _DP, _DP], "qkx_prof_reset": [], "qkx_set_tuning": [_I, _I, _I, _I], "qkx_set_cg_batch": [_I], "qkx_set_rhs_tile": [_I], "qkx_set_site_order": [_I], "qkx_set_neighbour_staging": [_I], "qkx_gauge_su3_deviation": [_P, _DP], "qkx_set_batch_link_sharing": [_I], "qkx_set_autotune": [_I], "qkx_autotune_report": [ctypes.c_char_p, _I, _IP], "qkx_set_cg_fuse": [_I], "qkx_set_cg_graph": [_I],
     "qkx_gauge_create": [ctypes.POINTER(_P), _IP, _I, _I, _I], "qkx_gauge_destroy": [_P],
     "qkx_gauge_upload": [_P, _P], "qkx_gauge_convert": [_P, _P], "qkx_gauge_download": [_P, _P],
     "qkx_ape_smear": [_P, _P, _I, _D], "qkx_gauge_project_su3": [_P], "qkx_plaquette": [_DP, _P],
@@ -754,6 +754,13 @@ def set_site_order(mode=2):
     checkerboard index order, 2 (default) blocked for the full-lattice FP64
     hops only."""
     _check(lib().qkx_set_site_order(int(mode)), "qkx_set_site_order")
+
+
+def set_neighbour_staging(mode=0):
+    """Single-RHS FP16 hops on one parity stage the x and y neighbour spinors
+    in LDS -- 1: per workgroup, 2: per wave -- or gather them per thread (0,
+    the default); the results are identical."""
+    _check(lib().qkx_set_neighbour_staging(int(mode)), "qkx_set_neighbour_staging")
 
 
 def set_tuning(xcd_remap=True, block=256, depth64=2, depth32=2):

@@ -326,6 +326,13 @@ int qkx_set_site_order(int blocked)
     return QKX_SUCCESS;
 }
 
+int qkx_set_neighbour_staging(int on)
+{
+    QKX_CHECK_ARG(on >= 0 && on <= 2, "qkx_set_neighbour_staging: 0, 1 or 2");
+    qkx::set_neighbour_staging(on);
+    return QKX_SUCCESS;
+}
+
 int qkx_set_tuning(int xcd_remap, int block, int depth64, int depth32)
 {
     QKX_CHECK_ARG(block == 64 || block == 128 || block == 256, "qkx_set_tuning: block must be 64, 128 or 256");

@@ -81,6 +81,7 @@ void set_hop_tuning(int xcd_remap, int block, int depth64, int depth32);
 void set_hop_skip(const int *flag);
 void set_rhs_tile(int rt);
 void set_site_order(int blocked);
+void set_neighbour_staging(int on);
 void set_batch_link_sharing(int on);
 // hop autotuner (qkx_dslash.hip): 0 off, 1 batched launches, 2 every launch;
 // set_no_tune(1) keeps the calling thread's launches on cached or default

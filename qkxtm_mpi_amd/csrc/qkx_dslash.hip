@@ -657,6 +657,81 @@ template <> struct HopWaves<SpinorD> { static constexpr int v = QKX_HOP64_WAVES;
 template <> struct HopWaves<SpinorH> { static constexpr int v = QKX_HOP16_WAVES; };
 template <> struct HopWaves<SpinorB> { static constexpr int v = QKX_HOP16_WAVES; };
 
+// The epilogue of output unit (cb, rr) of parity p and what Fin does with it:
+// store it (adding |out|^2 to nrm2 when partials are requested) or apply
+// the fused CG update (adding |r|^2 to nrm2).
+template <int B, bool AUX, class SIn, class SOut, class SAux, class G, int NEPI, class Fin, class T>
+__device__ __forceinline__ void hop_finish(const HopArgs<SIn, SOut, SAux, G, NEPI> &a, Fin &fin, int p, int cb,
+                                           int rr, const Epi &ep, const Cx<T> R[12], double &nrm2)
+{
+    constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
+    const int nr = a.nrhs;
+    Cx<T> res[12];
+    if constexpr (AUX) {
+        Cx<T> xv[12];
+        a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
+        epilogue<B>(ep, R, xv, res);
+    } else {
+        epilogue<B, T>(ep, R, nullptr, res);
+    }
+    if constexpr (CG) {
+        // x += alpha p, r -= alpha Ap (single right-hand side)
+        const int so = a.out_off[p] + cb;
+        using TX = typename decltype(fin.x)::Real;
+        using TR = typename decltype(fin.r)::Real;
+        using TP = typename decltype(fin.p)::Real;
+        Cx<TX> xv[12];
+        Cx<TP> pv[12];
+        fin.x.load(so, xv);
+        fin.p.load(so, pv);
+        const TX ax = TX(fin.alpha);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+        fin.x.store(so, xv);
+        Cx<TR> rv[12];
+        fin.r.load(so, rv);
+        const TR ar = TR(fin.alpha);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            rv[k] = {rv[k].re - ar * TR(res[k].re), rv[k].im - ar * TR(res[k].im)};
+            nrm2 += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+        }
+        fin.r.store(so, rv);
+    } else {
+        if constexpr (std::is_same_v<SOut, SpinorD> && QKX_NT_OUT64)
+            a.out.store_nt(a.out_off[p] + cb * nr + rr, res);
+        else
+            a.out.store(a.out_off[p] + cb * nr + rr, res);
+        if (a.norm_partial) {
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
+        }
+    }
+}
+
+// The workgroup's norm partials (one per right-hand side) from each
+// thread's nrm2; u is the thread's unit.  Every thread of the workgroup
+// calls it.
+template <class A, class Fin>
+__device__ __forceinline__ void hop_partials(const A &a, Fin &fin, int bid, int u, double nrm2)
+{
+    constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
+    const int nr = a.nrhs;
+    if constexpr (CG) {
+        block_partial(nrm2, fin.r2_part + bid);
+    } else if (a.norm_partial) {
+        if (nr == 1) {
+            block_partial(nrm2, a.norm_partial + bid);
+        } else {
+            // one partial per right-hand side, summed in thread order
+            const int ub = u - (int)threadIdx.x;   // the workgroup's first unit, >= 0
+            const int u0 = ub - fdiv(ub, a.dnr) * nr;
+            block_partial_rhs(nrm2, nr, u0, a.norm_partial + bid * nr);
+        }
+    }
+}
+
 // The work of one hop workgroup: bid is its index within the launch (or
 // within its instance of a multi-instance launch), (p, u) its thread's parity
 // and unit (site x rhs) index within it (hop_unit), ep the epilogue of the
@@ -668,7 +743,6 @@ __device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI>
                                          const Epi &ep)
 {
     using T = typename SOut::Real;
-    constexpr bool CG = !std::is_same_v<Fin, StoreFin>;
     const Geom &g = a.g;
     const int nr = a.nrhs;
     const int v = nr == 1 ? u : fdiv(u, a.dnr);   // position of the site in the walk
@@ -741,61 +815,9 @@ __device__ __forceinline__ void hop_body(const HopArgs<SIn, SOut, SAux, G, NEPI>
                                   tw_m ? bc : T(1));
     }
 
-    Cx<T> res[12];
-    if constexpr (AUX) {
-        Cx<T> xv[12];
-        a.aux.load(a.aux_off[p] + cb * nr + rr, xv);
-        epilogue<B>(ep, R, xv, res);
-    } else {
-        epilogue<B, T>(ep, R, nullptr, res);
+    hop_finish<B, AUX>(a, fin, p, cb, rr, ep, R, nrm2);
     }
-    if constexpr (CG) {
-        // x += alpha p, r -= alpha Ap (single right-hand side)
-        const int so = a.out_off[p] + cb;
-        using TX = typename decltype(fin.x)::Real;
-        using TR = typename decltype(fin.r)::Real;
-        using TP = typename decltype(fin.p)::Real;
-        Cx<TX> xv[12];
-        Cx<TP> pv[12];
-        fin.x.load(so, xv);
-        fin.p.load(so, pv);
-        const TX ax = TX(fin.alpha);
-#pragma unroll
-        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
-        fin.x.store(so, xv);
-        Cx<TR> rv[12];
-        fin.r.load(so, rv);
-        const TR ar = TR(fin.alpha);
-#pragma unroll
-        for (int k = 0; k < 12; k++) {
-            rv[k] = {rv[k].re - ar * TR(res[k].re), rv[k].im - ar * TR(res[k].im)};
-            nrm2 += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
-        }
-        fin.r.store(so, rv);
-    } else {
-        if constexpr (std::is_same_v<SOut, SpinorD> && QKX_NT_OUT64)
-            a.out.store_nt(a.out_off[p] + cb * nr + rr, res);
-        else
-            a.out.store(a.out_off[p] + cb * nr + rr, res);
-        if (a.norm_partial) {
-#pragma unroll
-            for (int k = 0; k < 12; k++)
-                nrm2 += (double)res[k].re * (double)res[k].re + (double)res[k].im * (double)res[k].im;
-        }
-    }
-    }
-    if constexpr (CG) {
-        block_partial(nrm2, fin.r2_part + bid);
-    } else if (a.norm_partial) {
-        if (nr == 1) {
-            block_partial(nrm2, a.norm_partial + bid);
-        } else {
-            // one partial per right-hand side, summed in thread order
-            const int ub = u - (int)threadIdx.x;   // the workgroup's first unit, >= 0
-            const int u0 = ub - fdiv(ub, a.dnr) * nr;
-            block_partial_rhs(nrm2, nr, u0, a.norm_partial + bid * nr);
-        }
-    }
+    hop_partials(a, fin, bid, u, nrm2);
 }
 
 template <class SIn, class SOut, class SAux, class G, int B, bool DAG, bool AUX, int DEPTH, class Fin = StoreFin>
@@ -809,6 +831,147 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HopWaves<SI
     Epi ep = a.epi[0];                 // uniform (scalar) load for a single rhs
     if (a.nrhs > 1) ep = a.epi[u - fdiv(u, a.dnr) * a.nrhs];
     hop_body<B, DAG, AUX, DEPTH>(a, fin, blockIdx.x, p, u, ep);
+}
+
+// Single-RHS FP16 half-lattice hop with the x and y neighbours staged in
+// LDS.  A staging domain -- the workgroup (WAVE false) or each wave (WAVE
+// true: no workgroup barrier) -- computes the consecutive checkerboard
+// indices [db, db + ds); their +-x neighbours lie in their own x rows and
+// their +-y neighbours one row away, so all of them, bar the +-y neighbours
+// across the periodic y boundary, are input sites of [db - Lh0, db + ds +
+// Lh0).  The domain reads that range once, coalesced, into LDS; the z and t
+// neighbours, every link and the y-boundary neighbours are gathered from
+// global memory as in hop_kernel.  The arithmetic and its order are
+// hop_kernel's: the same bits.  The host launches it with Lh0 <= kStageRow,
+// blockDim.x <= 256 a multiple of 64, 2 Lh0 <= blockDim.x for workgroup
+// domains, and stage_lds_bytes() of dynamic LDS.
+constexpr int kStageRow = 64;
+__host__ __device__ constexpr int stage_sites(bool wave, int block, int lh0)
+{
+    return wave ? (block / 64) * (64 + 2 * lh0) : block + 2 * lh0;
+}
+inline size_t stage_lds_bytes(bool wave, int block, int lh0)
+{
+    return (size_t)stage_sites(wave, block, lh0) * (3 * sizeof(uint4) + sizeof(float));
+}
+// minimum waves per SIMD requested from the register allocator
+#ifndef QKX_STAGE_WAVES
+#define QKX_STAGE_WAVES 4
+#endif
+
+template <int B, bool DAG, bool AUX, bool WAVE, class Fin = StoreFin>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QKX_STAGE_WAVES, 8))) void hop_stage_kernel(
+    HopArgs<SpinorH, SpinorH, SpinorH, Gauge16<false>> a, Fin fin)
+{
+    using T = float;
+    using Raw = SpinorH::Raw;
+    using LRaw = Gauge16<false>::Raw;
+    using Ops = HopOps<SpinorH, Gauge16<false>>;
+    extern __shared__ uint4 stage_lds[];
+    if (a.skip && *a.skip) return;   // uniform over the grid
+    fin.begin(a);
+    const Geom &g = a.g;
+    int p, u;
+    hop_unit(g, a.par_out, a.nblk8, 1, p, u, blockIdx.x);
+    const int q = p ^ 1;
+    const int qoff = q * g.Vh, poff = p * g.Vh;
+    const int in_q = a.in_off[q];
+    // this thread's staging domain and its LDS region: sc[3][cap], sn[cap]
+    const int cap = stage_sites(WAVE, blockDim.x, g.Lh0);
+    const int ds = WAVE ? 64 : (int)blockDim.x;
+    const int li = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;   // index within the domain
+    const int reg = WAVE ? (int)(threadIdx.x >> 6) * (64 + 2 * g.Lh0) : 0;
+    uint4 *sc = stage_lds + reg;
+    float *sn = reinterpret_cast<float *>(stage_lds + 3 * cap) + reg;
+    const int db = u - li;
+    const int lo = max(db - g.Lh0, 0);
+    const int n = min(db + ds + g.Lh0, g.Vh) - lo;   // <= 0 past the last site
+    const SpinorH &in = a.in;
+    const Gauge16<false> &U = a.U;
+    constexpr int FW = DAG ? 1 : 0;
+    constexpr int BW = DAG ? 0 : 1;
+    const bool live = u < g.Vh;
+    HopSite hs{};
+    if (live) hs = hop_site(g, p, u);
+    // the x links go out first and stay in flight across the staging
+    LRaw lxp, lxm;
+    if (live) {
+        lxp = U.fetch(0, poff + u);
+        lxm = U.fetch(0, qoff + hs.nb[1]);
+    }
+    {
+        // n <= ds + 2 Lh0: at most 2 rows of ds for a workgroup, 3 for a wave
+        constexpr int K = WAVE ? 3 : 2;
+        Raw sr[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (li + k * ds < n) sr[k] = in.fetch<3>(in_q + lo + li + k * ds);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = li + k * ds;
+            if (i < n) {
+                sc[i] = sr[k].c[0];
+                sc[cap + i] = sr[k].c[1];
+                sc[2 * cap + i] = sr[k].c[2];
+                sn[i] = sr[k].n;
+            }
+        }
+    }
+    // a wave's LDS accesses complete in program order
+    if constexpr (WAVE) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    else __syncthreads();
+    double nrm2 = 0.0;
+    if (live) {
+        auto staged = [&](int nb) {
+            const int i = nb - lo;
+            Raw r;
+            r.c[0] = sc[i];
+            r.c[1] = sc[cap + i];
+            r.c[2] = sc[2 * cap + i];
+            r.n = sn[i];
+            return r;
+        };
+        // a +-y neighbour across the y boundary can lie outside the staged
+        // range: gathered one hop ahead, like the z and t operands
+        auto y_ops = [&](int nb, int s_link, auto mask) {
+            Ops o;
+            o.link = U.fetch(1, s_link);
+            if ((unsigned)(nb - lo) >= (unsigned)n) o.psi = in.fetch<decltype(mask)::value>(in_q + nb);
+            return o;
+        };
+        auto y_psi = [&](Ops &o, int nb) {
+            if ((unsigned)(nb - lo) < (unsigned)n) o.psi = staged(nb);
+        };
+        const T bc = T(g.bc);
+        Cx<T> R[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) R[k] = {T(0), T(0)};
+        Ops oyp = y_ops(hs.nb[2], poff + u, std::integral_constant<int, spin_mask<B, 1, FW>()>{});
+        apply_dir<B, 0, FW, false>(R, in, U, Ops{staged(hs.nb[0]), lxp}, T(1));
+        QKX_SCHED_FENCE();
+        Ops oym = y_ops(hs.nb[3], qoff + hs.nb[3], std::integral_constant<int, spin_mask<B, 1, BW>()>{});
+        apply_dir<B, 0, BW, true>(R, in, U, Ops{staged(hs.nb[1]), lxm}, T(1));
+        QKX_SCHED_FENCE();
+        const Ops ozp = fetch_dir<B, 2, FW>(in, in_q + hs.nb[4], U, poff + u);
+        y_psi(oyp, hs.nb[2]);
+        apply_dir<B, 1, FW, false>(R, in, U, oyp, T(1));
+        QKX_SCHED_FENCE();
+        const Ops ozm = fetch_dir<B, 2, BW>(in, in_q + hs.nb[5], U, qoff + hs.nb[5]);
+        y_psi(oym, hs.nb[3]);
+        apply_dir<B, 1, BW, true>(R, in, U, oym, T(1));
+        QKX_SCHED_FENCE();
+        const Ops otp = fetch_dir<B, 3, FW>(in, in_q + hs.nb[6], U, poff + u);
+        apply_dir<B, 2, FW, false>(R, in, U, ozp, T(1));
+        QKX_SCHED_FENCE();
+        const Ops otm = fetch_dir<B, 3, BW>(in, in_q + hs.nb[7], U, qoff + hs.nb[7]);
+        apply_dir<B, 2, BW, true>(R, in, U, ozm, T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 3, FW, false>(R, in, U, otp, hs.tw_p ? bc : T(1));
+        QKX_SCHED_FENCE();
+        apply_dir<B, 3, BW, true>(R, in, U, otm, hs.tw_m ? bc : T(1));
+        hop_finish<B, AUX>(a, fin, p, u, 0, a.epi[0], R, nrm2);
+    }
+    hop_partials(a, fin, blockIdx.x, u, nrm2);
 }
 
 // Several independent hops -- each on its own lattice, gauge field and
@@ -1321,6 +1484,12 @@ bool share_links_for(int nrhs, long units)
 // 0: checkerboard index order; 1: blocked walk (HopTrav) for every hop;
 // 2: blocked walk for the full-lattice FP64 hops only
 int g_site_order = QKX_SITE_ORDER_DEFAULT;
+#ifndef QKX_STAGE_DEFAULT
+#define QKX_STAGE_DEFAULT 0
+#endif
+// single-RHS FP16 half-lattice hops stage their x and y neighbours in LDS
+// (hop_stage_kernel): 1 on, 0 off
+int g_nb_stage = QKX_STAGE_DEFAULT;
 const int *g_skip = nullptr;
 
 // Launch shape of one hop: workgroup size, links shared through LDS by the
@@ -1555,6 +1724,13 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     if constexpr (!store)
         bytes += (double)nout * (2 * site_bytes_of<decltype(fin.x)>() + 2 * site_bytes_of<decltype(fin.r)>() +
                                  site_bytes_of<decltype(fin.p)>());
+    // single-RHS FP16 half-lattice hops in checkerboard order: x and y
+    // neighbours staged in LDS when switched on and the x row fits
+    constexpr bool stage16 = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> && QKX_HOP16_DOT2;
+    const bool stage_wave = g_nb_stage == 2;
+    const bool stage = stage16 && g_nb_stage && !share && rt == 1 && a.nrhs == 1 && par_out >= 0 && !a.tr.on &&
+                       a.g.Lh0 <= kStageRow && bs <= 256 && bs % 64 == 0 && (stage_wave || 2 * a.g.Lh0 <= bs);
+    const size_t stage_bytes = stage ? stage_lds_bytes(stage_wave, bs, a.g.Lh0) : 0;
     qkx::prof_begin();
     const int depth = sh.depth;
     const dim3 grid(nb), blk(bs);
@@ -1608,6 +1784,20 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
                 else        { if (aux) QKX_LAUNCH_SHARE(false, true, kShareSites); else QKX_LAUNCH_SHARE(false, false, kShareSites); }
             }
 #undef QKX_LAUNCH_SHARE
+        }
+    } else if (stage) {
+        if constexpr (stage16 && !bcg) {
+#define QKX_LAUNCH_STAGE(DG, AX, WV) \
+    hipLaunchKernelGGL((hop_stage_kernel<B, DG, AX, WV, Fin>), grid, blk, stage_bytes, st, a, fin)
+#define QKX_LAUNCH_STAGES(WV)                                                                                 \
+    do {                                                                                                      \
+        if (dagger) { if (aux) QKX_LAUNCH_STAGE(true, true, WV); else if constexpr (store) QKX_LAUNCH_STAGE(true, false, WV); } \
+        else        { if (aux) QKX_LAUNCH_STAGE(false, true, WV); else if constexpr (store) QKX_LAUNCH_STAGE(false, false, WV); } \
+    } while (0)
+            if (stage_wave) QKX_LAUNCH_STAGES(true);
+            else QKX_LAUNCH_STAGES(false);
+#undef QKX_LAUNCH_STAGES
+#undef QKX_LAUNCH_STAGE
         }
     } else if (rt == 4) {
         if constexpr (tiles) QKX_LAUNCH_TILES(4);
@@ -1734,6 +1924,8 @@ void set_hop_skip(const int *flag) { g_skip = flag; }
 void set_rhs_tile(int rt) { g_rhs_tile = rt; }
 
 void set_site_order(int blocked) { g_site_order = blocked; }
+
+void set_neighbour_staging(int on) { g_nb_stage = on; }
 
 void set_batch_link_sharing(int on) { g_share_links = on; }
 

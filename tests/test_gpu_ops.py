@@ -115,6 +115,56 @@ def test_blocked_site_order_identical(q, L):
         assert rel(g, oracle_op(n, Ub, psi, phi, L, kappa, mu, 1)) <= TOL64, n
 
 
+@pytest.mark.parametrize("L", [(8, 8, 4, 8), (8, 80, 4, 4), (128, 4, 4, 4), (6, 10, 6, 6)])
+def test_fp16_neighbour_staging_identical(q, L):
+    """Single-RHS FP16 hops on one parity with the x and y neighbours staged
+    in LDS, per workgroup or per wave, give the same bits as the per-thread
+    gathers: the hop into either
+    parity of a full field, the even-odd twisted operator and its normal
+    operator (both hop directions) on parity fields, and the FP16 mixed-precision
+    solve (iterations and solution).  The lattices put y-boundary neighbours
+    inside and outside a workgroup's staged range (an xy slice of 320 sites),
+    stage the largest x row (64 sites) and leave the last workgroup partly
+    empty (540 sites of a parity)."""
+    print(f"case L={L}", flush=True)
+    kappa, mu = 0.1632, 0.004
+    U, psi, _ = _setup(q, L, 0.3, 41, -1)
+    u = q.Gauge(L, boundaryT=-1, host=U)
+    uh = q.Gauge(L, q.HALF, q.RECON_12, -1)
+    uh.convert_from(u)
+    p = q.params(L, kappa, mu, 1, -1)
+    b = q.Spinor(L, host=psi)
+    xf = q.Spinor(L, q.HALF, host=psi)
+    runs = {}
+    try:
+        for on in (0, 1, 2):
+            q.set_neighbour_staging(on)
+            out = []
+            for par in (q.EO, q.OE):
+                y = q.Spinor(L, q.HALF, host=np.zeros_like(psi))
+                y.applyDslash_xx(xf, uh, par)
+                out.append(y.download())
+            for parity, sub in ((q.EE, q.EVEN), (q.OO, q.ODD)):
+                x = q.Spinor(L, q.HALF, sub, host=psi)
+                for normal in (0, 1):
+                    y = q.Spinor(L, q.HALF, sub)
+                    (y.tm_xxDagTm_xx if normal else y.tm_xx)(x, uh, p, parity)
+                    out.append(y.download())
+            xs = q.Spinor(L)
+            st = q.solve(xs, b, u, p, eo=True, inner_prec=q.HALF, delta=0.01, tol=1e-10)
+            assert st.converged
+            runs[on] = (out, xs.download(), st.iterations)
+    finally:
+        q.set_neighbour_staging(0)
+    a, xa, ia = runs[0]
+    for mode in (1, 2):
+        c, xc, ic = runs[mode]
+        assert ia == ic, mode
+        assert np.array_equal(xa, xc), mode
+        for k, (g0, g1) in enumerate(zip(a, c)):
+            assert np.array_equal(g0, g1), (mode, k)
+
+
 @pytest.mark.parametrize("prec,recon,tol", [
     ("SINGLE", 18, 2e-6), ("SINGLE", 12, 2e-6), ("HALF", 12, 2e-3), ("BF16", 12, 2e-2)])
 def test_low_precision_tm(q, prec, recon, tol):
