@@ -76,6 +76,9 @@ def parse():
                     help="CG iterations from captured HIP graphs: bit 0 single-RHS solves, bit 1 batched (default 0)")
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
                     help="hop autotuner: 0 fixed launch shapes, 1 batched launches (default), 2 every launch")
+    ap.add_argument("--cg-fuse", type=int, default=2, choices=[0, 1, 2],
+                    help="CG update: 1 fused into the last hop, 0 separate kernel, 2 fused with x += alpha p in the "
+                         "direction kernel for FP16 solves (default)")
     ap.add_argument("--staging", type=int, default=0, choices=[0, 1, 2],
                     help="x and y neighbours of the single-RHS FP16 half-lattice hop staged in LDS: 0 per-thread "
                          "gathers (default), 1 per workgroup, 2 per wave")
@@ -785,6 +788,7 @@ def main():
     q.set_autotune(args.autotune)
     q.set_cg_graph(args.cg_graph)
     q.set_neighbour_staging(args.staging)
+    q.set_cg_fuse(args.cg_fuse)
     if args.workload == "learned":
         run_learned(q, args)
         return
@@ -948,7 +952,7 @@ def main():
         "config": {"workload": "24^3x48 single-RHS twisted-mass solve to true residual 1e-10, even-odd "
                                "preconditioned, kappa=0.1632 mu=0.004 antiperiodic t",
                    "lattice": list(L), "solver": best, "reliable_delta": kw.get("delta"), "sites": V,
-                   "neighbour_staging": args.staging},
+                   "neighbour_staging": args.staging, "cg_fuse": args.cg_fuse},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("hop_kernel" if not args.staging else "hop_stage_kernel") +
@@ -959,8 +963,9 @@ def main():
                      "in_solve": {"launches": n_in, "avg_launch_us": ms_in / max(n_in, 1) * 1e3,
                                   "GB/s": by_in / max(ms_in, 1e-30) / 1e6,
                                   "note": "per-launch events inside the timed solves (adds event overhead); "
-                                          "compulsory bytes, with the x, r and p traffic of the CG update fused "
-                                          "into the last hop of each normal operator"}},
+                                          "compulsory bytes, with the traffic of the CG update fused into the "
+                                          "last hop of each normal operator (r; x and p only where x is not "
+                                          "left to the direction kernel)"}},
         "cpu_baseline": cpu,
         "time_to_solution": results,
         "speedup_vs_fp64_cg": results["fp64_cg_full"]["time_s"] / med,

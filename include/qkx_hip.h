@@ -165,10 +165,12 @@ int qkx_autotune_report(char *buf, int len, int *needed);
 /* CG iterations enqueued between host reads of the solver state (default 16;
  * 1 gives no launches past the stopping iteration). */
 int  qkx_set_cg_batch(int iterations);
-/* 1 (default): the CG update x += alpha p, r -= alpha Ap is fused into the
- * last hop of the normal operator (no Ap field, one launch less per
- * iteration); 0: separate update kernel. */
-int  qkx_set_cg_fuse(int on);
+/* 1: the CG update x += alpha p, r -= alpha Ap is fused into the last hop
+ * of the normal operator (no Ap field, one launch less per iteration);
+ * 2 (default): as 1, but where p and r are FP16 and x FP32 (the FP16 solves)
+ * x += alpha p moves into the direction kernel, which reads p anyway (the
+ * same bits, one read of p less per iteration); 0: separate update kernel. */
+int  qkx_set_cg_fuse(int mode);
 /* CG iterations replayed two at a time from captured HIP graphs (one launch
  * per pair) instead of every kernel launched on its own; the results are
  * identical.  mode bit 0: the single-RHS solvers (qkx_solve, qkx_cg), bit 1:

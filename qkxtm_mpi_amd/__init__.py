@@ -713,8 +713,11 @@ def set_cg_batch(iterations=16):
     _check(lib().qkx_set_cg_batch(iterations), "qkx_set_cg_batch")
 
 
-def set_cg_fuse(on=True):
-    _check(lib().qkx_set_cg_fuse(int(on)), "qkx_set_cg_fuse")
+def set_cg_fuse(mode=2):
+    """CG update fused into the normal operator's last hop (1 or True), in a
+    separate kernel (0 or False), or fused with x += alpha p moved into the
+    direction kernel for FP16 solves (2, the default)."""
+    _check(lib().qkx_set_cg_fuse(int(mode)), "qkx_set_cg_fuse")
 
 
 def set_cg_graph(mode=3):

@@ -44,6 +44,7 @@ struct CgFuse {
     const double *pap_part;
     int npap;
     double *r2_part;
+    bool x_in_dir = false;   // update r only; the direction kernel updates x
 };
 int hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gauge_s *U, int dagger, int par_out,
            double cR, double gR, double cX, double gX, int g5out, const CgFuse &f, int *nblocks_out);

@@ -582,12 +582,15 @@ __device__ __forceinline__ HopSite hop_site_walk(const Geom &g, const HopTrav &t
 // the last hop of the normal operator -- with alpha = r2 / p.Ap (p.Ap summed
 // from the partials of an earlier hop, in a fixed order), x += alpha p and
 // r -= alpha Ap at this site, per-workgroup |r|^2 partials; Ap itself is never
-// stored.  x, r, p share the output field's layout.
+// stored.  x, r, p share the output field's layout.  UPDX false: x is left to
+// the direction kernel (cg_dir_kernel), which reads p there anyway, and only
+// r is updated here.
 struct StoreFin {
     template <class A> __device__ __forceinline__ void begin(const A &) {}
 };
 
-template <class SX, class SR, class SP> struct CgFin {
+template <class SX, class SR, class SP, bool UPDX = true> struct CgFin {
+    static constexpr bool kUpdX = UPDX;
     SX x;
     SR r;
     SP p;
@@ -675,19 +678,22 @@ __device__ __forceinline__ void hop_finish(const HopArgs<SIn, SOut, SAux, G, NEP
         epilogue<B, T>(ep, R, nullptr, res);
     }
     if constexpr (CG) {
-        // x += alpha p, r -= alpha Ap (single right-hand side)
+        // x += alpha p (unless left to the direction kernel), r -= alpha Ap
+        // (single right-hand side)
         const int so = a.out_off[p] + cb;
         using TX = typename decltype(fin.x)::Real;
         using TR = typename decltype(fin.r)::Real;
         using TP = typename decltype(fin.p)::Real;
-        Cx<TX> xv[12];
-        Cx<TP> pv[12];
-        fin.x.load(so, xv);
-        fin.p.load(so, pv);
-        const TX ax = TX(fin.alpha);
+        if constexpr (Fin::kUpdX) {
+            Cx<TX> xv[12];
+            Cx<TP> pv[12];
+            fin.x.load(so, xv);
+            fin.p.load(so, pv);
+            const TX ax = TX(fin.alpha);
 #pragma unroll
-        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
-        fin.x.store(so, xv);
+            for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+            fin.x.store(so, xv);
+        }
         Cx<TR> rv[12];
         fin.r.load(so, rv);
         const TR ar = TR(fin.alpha);
@@ -1719,11 +1725,14 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     double bytes = (double)sites_out * (par_out < 0 ? 4 : 8) * wg + (double)nout * 2 * ws;
     if (aux && aux != in) bytes += (double)nout * ws;
     a.tr = hop_trav(a.g, par_out, bytes / sites_out);
-    // a fused CG update also reads x, r and p and writes x and r at each
-    // output (site, rhs)
-    if constexpr (!store)
-        bytes += (double)nout * (2 * site_bytes_of<decltype(fin.x)>() + 2 * site_bytes_of<decltype(fin.r)>() +
-                                 site_bytes_of<decltype(fin.p)>());
+    // a fused CG update also reads and writes r at each output (site, rhs),
+    // and reads p, reads and writes x unless x is left to the direction kernel
+    if constexpr (!store) {
+        constexpr bool updx = !std::is_same_v<Fin, CgFin<decltype(fin.x), decltype(fin.r), decltype(fin.p), false>>;
+        bytes += (double)nout * 2 * site_bytes_of<decltype(fin.r)>();
+        if constexpr (updx)
+            bytes += (double)nout * (2 * site_bytes_of<decltype(fin.x)>() + site_bytes_of<decltype(fin.p)>());
+    }
     // single-RHS FP16 half-lattice hops in checkerboard order: x and y
     // neighbours staged in LDS when switched on and the x row fits
     constexpr bool stage16 = std::is_same_v<S, SpinorH> && std::is_same_v<G, Gauge16<false>> && QKX_HOP16_DOT2;
@@ -2040,6 +2049,19 @@ int launch_hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gau
         return launch_hop_t<S, G, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, &e, nullptr, nblocks_out, fin);
     };
     const int xp = f.x->prec, rp = f.r->prec;
+    if (f.x_in_dir) {
+        // r only: x is updated by the direction kernel (FP16 p and r, FP32 x)
+        if constexpr (std::is_same_v<S, SpinorH>) {
+            if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_HALF) {
+                CgFin<SpinorF, SpinorH, S, false> fin{make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r),
+                                                      make_spinor<S>(f.p), f.r2cur, f.pap_part, f.npap, f.r2_part, 0.0};
+                return launch_hop_t<S, G, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, &e, nullptr,
+                                                           nblocks_out, fin);
+            }
+        }
+        set_error("hop_cg: x updated in the direction kernel needs FP16 p and r, FP32 x");
+        return QKX_ERR_ARG;
+    }
     if constexpr (std::is_same_v<S, SpinorD>) {
         if (xp == QKX_PREC_DOUBLE && rp == QKX_PREC_DOUBLE) return go(make_spinor<SpinorD>(f.x), make_spinor<SpinorD>(f.r));
     } else {

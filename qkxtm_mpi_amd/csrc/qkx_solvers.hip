@@ -45,7 +45,7 @@ struct CGSlot {
     int k;          // iterations done
 };
 
-int g_cg_fuse = 1;               // fuse the CG update into the operator's last hop
+int g_cg_fuse = 2;               // fuse the CG update into the operator's last hop (2: x in cg_dir for FP16)
 int g_cg_graph = 0;              // replay pairs of CG iterations from captured graphs: bit 0 single-RHS
                                  // solves, bit 1 batched solves (both off: measured no faster)
 CGSlot *g_slots = nullptr;       // device, two slots
@@ -183,8 +183,11 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(SX x, SR r, S p, S 
 // |r|^2, write the next slot (workgroup 0) and, when no flag is raised,
 // p = r + beta p.  mode 1: restart after a reliable update -- |r|^2 from the
 // partials, beta = |r|^2 / cur->r2_old, always update p, clear the flag.
-template <class S, class SR>
-__global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, int n, const CGSlot *cur, CGSlot *next,
+// HASX: in mode 0 also x += alpha p with the p of the iteration (before its
+// update), alpha = cur->r2 / p.Ap summed as the fused hop sums it, so x gets
+// the same bits as when the hop updates it.
+template <class S, class SR, class SX, bool HASX>
+__global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, SX x, int n, const CGSlot *cur, CGSlot *next,
                                                           const double *r2_part, int nr2, const double *pap_part,
                                                           int npap, double stop2, double delta2, int mode)
 {
@@ -192,21 +195,33 @@ __global__ __launch_bounds__(kThreads) void cg_dir_kernel(S p, SR r, int n, cons
         if (blockIdx.x == 0 && threadIdx.x == 0) *next = *cur;
         return;
     }
-    // this site's p and r are fetched before the partial sums are reduced,
-    // so their HBM latency overlaps the reduction
+    // this site's p and r (and x) are fetched before the partial sums are
+    // reduced, so their HBM latency overlaps the reduction
     using T = typename SR::Real;
+    using TX = typename SX::Real;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     Cx<typename S::Real> pv[12];
     Cx<T> rv[12];
+    Cx<TX> xv[12];
     if (s < n) {
         p.load(s, pv);
         r.load(s, rv);
+        if (HASX && mode == 0) x.load(s, xv);
     }
     const double r2n = block_sum_array(r2_part, nr2);
     double beta;
     int flag = 0;
     if (mode == 0) {
         const double pap = block_sum_array(pap_part, npap);
+        if constexpr (HASX) {
+            if (s < n) {
+                const TX ax = TX(cur->r2 / pap);
+#pragma unroll
+                for (int k = 0; k < 12; k++)
+                    xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+                x.store(s, xv);
+            }
+        }
         if (!(pap > 0.0) || !isfinite(r2n)) flag = 3;
         else if (r2n <= stop2) flag = 2;
         else if (r2n < delta2 * cur->maxr2) flag = 1;
@@ -355,22 +370,29 @@ int enqueue_blas(Engine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p, q
     const double *r2_src = r2_part;
     int nr2 = nblk(n);
     reduce_if_many(r2_src, nr2, g_red + 1);
-    hipLaunchKernelGGL((cg_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P, R, n, E.cur(),
-                       E.next(), r2_src, nr2, pap_part, npap, E.stop2, E.delta2, 0);
+    hipLaunchKernelGGL((cg_dir_kernel<S, SR, S, false>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P, R, P, n,
+                       E.cur(), E.next(), r2_src, nr2, pap_part, npap, E.stop2, E.delta2, 0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
 
-// direction step alone (the update was fused into the operator's last hop)
+// direction step alone (the update was fused into the operator's last hop;
+// with f.x_in_dir the x part of it is done here, on an FP32 x)
 template <class S, class SR>
 int enqueue_dir(Engine &E, qkx_spinor_s *r, qkx_spinor_s *p, const qkx::CgFuse &f, int nr2)
 {
     const int n = entries(p);
     const double *r2_src = f.r2_part;
     reduce_if_many(r2_src, nr2, g_red + 1);
-    hipLaunchKernelGGL((cg_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, make_spinor<S>(p),
-                       make_spinor<SR>(r), n, E.cur(), E.next(), r2_src, nr2, f.pap_part, f.npap, E.stop2, E.delta2,
-                       0);
+    const S P = make_spinor<S>(p);
+    if (f.x_in_dir)
+        hipLaunchKernelGGL((cg_dir_kernel<S, SR, SpinorF, true>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P,
+                           make_spinor<SR>(r), make_spinor<SpinorF>(f.x), n, E.cur(), E.next(), r2_src, nr2,
+                           f.pap_part, f.npap, E.stop2, E.delta2, 0);
+    else
+        hipLaunchKernelGGL((cg_dir_kernel<S, SR, S, false>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P,
+                           make_spinor<SR>(r), P, n, E.cur(), E.next(), r2_src, nr2, f.pap_part, f.npap, E.stop2,
+                           E.delta2, 0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -382,6 +404,9 @@ int enqueue_iteration(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_
     if (g_cg_fuse) {
         qkx::set_hop_skip(&E.cur()->flag);
         qkx::CgFuse f{x, r, p, &E.cur()->r2, nullptr, 0, g_part + g_part_cap};
+        // mode 2: FP16 p and r with an FP32 x -- x is updated by the direction
+        // kernel, which reads p anyway, so the fused hop does not read p
+        f.x_in_dir = g_cg_fuse == 2 && pp == QKX_PREC_HALF && pr == QKX_PREC_HALF && px == QKX_PREC_SINGLE;
         int nr2 = 0;
         int rc = A.apply_cg(p, g_part, f, &nr2);
         qkx::set_hop_skip(nullptr);
@@ -510,9 +535,9 @@ int restart(Engine &E, qkx_spinor_s *p, qkx_spinor_s *r, int mode)
     reduce_if_many(r2_src, nr2, g_red + 1);
     rc = with_spinor(p, [&](auto P) {
         return with_spinor(r, [&](auto R) {
-            hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0,
-                               qkx::g_stream, P, R, mode == 0 ? 0 : n, E.cur(), E.next(), r2_src, nr2, g_part, 0,
-                               0.0, 0.0, 1);
+            hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R), decltype(P), false>), dim3(nblk(n)),
+                               dim3(kThreads), 0, qkx::g_stream, P, R, P, mode == 0 ? 0 : n, E.cur(), E.next(),
+                               r2_src, nr2, g_part, 0, 0.0, 0.0, 1);
             QKX_HIP(hipGetLastError());
             return QKX_SUCCESS;
         });
@@ -854,9 +879,9 @@ int restart_inst(MInst &I, int i, int j, int mode)
     reduce_if_many(r2_src, nr2, g_ms.red + kMaxInst + i);
     return with_spinor(I.pp, [&](auto P) {
         return with_spinor(I.r, [&](auto R) {
-            hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kThreads), 0,
-                               qkx::g_stream, P, R, mode == 0 ? 0 : n, mslot(j, i), mslot(j + 1, i), r2_src, nr2,
-                               I.pap, 0, 0.0, 0.0, 1);
+            hipLaunchKernelGGL((cg_dir_kernel<decltype(P), decltype(R), decltype(P), false>), dim3(nblk(n)),
+                               dim3(kThreads), 0, qkx::g_stream, P, R, P, mode == 0 ? 0 : n, mslot(j, i),
+                               mslot(j + 1, i), r2_src, nr2, I.pap, 0, 0.0, 0.0, 1);
             QKX_HIP(hipGetLastError());
             return QKX_SUCCESS;
         });
@@ -891,7 +916,8 @@ extern "C" {
 
 int qkx_set_cg_fuse(int on)
 {
-    g_cg_fuse = on ? 1 : 0;
+    QKX_CHECK_ARG(on >= 0 && on <= 2, "qkx_set_cg_fuse: 0, 1 or 2");
+    g_cg_fuse = on;
     return QKX_SUCCESS;
 }
 

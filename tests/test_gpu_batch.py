@@ -288,7 +288,7 @@ def test_batched_fused_update_matches_unfused(q):
             sols.append(x.download())
             its.append(st[0].iterations)
     finally:
-        q.set_cg_fuse(1)
+        q.set_cg_fuse(2)
         q.set_batch_link_sharing(1)
     assert abs(its[0] - its[1]) <= 2
     for r in range(12):
