@@ -58,6 +58,7 @@ struct CgBatchFuse {
     const int *active;
     const double *pap;
     double *r2_part;
+    bool x_in_dir = false;   // update r only; the direction kernel updates x
 };
 // max over links of |row2 - conj(row0 x row1)| of an FP64 18-real field,
 // measured once per generation of its links

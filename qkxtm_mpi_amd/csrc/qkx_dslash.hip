@@ -624,8 +624,10 @@ template <class SX, class SR, class SP, bool UPDX = true> struct CgFin {
 // iteration: alpha_r = |r_r|^2 / (p.Ap)_r (0 for a frozen right-hand side:
 // not active, or |r|^2 at its stopping threshold), x += alpha p and
 // r -= alpha Ap at this (site, RHS), per-workgroup per-RHS |r|^2 partials;
-// Ap itself is never stored.  pap is one row of per-RHS sums.
-template <class SX, class SR, class SP> struct BCgFin {
+// Ap itself is never stored.  pap is one row of per-RHS sums.  UPDX false:
+// r only, x is left to the direction kernel (cgb_dir_kernel).
+template <class SX, class SR, class SP, bool UPDX = true> struct BCgFin {
+    static constexpr bool kUpdX = UPDX;
     SX x;
     SR r;
     SP p;
@@ -644,7 +646,11 @@ template <class S> constexpr double site_bytes_of()
 }
 
 template <class F> struct IsBCg : std::false_type {};
-template <class SX, class SR, class SP> struct IsBCg<BCgFin<SX, SR, SP>> : std::true_type {};
+template <class SX, class SR, class SP, bool U> struct IsBCg<BCgFin<SX, SR, SP, U>> : std::true_type {};
+// whether a fused CG update also updates x (and so reads p)
+template <class F> struct FinUpdX { static constexpr bool v = true; };
+template <class SX, class SR, class SP> struct FinUpdX<CgFin<SX, SR, SP, false>> { static constexpr bool v = false; };
+template <class SX, class SR, class SP> struct FinUpdX<BCgFin<SX, SR, SP, false>> { static constexpr bool v = false; };
 
 // Minimum waves per SIMD requested from the register allocator for the 16-bit
 // hops (1: no request).  At 24^3x48 a half-lattice launch is 1296 workgroups;
@@ -1136,18 +1142,21 @@ __global__ __launch_bounds__(256) void hop_share_kernel(HopArgs<SpinorH, SpinorH
             epilogue<B, T>(ep, R, nullptr, res);
         }
         if constexpr (CG) {
-            // x += alpha p, r -= alpha Ap for this (site, RHS)
+            // x += alpha p (unless left to the direction kernel), r -= alpha Ap
+            // for this (site, RHS)
             const int so = a.out_off[p] + cb * nr + rr;
             using TX = typename decltype(fin.x)::Real;
             using TR = typename decltype(fin.r)::Real;
-            Cx<TX> xv[12];
-            Cx<typename decltype(fin.p)::Real> pv[12];
-            fin.x.load(so, xv);
-            fin.p.load(so, pv);
-            const TX ax = TX(alpha_s[rr]);
+            if constexpr (Fin::kUpdX) {
+                Cx<TX> xv[12];
+                Cx<typename decltype(fin.p)::Real> pv[12];
+                fin.x.load(so, xv);
+                fin.p.load(so, pv);
+                const TX ax = TX(alpha_s[rr]);
 #pragma unroll
-            for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
-            fin.x.store(so, xv);
+                for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+                fin.x.store(so, xv);
+            }
             Cx<TR> rv[12];
             fin.r.load(so, rv);
             const TR ar = TR(alpha_s[rr]);
@@ -1728,9 +1737,8 @@ int launch_hop_t(qkx_spinor_s *out, const qkx_spinor_s *in, const qkx_spinor_s *
     // a fused CG update also reads and writes r at each output (site, rhs),
     // and reads p, reads and writes x unless x is left to the direction kernel
     if constexpr (!store) {
-        constexpr bool updx = !std::is_same_v<Fin, CgFin<decltype(fin.x), decltype(fin.r), decltype(fin.p), false>>;
         bytes += (double)nout * 2 * site_bytes_of<decltype(fin.r)>();
-        if constexpr (updx)
+        if constexpr (FinUpdX<Fin>::v)
             bytes += (double)nout * (2 * site_bytes_of<decltype(fin.x)>() + site_bytes_of<decltype(fin.p)>());
     }
     // single-RHS FP16 half-lattice hops in checkerboard order: x and y
@@ -2093,6 +2101,13 @@ int hop_cg_batch(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gaug
     }
     Epi e[QKX_MAX_RHS];
     for (int r = 0; r < in->nrhs; r++) e[r] = Epi{coef[r][0], coef[r][1], coef[r][2], coef[r][3], g5out};
+    if (f.x_in_dir) {
+        BCgFin<SpinorF, SpinorH, SpinorH, false> fin{make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r),
+                                                     make_spinor<SpinorH>(f.p), f.r2cur, f.stop2, f.active, f.pap,
+                                                     f.r2_part};
+        return launch_hop_t<SpinorH, GaugeH12, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, e, nullptr,
+                                                                nblocks_out, fin);
+    }
     BCgFin<SpinorF, SpinorH, SpinorH> fin{make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r),
                                           make_spinor<SpinorH>(f.p), f.r2cur, f.stop2, f.active, f.pap, f.r2_part};
     return launch_hop_t<SpinorH, GaugeH12, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, e, nullptr, nblocks_out,

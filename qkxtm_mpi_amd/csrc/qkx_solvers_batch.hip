@@ -141,9 +141,12 @@ __global__ __launch_bounds__(kT) void cgb_update_kernel(SX x, SR r, S p, S Ap, i
     block_partial_rhs(acc, valid, nr, r2_part);
 }
 
-// mode 0: iteration; mode 1: restart after a reliable update (beta from r2_old)
-template <class S, class SR>
-__global__ __launch_bounds__(kT) void cgb_dir_kernel(S p, SR r, int n, int nr, const BSlot *cur, BSlot *next,
+// mode 0: iteration; mode 1: restart after a reliable update (beta from r2_old).
+// HASX: in mode 0 also x += alpha p with the p of the iteration (before its
+// update), alpha as the fused hop forms it (0 for a frozen right-hand side),
+// so x gets the same bits as when the hop updates it.
+template <class S, class SR, class SX, bool HASX>
+__global__ __launch_bounds__(kT) void cgb_dir_kernel(S p, SR r, SX x, int n, int nr, const BSlot *cur, BSlot *next,
                                                      const double *r2_part, int nr2, const double *pap_part, int npap,
                                                      double delta2, int mode)
 {
@@ -174,6 +177,20 @@ __global__ __launch_bounds__(kT) void cgb_dir_kernel(S p, SR r, int n, int nr, c
     }
     __syncthreads();
     const int flag = flag_s;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rr = u % nr;
+    Cx<typename S::Real> pv[12];
+    const bool xupd = HASX && mode == 0 && u < n;
+    if (xupd) {
+        using TX = typename SX::Real;
+        const TX ax = TX(frozen(cur, rr) ? 0.0 : cur->r2[rr] / pap[rr]);
+        Cx<TX> xv[12];
+        x.load(u, xv);
+        p.load(u, pv);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {xv[k].re + ax * TX(pv[k].re), xv[k].im + ax * TX(pv[k].im)};
+        x.store(u, xv);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         BSlot nx = *cur;
         for (int q = 0; q < nr; q++) {
@@ -190,17 +207,14 @@ __global__ __launch_bounds__(kT) void cgb_dir_kernel(S p, SR r, int n, int nr, c
         *next = nx;
     }
     if (flag) return;
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
-    const int rr = u % nr;
     double beta;
     if (mode == 0) beta = frozen(cur, rr) ? 0.0 : r2n[rr] / cur->r2[rr];
     else beta = cur->r2_old[rr] > 0 ? r2n[rr] / cur->r2_old[rr] : 0.0;
     using T = typename SR::Real;
     const T b = T(beta);
-    Cx<typename S::Real> pv[12];
     Cx<T> rv[12];
-    p.load(u, pv);
+    if (!xupd) p.load(u, pv);
     r.load(u, rv);
     Cx<typename S::Real> out[12];
 #pragma unroll
@@ -461,8 +475,8 @@ int enqueue_bblas(BEngine &E, qkx_spinor_s *x, qkx_spinor_s *r, qkx_spinor_s *p,
     const double *r2_src = r2_part;
     int nr2 = nblk(n);
     reduce_if_many(r2_src, nr2, nr, g_bred + NR);
-    hipLaunchKernelGGL((cgb_dir_kernel<S, SR>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, P, R, n, nr, E.cur(),
-                       E.next(), r2_src, nr2, pap_part, npap, E.delta2, 0);
+    hipLaunchKernelGGL((cgb_dir_kernel<S, SR, S, false>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream, P, R, P, n, nr,
+                       E.cur(), E.next(), r2_src, nr2, pap_part, npap, E.delta2, 0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -483,16 +497,24 @@ int enqueue_biter_fused(BEngine &E, const BatchOp &A, qkx_spinor_s *x, qkx_spino
     sum_rows(g_bpart, npap, nr, g_bred);
     double *r2_part = g_bpart + g_bpart_cap;
     BSlot *cur = E.cur();
-    const qkx::CgBatchFuse f{x, r, p, cur->r2, cur->stop2, cur->active, g_bred, r2_part};
+    qkx::CgBatchFuse f{x, r, p, cur->r2, cur->stop2, cur->active, g_bred, r2_part};
+    // qkx_set_cg_fuse(2): x += alpha p in the direction kernel, which reads p anyway
+    f.x_in_dir = qkx::cg_fuse_on() == 2;
     int nr2 = 0;
     rc = A.schur_cg(A.tmp_even, -1, 1, f, &nr2);
     qkx::set_hop_skip(nullptr);
     if (rc) return rc;
     const double *r2_src = r2_part;
     reduce_if_many(r2_src, nr2, nr, g_bred + NR);
-    hipLaunchKernelGGL((cgb_dir_kernel<SpinorH, SpinorH>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream,
-                       make_spinor<SpinorH>(p), make_spinor<SpinorH>(r), n, nr, E.cur(), E.next(), r2_src, nr2,
-                       (const double *)g_bred, 1, E.delta2, 0);
+    const SpinorH P = make_spinor<SpinorH>(p), R = make_spinor<SpinorH>(r);
+    if (f.x_in_dir)
+        hipLaunchKernelGGL((cgb_dir_kernel<SpinorH, SpinorH, SpinorF, true>), dim3(nblk(n)), dim3(kT), 0,
+                           qkx::g_stream, P, R, make_spinor<SpinorF>(x), n, nr, E.cur(), E.next(), r2_src, nr2,
+                           (const double *)g_bred, 1, E.delta2, 0);
+    else
+        hipLaunchKernelGGL((cgb_dir_kernel<SpinorH, SpinorH, SpinorH, false>), dim3(nblk(n)), dim3(kT), 0,
+                           qkx::g_stream, P, R, P, n, nr, E.cur(), E.next(), r2_src, nr2, (const double *)g_bred, 1,
+                           E.delta2, 0);
     QKX_HIP(hipGetLastError());
     E.j++;
     return QKX_SUCCESS;
@@ -662,8 +684,9 @@ int brestart(BEngine &E, qkx_spinor_s *p, qkx_spinor_s *r, int initial)
     reduce_if_many(r2_src, nr2, nr, g_bred + NR);
     rc = with_spinor(p, [&](auto P) {
         return with_spinor(r, [&](auto R) {
-            hipLaunchKernelGGL((cgb_dir_kernel<decltype(P), decltype(R)>), dim3(nblk(n)), dim3(kT), 0, qkx::g_stream,
-                               P, R, initial ? 0 : n, nr, E.cur(), E.next(), r2_src, nr2, g_bpart, 0, 0.0, 1);
+            hipLaunchKernelGGL((cgb_dir_kernel<decltype(P), decltype(R), decltype(P), false>), dim3(nblk(n)),
+                               dim3(kT), 0, qkx::g_stream, P, R, P, initial ? 0 : n, nr, E.cur(), E.next(), r2_src,
+                               nr2, g_bpart, 0, 0.0, 1);
             QKX_HIP(hipGetLastError());
             return QKX_SUCCESS;
         });
