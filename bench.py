@@ -78,7 +78,7 @@ def parse():
                     help="hop autotuner: 0 fixed launch shapes, 1 batched launches (default), 2 every launch")
     ap.add_argument("--cg-fuse", type=int, default=2, choices=[0, 1, 2],
                     help="CG update: 1 fused into the last hop, 0 separate kernel, 2 fused with x += alpha p in the "
-                         "direction kernel for FP16 solves (default)")
+                         "direction kernel where x is FP32 (default)")
     ap.add_argument("--staging", type=int, default=0, choices=[0, 1, 2],
                     help="x and y neighbours of the single-RHS FP16 half-lattice hop staged in LDS: 0 per-thread "
                          "gathers (default), 1 per workgroup, 2 per wave")

@@ -167,9 +167,10 @@ int qkx_autotune_report(char *buf, int len, int *needed);
 int  qkx_set_cg_batch(int iterations);
 /* 1: the CG update x += alpha p, r -= alpha Ap is fused into the last hop
  * of the normal operator (no Ap field, one launch less per iteration);
- * 2 (default): as 1, but where p and r are FP16 and x FP32 (the FP16 solves)
- * x += alpha p moves into the direction kernel, which reads p anyway (the
- * same bits, one read of p less per iteration); 0: separate update kernel. */
+ * 2 (default): as 1, but where x is FP32 (the inner CG of the mixed-precision
+ * solves, single and batched) x += alpha p moves into the direction kernel,
+ * which reads p anyway (the same bits, one read of p less per iteration);
+ * 0: separate update kernel. */
 int  qkx_set_cg_fuse(int mode);
 /* CG iterations replayed two at a time from captured HIP graphs (one launch
  * per pair) instead of every kernel launched on its own; the results are

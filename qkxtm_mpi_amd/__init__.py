@@ -716,7 +716,7 @@ def set_cg_batch(iterations=16):
 def set_cg_fuse(mode=2):
     """CG update fused into the normal operator's last hop (1 or True), in a
     separate kernel (0 or False), or fused with x += alpha p moved into the
-    direction kernel for FP16 solves (2, the default)."""
+    direction kernel where x is FP32 (2, the default)."""
     _check(lib().qkx_set_cg_fuse(int(mode)), "qkx_set_cg_fuse")
 
 

@@ -2058,16 +2058,20 @@ int launch_hop_cg(const qkx_spinor_s *in, const qkx_spinor_s *aux, const qkx_gau
     };
     const int xp = f.x->prec, rp = f.r->prec;
     if (f.x_in_dir) {
-        // r only: x is updated by the direction kernel (FP16 p and r, FP32 x)
-        if constexpr (std::is_same_v<S, SpinorH>) {
-            if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_HALF) {
-                CgFin<SpinorF, SpinorH, S, false> fin{make_spinor<SpinorF>(f.x), make_spinor<SpinorH>(f.r),
-                                                      make_spinor<S>(f.p), f.r2cur, f.pap_part, f.npap, f.r2_part, 0.0};
-                return launch_hop_t<S, G, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, &e, nullptr,
-                                                           nblocks_out, fin);
+        // r only: x (FP32) is updated by the direction kernel
+        auto go_r = [&](auto R) {
+            CgFin<SpinorF, decltype(R), S, false> fin{make_spinor<SpinorF>(f.x), R, make_spinor<S>(f.p), f.r2cur,
+                                                      f.pap_part, f.npap, f.r2_part, 0.0};
+            return launch_hop_t<S, G, QKX_BASIS_UKQCD>(f.p, in, aux, U, dagger, par_out, &e, nullptr, nblocks_out,
+                                                       fin);
+        };
+        if constexpr (!std::is_same_v<S, SpinorD>) {
+            if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_SINGLE) return go_r(make_spinor<SpinorF>(f.r));
+            if constexpr (std::is_same_v<S, SpinorH>) {
+                if (xp == QKX_PREC_SINGLE && rp == QKX_PREC_HALF) return go_r(make_spinor<SpinorH>(f.r));
             }
         }
-        set_error("hop_cg: x updated in the direction kernel needs FP16 p and r, FP32 x");
+        set_error("hop_cg: x updated in the direction kernel needs an FP32 x and a 32- or 16-bit p");
         return QKX_ERR_ARG;
     }
     if constexpr (std::is_same_v<S, SpinorD>) {

@@ -385,14 +385,18 @@ int enqueue_dir(Engine &E, qkx_spinor_s *r, qkx_spinor_s *p, const qkx::CgFuse &
     const double *r2_src = f.r2_part;
     reduce_if_many(r2_src, nr2, g_red + 1);
     const S P = make_spinor<S>(p);
-    if (f.x_in_dir)
-        hipLaunchKernelGGL((cg_dir_kernel<S, SR, SpinorF, true>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P,
-                           make_spinor<SR>(r), make_spinor<SpinorF>(f.x), n, E.cur(), E.next(), r2_src, nr2,
-                           f.pap_part, f.npap, E.stop2, E.delta2, 0);
-    else
-        hipLaunchKernelGGL((cg_dir_kernel<S, SR, S, false>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P,
-                           make_spinor<SR>(r), P, n, E.cur(), E.next(), r2_src, nr2, f.pap_part, f.npap, E.stop2,
-                           E.delta2, 0);
+    if constexpr (!std::is_same_v<S, SpinorD>) {
+        if (f.x_in_dir) {
+            hipLaunchKernelGGL((cg_dir_kernel<S, SR, SpinorF, true>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream,
+                               P, make_spinor<SR>(r), make_spinor<SpinorF>(f.x), n, E.cur(), E.next(), r2_src, nr2,
+                               f.pap_part, f.npap, E.stop2, E.delta2, 0);
+            QKX_HIP(hipGetLastError());
+            return QKX_SUCCESS;
+        }
+    }
+    hipLaunchKernelGGL((cg_dir_kernel<S, SR, S, false>), dim3(nblk(n)), dim3(kThreads), 0, qkx::g_stream, P,
+                       make_spinor<SR>(r), P, n, E.cur(), E.next(), r2_src, nr2, f.pap_part, f.npap, E.stop2, E.delta2,
+                       0);
     QKX_HIP(hipGetLastError());
     return QKX_SUCCESS;
 }
@@ -404,9 +408,10 @@ int enqueue_iteration(Engine &E, const NormalOp &A, qkx_spinor_s *x, qkx_spinor_
     if (g_cg_fuse) {
         qkx::set_hop_skip(&E.cur()->flag);
         qkx::CgFuse f{x, r, p, &E.cur()->r2, nullptr, 0, g_part + g_part_cap};
-        // mode 2: FP16 p and r with an FP32 x -- x is updated by the direction
-        // kernel, which reads p anyway, so the fused hop does not read p
-        f.x_in_dir = g_cg_fuse == 2 && pp == QKX_PREC_HALF && pr == QKX_PREC_HALF && px == QKX_PREC_SINGLE;
+        // mode 2: an FP32 x with a 32- or 16-bit p (the mixed solves' inner CG)
+        // is updated by the direction kernel, which reads p anyway, so the
+        // fused hop does not read p
+        f.x_in_dir = g_cg_fuse == 2 && px == QKX_PREC_SINGLE && pp != QKX_PREC_DOUBLE;
         int nr2 = 0;
         int rc = A.apply_cg(p, g_part, f, &nr2);
         qkx::set_hop_skip(nullptr);

@@ -120,11 +120,13 @@ def test_half_inner_point_source(q):
     assert st.converged and res <= 1e-10
 
 
-@pytest.mark.parametrize("prec", ["DOUBLE", "HALF"])
+@pytest.mark.parametrize("prec", ["DOUBLE", "SINGLE", "HALF", "BF16"])
 def test_solve_with_separate_update_kernel(q, cfg1, prec):
     """The unfused CG (separate update kernel and Ap field) reaches the same
-    solution as the default fused one; moving x += alpha p from the fused hop
-    into the direction kernel (mode 2) gives the same bits and iterations."""
+    solution as the default fused one (checked at FP64 and FP16; at FP32 and
+    BF16 it stores Ap at p's precision and only converges alike); moving
+    x += alpha p from the fused hop into the direction kernel (mode 2) gives
+    the same bits and iterations at every inner precision."""
     U, Ub, src = cfg1
     u = q.Gauge(L1, boundaryT=-1, host=U)
     p = q.params(L1, K1, M1, 1, -1)
@@ -142,7 +144,8 @@ def test_solve_with_separate_update_kernel(q, cfg1, prec):
             assert res <= 1e-10
     finally:
         q.set_cg_fuse(2)
-    assert rel(out[0], out[1]) <= 1e-9
+    if prec in ("DOUBLE", "HALF"):
+        assert rel(out[0], out[1]) <= 1e-9
     assert its[2] == its[0] and np.array_equal(out[2], out[0])
     x = q.Spinor(L1)
     try:
