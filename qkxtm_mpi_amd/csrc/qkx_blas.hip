@@ -60,6 +60,35 @@ __global__ __launch_bounds__(kRedThreads) void redot_kernel(S x, S y, int n, int
     block_partial(acc, partial + blockIdx.x);
 }
 
+// The true-residual step of a reliable update in one pass: s = b - s (as
+// axpby_kernel forms it with a = -1, b = 1), per-workgroup |s|^2 partials in
+// redot_kernel's grid and order (so finish_kernel gives the bits qkx_norm2
+// gives), and, with H, s rounded into the 16-bit field h as spinor_convert
+// rounds it.
+template <class S, class SH, bool H>
+__global__ __launch_bounds__(kRedThreads) void resid_norm_kernel(S s_, S b_, SH h, int n, double *partial)
+{
+    using T = typename S::Real;
+    double acc = 0.0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        Cx<T> a[12], b[12];
+        s_.load(s, a);
+        b_.load(s, b);
+#pragma unroll
+        for (int k = 0; k < 12; k++) a[k] = {T(-1.0) * a[k].re + T(1.0) * b[k].re, T(-1.0) * a[k].im + T(1.0) * b[k].im};
+        s_.store(s, a);
+        if constexpr (H) {
+            Cx<typename SH::Real> w[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) w[k] = {(typename SH::Real)a[k].re, (typename SH::Real)a[k].im};
+            h.store(s, w);
+        }
+#pragma unroll
+        for (int k = 0; k < 12; k++) acc += (double)a[k].re * (double)a[k].re + (double)a[k].im * (double)a[k].im;
+    }
+    block_partial(acc, partial + blockIdx.x);
+}
+
 // One workgroup: result = compensated (double-double) sum of partial[0..n).
 __global__ __launch_bounds__(kRedThreads) void finish_kernel(const double *partial, int n, double *result)
 {
@@ -137,6 +166,31 @@ int finish_sum_dev(const double *partial, int n, double *result)
 {
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, g_stream, partial, n, result);
     QKX_HIP(hipGetLastError());
+    return QKX_SUCCESS;
+}
+
+int resid_norm(qkx_spinor_s *s, const qkx_spinor_s *b, qkx_spinor_s *h, double *s2)
+{
+    int rc = check_pair(s, b, "resid_norm");
+    if (rc) return rc;
+    if (s->prec != QKX_PREC_DOUBLE || (h && (h->prec != QKX_PREC_HALF || h->nsites != s->nsites || h->nrhs != s->nrhs ||
+                                             h->subset != s->subset))) {
+        qkx::set_error("resid_norm: FP64 s and b, and an FP16 h of the same sites");
+        return QKX_ERR_ARG;
+    }
+    const int n = entries(s);
+    const int nb = std::min(kRedBlocks, (n + kRedThreads - 1) / kRedThreads);
+    const SpinorD S = make_spinor<SpinorD>(s), Bv = make_spinor<SpinorD>(b);
+    if (h)
+        hipLaunchKernelGGL((resid_norm_kernel<SpinorD, SpinorH, true>), dim3(nb), dim3(kRedThreads), 0, g_stream, S,
+                           Bv, make_spinor<SpinorH>(h), n, reduce_scratch());
+    else
+        hipLaunchKernelGGL((resid_norm_kernel<SpinorD, SpinorD, false>), dim3(nb), dim3(kRedThreads), 0, g_stream, S,
+                           Bv, S, n, reduce_scratch());
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kRedThreads), 0, g_stream, reduce_scratch(), nb, reduce_result());
+    QKX_HIP(hipGetLastError());
+    QKX_HIP(hipMemcpyAsync(s2, reduce_result(), sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    QKX_HIP(hipStreamSynchronize(g_stream));
     return QKX_SUCCESS;
 }
 

@@ -118,6 +118,10 @@ int redot_dev(const qkx_spinor_s *x, const qkx_spinor_s *y, double *result);
 int finish_sum_dev(const double *partial, int n, double *result);
 // x = a x + b y; sc_mode 1: b = sc[0], 2: a = sc[0], 3: b = -sc[0]
 int axpby_dev(qkx_spinor_s *x, const qkx_spinor_s *y, double a, double b, const double *sc, int sc_mode);
+// s = b - s and *s2 = |s|^2 (host value, the bits of qkx_norm2) in one pass
+// over FP64 fields; with h (FP16, same sites) also h = s rounded as a copy
+// rounds it (qkx_blas.hip)
+int resid_norm(qkx_spinor_s *s, const qkx_spinor_s *b, qkx_spinor_s *h, double *s2);
 
 // instrumentation on (qkx_core.cpp)
 bool prof_active();

@@ -1233,10 +1233,14 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         if (mixed) {
             if ((rc = launch_accum(xs.f, xl.f)) || (rc = qkx_spinor_zero(xl.f))) return rc;
         }
-        // s = bh - Op x ; true relative residual |s|/|b|
+        // s = bh - Op x ; true relative residual |s|/|b|.  For an FP16 r the
+        // same pass also rounds s into Ap, where the reliable update takes it
+        // from (Ap is free between iterations).
         if ((rc = Ah.apply_op(s.f, xs.f))) return rc;
-        if ((rc = qkx::axpby_dev(s.f, bh.f, -1.0, 1.0, nullptr, 0))) return rc;
-        double s2 = host_norm2(s.f);
+        const bool s_to_ap = mixed && r.f->prec == QKX_PREC_HALF && Ap.f->prec == QKX_PREC_HALF &&
+                             Ap.f->nsites == s.f->nsites && Ap.f->subset == s.f->subset;
+        double s2 = 0.0;
+        if ((rc = qkx::resid_norm(s.f, bh.f, s_to_ap ? Ap.f : nullptr, &s2))) return rc;
         if (!std::isfinite(s2)) { status = QKX_ERR_NONFINITE; qkx::set_error("qkx_solve: non-finite residual"); break; }
         if (s2 <= tol2 * b2) { status = QKX_SUCCESS; break; }
         if (h.flag == 2) {
@@ -1251,7 +1255,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
             // storage, and it is cheaper than the FP32 operator.
             nrel++;
             if (r.f->prec == QKX_PREC_HALF) {
-                if ((rc = qkx_spinor_copy(Ap.f, s.f)) || (rc = Al.apply_dag(r.f, Ap.f))) return rc;
+                if ((!s_to_ap && (rc = qkx_spinor_copy(Ap.f, s.f))) || (rc = Al.apply_dag(r.f, Ap.f))) return rc;
             } else if ((rc = qkx_spinor_copy(sm->f, s.f))) {
                 return rc;
             } else if (r.f->prec == QKX_PREC_SINGLE) {
