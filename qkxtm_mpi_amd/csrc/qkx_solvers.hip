@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void norm_partial_kernel(S r, int n, doub
     block_partial(acc, part + blockIdx.x);
 }
 
-// xh += xl (FP64 += lower precision)
+// xh += xl (FP64 += lower precision), then xl = 0
 template <class SL>
 __global__ __launch_bounds__(256) void accum_kernel(SpinorD xh, SL xl, int n)
 {
@@ -278,6 +278,10 @@ __global__ __launch_bounds__(256) void accum_kernel(SpinorD xh, SL xl, int n)
 #pragma unroll
     for (int k = 0; k < 12; k++) a[k] = {a[k].re + (double)b[k].re, a[k].im + (double)b[k].im};
     xh.store(s, a);
+    // xl restarts from zero (stored as a zero field stores it)
+#pragma unroll
+    for (int k = 0; k < 12; k++) b[k] = {0, 0};
+    xl.store(s, b);
 }
 
 int nblk(int n) { return (n + kThreads - 1) / kThreads; }
@@ -567,7 +571,8 @@ int write_slot(const Engine &E, const CGSlot &h)
     return QKX_SUCCESS;
 }
 
-int launch_accum(qkx_spinor_s *xh, const qkx_spinor_s *xl)
+// xh += xl, then xl = 0, in one pass
+int launch_accum(qkx_spinor_s *xh, qkx_spinor_s *xl)
 {
     int n = entries(xh);
     return with_spinor(xl, [&](auto S) {
@@ -1231,7 +1236,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         }
         // flag 1 (reliable update due) or 2 (iterated residual small): check in FP64
         if (mixed) {
-            if ((rc = launch_accum(xs.f, xl.f)) || (rc = qkx_spinor_zero(xl.f))) return rc;
+            if ((rc = launch_accum(xs.f, xl.f))) return rc;
         }
         // s = bh - Op x ; true relative residual |s|/|b|.  For an FP16 r the
         // same pass also rounds s into Ap, where the reliable update takes it
@@ -1271,7 +1276,7 @@ int qkx_solve(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge *U, const qkx_
         if ((rc = read_slot(E, h))) return rc;
     }
     if (status == QKX_NOT_CONVERGED) {
-        if (mixed) { launch_accum(xs.f, xl.f); qkx_spinor_zero(xl.f); }
+        if (mixed) launch_accum(xs.f, xl.f);
         qkx::set_error("qkx_solve: maxIter reached");
     }
 
@@ -1441,7 +1446,7 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
             }
             // reliable update or iterated stop: the true residual in FP64
             if (mixed) {
-                if ((rc = launch_accum(M.xs, M.xl)) || (rc = qkx_spinor_zero(M.xl))) return rc;
+                if ((rc = launch_accum(M.xs, M.xl))) return rc;
             }
             if ((rc = M.Ah.apply_op(M.s, M.xs))) return rc;
             if ((rc = qkx::axpby_dev(M.s, M.bh, -1.0, 1.0, nullptr, 0))) return rc;
@@ -1489,7 +1494,7 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
     for (int i = 0; i < n; i++) {
         MInst &M = I[i];
         if (M.state == 0 && mixed) {   // maxIter: keep the increments accumulated so far
-            if ((rc = launch_accum(M.xs, M.xl)) || (rc = qkx_spinor_zero(M.xl))) return rc;
+            if ((rc = launch_accum(M.xs, M.xl))) return rc;
         }
         // x_o = A^-1 (b_o + kappa H_oe x_e)
         if ((rc = qkx::hop(M.to, M.xs, M.bo, M.Uop, QKX_BASIS_UKQCD, 0, 1, M.kappa / M.nrm, -M.a * M.kappa / M.nrm,
