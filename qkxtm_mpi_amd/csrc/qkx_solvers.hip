@@ -1449,8 +1449,11 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
                 if ((rc = launch_accum(M.xs, M.xl))) return rc;
             }
             if ((rc = M.Ah.apply_op(M.s, M.xs))) return rc;
-            if ((rc = qkx::axpby_dev(M.s, M.bh, -1.0, 1.0, nullptr, 0))) return rc;
-            const double s2 = host_norm2(M.s);
+            // s = bh - Op x and |s|^2 in one pass, s also rounded into an FP16 Ap
+            const bool s_to_ap = mixed && M.r->prec == QKX_PREC_HALF && M.Ap->prec == QKX_PREC_HALF &&
+                                 M.Ap->nsites == M.s->nsites && M.Ap->subset == M.s->subset;
+            double s2 = 0.0;
+            if ((rc = qkx::resid_norm(M.s, M.bh, s_to_ap ? M.Ap : nullptr, &s2))) return rc;
             if (!std::isfinite(s2)) {
                 M.state = 2;
                 h[i].flag = kDone;
@@ -1466,7 +1469,7 @@ int qkx_solve_multi(int n, qkx_spinor *const *x, const qkx_spinor *const *b, con
                 // as in qkx_solve: an FP16 r from the FP16 operator on s rounded to FP16
                 M.nrel++;
                 if (M.r->prec == QKX_PREC_HALF) {
-                    if ((rc = qkx_spinor_copy(M.Ap, M.s)) || (rc = M.Al.apply_dag(M.r, M.Ap))) return rc;
+                    if ((!s_to_ap && (rc = qkx_spinor_copy(M.Ap, M.s))) || (rc = M.Al.apply_dag(M.r, M.Ap))) return rc;
                 } else if ((rc = qkx_spinor_copy(M.sm, M.s))) {
                     return rc;
                 } else if (M.r->prec == QKX_PREC_SINGLE) {
