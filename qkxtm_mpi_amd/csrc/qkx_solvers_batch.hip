@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kT) void sum_rhs_chunks_kernel(const double *part, 
 }
 
 template <class SL>
-__global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
+__global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n, int zero)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -299,6 +299,41 @@ __global__ __launch_bounds__(256) void accumb_kernel(SpinorD xh, SL xl, int n)
 #pragma unroll
     for (int k = 0; k < 12; k++) a[k] = {a[k].re + (double)b[k].re, a[k].im + (double)b[k].im};
     xh.store(s, a);
+    if (zero) {
+        // xl restarts from zero (stored as a zero field stores it)
+#pragma unroll
+        for (int k = 0; k < 12; k++) b[k] = {0, 0};
+        xl.store(s, b);
+    }
+}
+
+// The true-residual step of a batched reliable update in one pass: s = b - s
+// (as axpby_kernel forms it with a = -1, b = 1), per-workgroup per-RHS |s|^2
+// partials in normb_partial_kernel's grid and order, and, with H, s rounded
+// into the 16-bit field h as spinor_convert rounds it.
+template <class SH, bool H>
+__global__ __launch_bounds__(kT) void residb_kernel(SpinorD s_, SpinorD b_, SH h, int n, int nr, double *part)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0;
+    const bool valid = u < n;
+    if (valid) {
+        Cx<double> a[12], b[12];
+        s_.load(u, a);
+        b_.load(u, b);
+#pragma unroll
+        for (int k = 0; k < 12; k++) a[k] = {-1.0 * a[k].re + 1.0 * b[k].re, -1.0 * a[k].im + 1.0 * b[k].im};
+        s_.store(u, a);
+        if constexpr (H) {
+            Cx<typename SH::Real> w[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) w[k] = {(typename SH::Real)a[k].re, (typename SH::Real)a[k].im};
+            h.store(u, w);
+        }
+#pragma unroll
+        for (int k = 0; k < 12; k++) acc += a[k].re * a[k].re + a[k].im * a[k].im;
+    }
+    block_partial_rhs(acc, valid, nr, part);
 }
 
 int nblk(int n) { return (n + kT - 1) / kT; }
@@ -337,6 +372,28 @@ template <class F> int with_spinor(const qkx_spinor_s *f, F fn)
 }
 
 // per-rhs squared norms of a batched field, to the host
+// s = b - s and out[r] = |s_r|^2 (the bits norms() gives) in one pass over
+// FP64 fields; with h (FP16, same sites and RHS) also h = s rounded
+int resid_norms(qkx_spinor_s *s, const qkx_spinor_s *b, qkx_spinor_s *h, std::vector<double> &out)
+{
+    const int n = entries(s), nr = s->nrhs, nb = nblk(n);
+    if (h)
+        hipLaunchKernelGGL((residb_kernel<SpinorH, true>), dim3(nb), dim3(kT), 0, qkx::g_stream,
+                           make_spinor<SpinorD>(s), make_spinor<SpinorD>(b), make_spinor<SpinorH>(h), n, nr,
+                           g_bpart + g_bpart_cap);
+    else
+        hipLaunchKernelGGL((residb_kernel<SpinorD, false>), dim3(nb), dim3(kT), 0, qkx::g_stream,
+                           make_spinor<SpinorD>(s), make_spinor<SpinorD>(b), make_spinor<SpinorD>(s), n, nr,
+                           g_bpart + g_bpart_cap);
+    QKX_HIP(hipGetLastError());
+    sum_rows(g_bpart + g_bpart_cap, nb, nr, g_bres);
+    QKX_HIP(hipGetLastError());
+    QKX_HIP(hipMemcpyAsync(g_bres_host, g_bres, nr * sizeof(double), hipMemcpyDeviceToHost, qkx::g_stream));
+    QKX_HIP(hipStreamSynchronize(qkx::g_stream));
+    out.assign(g_bres_host, g_bres_host + nr);
+    return QKX_SUCCESS;
+}
+
 int norms(const qkx_spinor_s *f, std::vector<double> &out)
 {
     const int n = entries(f), nr = f->nrhs, nb = nblk(n);
@@ -831,11 +888,15 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
     // r_out = Op^dag s in FP32 (mixed) or FP64, s an FP64 field; r_out any
     // precision.  An FP16 r_out is formed by the FP16 operator from s rounded
     // to FP16 (Ap is free between iterations): the rounding of its storage.
+    // ap_has_s: Ap already holds s rounded to FP16 (left by the residual check)
+    bool ap_has_s = false;
     auto dag_residual0 = [&](qkx_spinor_s *r_out, qkx_spinor_s *src64) -> int {
         if (!mixed) return Ah.apply_dag(r_out, src64);
         int e;
         if (r_out->prec == QKX_PREC_HALF) {
-            if ((e = qkx_spinor_copy(Ap, src64))) return e;
+            const bool have = ap_has_s && src64 == s;
+            ap_has_s = false;
+            if (!have && (e = qkx_spinor_copy(Ap, src64))) return e;
             return Al.apply_dag(r_out, Ap);
         }
         e = qkx_spinor_copy(sm, src64);
@@ -921,17 +982,21 @@ static int solve_batch_impl(qkx_spinor *x, const qkx_spinor *b, const qkx_gauge 
                 if ((rc = qkx::net_apply_eo(np->net, np->t32, xl, np->Un, 0))) return rc;
                 xacc = np->t32;
             }
+            // xl is reset in the same pass when it is what was accumulated
             rc = with_spinor(xacc, [&](auto S) {
                 hipLaunchKernelGGL((accumb_kernel<decltype(S)>), dim3((n + 255) / 256), dim3(256), 0, qkx::g_stream,
-                                   make_spinor<SpinorD>(xs), S, n);
+                                   make_spinor<SpinorD>(xs), S, n, xacc == xl ? 1 : 0);
                 QKX_HIP(hipGetLastError());
                 return QKX_SUCCESS;
             });
-            if (rc || (rc = qkx_spinor_zero(xl))) return rc;
+            if (rc || (xacc != xl && (rc = qkx_spinor_zero(xl)))) return rc;
         }
         if ((rc = Ah.apply_op(s, xs))) return rc;
-        if ((rc = qkx::axpby_dev(s, bh, -1.0, 1.0, nullptr, 0))) return rc;
-        if ((rc = norms(s, s2))) return rc;
+        // s = bh - Op x and its per-RHS norms in one pass; for an FP16 r, s is
+        // also rounded into Ap, where the reliable update takes it from
+        ap_has_s = mixed && r->prec == QKX_PREC_HALF && Ap->prec == QKX_PREC_HALF && Ap->nsites == s->nsites &&
+                   Ap->nrhs == s->nrhs && Ap->subset == s->subset;
+        if ((rc = resid_norms(s, bh, ap_has_s ? Ap : nullptr, s2))) return rc;
         bool nonfinite = false;
         for (int q = 0; q < nr; q++) {
             if (!std::isfinite(s2[q])) nonfinite = true;
